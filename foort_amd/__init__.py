@@ -1,0 +1,157 @@
+"""foort_amd -- MI355X (gfx950) per-pixel null-geodesic integrator with the
+plugin surface of FOORT (Metric / Diagnostic / Termination / Mesh /
+ViewScreen).  This module is the ctypes binding of the C ABI in
+include/foort_gpu.h; the integration itself runs in libfoort_gpu.so.
+
+There is no CPU fallback: if the library is missing, or no GPU is visible,
+the calls raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+from .abi import RunDesc, GeodesicResults, Stats, make_desc, default_desc  # noqa: F401
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfoort_gpu.so")
+_lib = None
+
+EXPORTS = [
+    "foort_desc_init", "foort_desc_size", "foort_gpu_open", "foort_gpu_close", "foort_gpu_last_error",
+    "foort_gpu_trace_grid", "foort_gpu_trace_pixels", "foort_gpu_trace_device", "foort_gpu_camera",
+]
+
+
+class FoortError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libfoort_gpu.so (built by foort_amd.build / __graft_entry__.build)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FoortError(f"{LIB_PATH} is not built: run python -c 'import __graft_entry__ as g; g.build()'")
+        L = C.CDLL(LIB_PATH)
+        L.foort_desc_size.restype = C.c_int
+        L.foort_gpu_last_error.restype = C.c_char_p
+        L.foort_gpu_last_error.argtypes = [C.c_void_p]
+        L.foort_gpu_open.argtypes = [C.POINTER(RunDesc), C.POINTER(C.c_void_p)]
+        L.foort_gpu_close.argtypes = [C.c_void_p]
+        L.foort_gpu_trace_grid.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64,
+                                           C.POINTER(GeodesicResults), C.POINTER(Stats)]
+        L.foort_gpu_trace_pixels.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p,
+                                             C.POINTER(GeodesicResults), C.POINTER(Stats)]
+        L.foort_gpu_trace_device.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_void_p,
+                                             C.POINTER(GeodesicResults), C.c_void_p, C.POINTER(Stats)]
+        L.foort_gpu_camera.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        if L.foort_desc_size() != C.sizeof(RunDesc):
+            raise FoortError(f"foort_run_desc layout mismatch: library {L.foort_desc_size()} bytes, "
+                             f"binding {C.sizeof(RunDesc)}")
+        _lib = L
+    return _lib
+
+
+def _host_results(count):
+    return {
+        "term": np.zeros(count, np.int32), "quadrant": np.zeros(count, np.int32),
+        "equat_passes": np.zeros(count, np.int32), "closest_radius": np.zeros(count, np.float64),
+        "final_pos": np.zeros((count, 4), np.float64), "final_vel": np.zeros((count, 4), np.float64),
+        "lambda": np.zeros(count, np.float64), "steps": np.zeros(count, np.uint32),
+    }
+
+
+def _results_struct(arrs, ptr):
+    return GeodesicResults(*[ptr(arrs[k]) if arrs.get(k) is not None else None for k in
+                             ("term", "quadrant", "equat_passes", "closest_radius", "final_pos",
+                              "final_vel", "lambda", "steps")])
+
+
+class Context:
+    """One configured run (foort_gpu_open): the objects FOORT/Main.cpp builds
+    from its configuration, ready to integrate batches of pixels."""
+
+    def __init__(self, desc):
+        L = lib()
+        self._L = L
+        self.desc = desc
+        h = C.c_void_p()
+        rc = L.foort_gpu_open(C.byref(desc), C.byref(h))
+        if rc != 0:
+            raise FoortError(f"foort_gpu_open failed ({rc}): {L.foort_gpu_last_error(None).decode()}")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.foort_gpu_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise FoortError(f"{what} failed ({rc}): {self._L.foort_gpu_last_error(self._h).decode()}")
+
+    def trace_grid(self, side, first=0, count=None):
+        """SimpleSquareMesh pixels first..first+count-1 of a side x side grid (host arrays)."""
+        if count is None:
+            count = side * side - first
+        out = _host_results(count)
+        st = Stats()
+        res = _results_struct(out, lambda a: a.ctypes.data_as(C.c_void_p))
+        self._check(self._L.foort_gpu_trace_grid(self._h, side, first, count, C.byref(res), C.byref(st)),
+                    "foort_gpu_trace_grid")
+        out["stats"] = st
+        return out
+
+    def trace_pixels(self, side, screen_index):
+        """Explicit (row, column) list, as the subdivision meshes queue pixels."""
+        si = np.ascontiguousarray(screen_index, dtype=np.uint64).reshape(-1, 2)
+        count = si.shape[0]
+        out = _host_results(count)
+        st = Stats()
+        res = _results_struct(out, lambda a: a.ctypes.data_as(C.c_void_p))
+        self._check(self._L.foort_gpu_trace_pixels(self._h, side, count, si.ctypes.data_as(C.c_void_p),
+                                                   C.byref(res), C.byref(st)), "foort_gpu_trace_pixels")
+        out["stats"] = st
+        return out
+
+    def trace_device(self, side, first, count, results, screen_index=None, stream=None, stats=True):
+        """Device-resident batch: `results` maps field name -> torch CUDA tensor (or None)."""
+        res = _results_struct(results, lambda t: C.c_void_p(t.data_ptr()))
+        si = C.c_void_p(screen_index.data_ptr()) if screen_index is not None else None
+        st = Stats() if stats else None
+        s = C.c_void_p(stream.cuda_stream) if stream is not None else None
+        self._check(self._L.foort_gpu_trace_device(self._h, side, first, count, si, C.byref(res), s,
+                                                   C.byref(st) if st is not None else None),
+                    "foort_gpu_trace_device")
+        return st
+
+    def camera(self):
+        V, g0 = np.zeros(16), np.zeros(4)
+        self._check(self._L.foort_gpu_camera(self._h, V.ctypes.data_as(C.c_void_p), g0.ctypes.data_as(C.c_void_p)),
+                    "foort_gpu_camera")
+        return V.reshape(4, 4), g0
+
+
+def device_results(count, device="cuda", fields=("term", "quadrant", "equat_passes", "closest_radius",
+                                                  "final_pos", "final_vel", "lambda", "steps")):
+    """Allocate device result tensors for trace_device."""
+    import torch
+    shapes = {"term": ((count,), torch.int32), "quadrant": ((count,), torch.int32),
+              "equat_passes": ((count,), torch.int32), "closest_radius": ((count,), torch.float64),
+              "final_pos": ((count, 4), torch.float64), "final_vel": ((count, 4), torch.float64),
+              "lambda": ((count,), torch.float64), "steps": ((count,), torch.int32)}
+    return {k: torch.empty(*shapes[k][0], dtype=shapes[k][1], device=device) for k in fields}

@@ -1,0 +1,662 @@
+// foort_gpu.hip -- gfx950 kernels and the C ABI of include/foort_gpu.h.
+//
+// Build (see foort_amd/build.py): hipcc --offload-arch=gfx950 -O3 -std=c++17
+// -ffp-contract=off -Xclang -target-feature -Xclang -packed-fp32-ops -fPIC -shared.
+// Contraction must stay off: the finite-difference path reproduces the CPU
+// oracle's roundings exactly.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "foort_gpu.h"
+#include "foort_path.hpp"
+
+#pragma clang fp contract(off)
+
+using foort::GeoState;
+using foort::RunConst;
+
+namespace {
+
+struct DevResults {
+    int32_t *term;
+    int32_t *quadrant;
+    int32_t *passes;
+    double *closest;
+    double *pos;
+    double *vel;
+    double *lambda;
+    uint32_t *steps;
+};
+
+__device__ __forceinline__ void store_result(const DevResults &out, uint64_t i, const GeoState &g)
+{
+    if (out.term)
+        out.term[i] = g.term;
+    if (out.quadrant)
+        out.quadrant[i] = g.quadrant;
+    if (out.passes)
+        out.passes[i] = g.passes;
+    if (out.closest)
+        out.closest[i] = g.closest;
+    if (out.lambda)
+        out.lambda[i] = g.lambda;
+    if (out.steps)
+        out.steps[i] = g.steps;
+    if (out.pos) {
+        double2 *p = reinterpret_cast<double2 *>(out.pos + 4 * i);
+        p[0] = make_double2(g.pos[0], g.pos[1]);
+        p[1] = make_double2(g.pos[2], g.pos[3]);
+    }
+    if (out.vel) {
+        double2 *p = reinterpret_cast<double2 *>(out.vel + 4 * i);
+        p[0] = make_double2(g.vel[0], g.vel[1]);
+        p[1] = make_double2(g.vel[2], g.vel[3]);
+    }
+}
+
+// Unit screen point of a pixel, as the meshes compute it (FOORT/Mesh.cpp:42-49, :204-205)
+__device__ __forceinline__ void pixel_unit(uint64_t side, uint64_t first, const uint64_t *si, uint64_t i,
+                                           double &ux, double &uy)
+{
+    uint64_t row, col;
+    if (si) {
+        row = si[2 * i];
+        col = si[2 * i + 1];
+    } else {
+        const uint64_t index = first + i;
+        row = index / side;
+        col = index - row * side;
+    }
+    const double den = static_cast<double>(side - 1);
+    ux = static_cast<double>(row) * 1.0 / den;
+    uy = static_cast<double>(col) * 1.0 / den;
+}
+
+__device__ __forceinline__ void add_steps(unsigned long long *total, unsigned long long mine)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        mine += __shfl_xor(mine, off);
+    if ((threadIdx.x & 63) == 0)
+        atomicAdd(total, mine);
+}
+
+// One lane per pixel, integrated to termination (the drop-in for the
+// per-thread loop of FOORT/Main.cpp:373-400).
+template <int METRIC, int INTEG>
+__global__ void __launch_bounds__(256) trace_oneshot(const RunConst R, uint64_t side, uint64_t first, uint64_t count,
+                                                     const uint64_t *__restrict__ si, DevResults out,
+                                                     unsigned long long *__restrict__ total_steps)
+{
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    unsigned long long mine = 0;
+    if (i < count) {
+        double ux, uy;
+        pixel_unit(side, first, si, i, ux, uy);
+        GeoState g;
+        foort::geo_init(R, ux, uy, g);
+        while (g.term == FOORT_TERM_CONTINUE)
+            foort::geo_update<METRIC, INTEG>(R, g);
+        store_result(out, i, g);
+        mine = g.steps;
+    }
+    add_steps(total_steps, mine);
+}
+
+using KernelFn = void (*)(const RunConst, uint64_t, uint64_t, uint64_t, const uint64_t *, DevResults,
+                          unsigned long long *);
+
+template <int M>
+KernelFn pick_integ(int integ)
+{
+    return integ == FOORT_INTEGRATOR_VERLET ? trace_oneshot<M, FOORT_INTEGRATOR_VERLET>
+                                            : trace_oneshot<M, FOORT_INTEGRATOR_RK4>;
+}
+
+KernelFn pick_kernel(int metric, int integ)
+{
+    switch (metric) {
+    case FOORT_METRIC_KERR: return pick_integ<FOORT_METRIC_KERR>(integ);
+    case FOORT_METRIC_FLATSPACE: return pick_integ<FOORT_METRIC_FLATSPACE>(integ);
+    case FOORT_METRIC_RASHEEDLARSEN: return pick_integ<FOORT_METRIC_RASHEEDLARSEN>(integ);
+    case FOORT_METRIC_JOHANNSEN: return pick_integ<FOORT_METRIC_JOHANNSEN>(integ);
+    default: return pick_integ<FOORT_METRIC_MANKONOVIKOV>(integ);
+    }
+}
+
+thread_local std::string g_open_error;
+
+uint32_t clamp_u32(uint64_t v)
+{
+    return v > 0xffffffffull ? 0xffffffffu : static_cast<uint32_t>(v);
+}
+
+// ---------------------------------------------------------------------------
+// Host-side set-up
+// ---------------------------------------------------------------------------
+
+// Metric constructors, FOORT/Metric.cpp:111-128, :205-215, :244-271, :378-407, :490-509
+bool init_metric(const foort_run_desc &d, foort::MetricParams &M, std::string &err)
+{
+    const double *P = d.metric_params;
+    std::memset(&M, 0, sizeof(M));
+    M.rlog = d.rlog_scale ? 1 : 0;
+    M.has_horizon = 1;
+    switch (d.metric) {
+    case FOORT_METRIC_KERR:
+        M.a = P[0];
+        M.horizon = 1 + std::sqrt(1 - P[0] * P[0]);
+        break;
+    case FOORT_METRIC_FLATSPACE:
+        M.has_horizon = 0;
+        M.rlog = 0;
+        break;
+    case FOORT_METRIC_RASHEEDLARSEN: {
+        const double mP = P[0], aP = P[1], pP = P[2], qP = P[3];
+        M.a = aP / ((pP + qP) / 4.0);
+        M.m = mP / ((pP + qP) / 4.0);
+        M.p = pP / ((pP + qP) / 4.0);
+        M.q = qP / ((pP + qP) / 4.0);
+        M.horizon = (mP + std::sqrt(mP * mP - aP * aP)) / ((pP + qP) / 4.0);
+        break;
+    }
+    case FOORT_METRIC_JOHANNSEN:
+        M.a = P[0];
+        M.a13 = P[1];
+        M.a22 = P[2];
+        M.a52 = P[3];
+        M.eps3 = P[4];
+        M.horizon = 1 + std::sqrt(1 - P[0] * P[0]);
+        break;
+    case FOORT_METRIC_MANKONOVIKOV:
+        M.a = P[0];
+        M.alpha3 = P[1];
+        M.mn_alpha = (P[0] == 0.0) ? 0.0 : (-1. + std::sqrt(1. - P[0] * P[0])) / P[0];
+        M.mn_k = std::sqrt(1. - P[0] * P[0]);
+        M.horizon = 1 + std::sqrt(1. - P[0] * P[0]);
+        break;
+    default:
+        err = "unknown metric id " + std::to_string(d.metric);
+        return false;
+    }
+    return true;
+}
+
+double inner(const double g[4][4], const double *a, const double *b)
+{
+    double ret = 0;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j)
+            ret += a[i] * g[i][j] * b[j];
+    return ret;
+}
+
+void expand(const double *g5, double g[4][4])
+{
+    std::memset(g, 0, sizeof(double) * 16);
+    g[0][0] = g5[0];
+    g[0][3] = g5[1];
+    g[3][0] = g5[1];
+    g[1][1] = g5[2];
+    g[2][2] = g5[3];
+    g[3][3] = g5[4];
+}
+
+// ViewScreen::ConstructVielbein, FOORT/ViewScreen.cpp:10-73: Gram-Schmidt on
+// the rows of g^uu in the order t, phi, r, theta.  The theta row keeps the
+// reference's form: its e_r projection is a discarded statement and its e_phi
+// projection multiplies the still-zero theta row (FOORT/ViewScreen.cpp:56-58).
+void build_camera(int metric, RunConst &R, const double cam[4])
+{
+    double s, c;
+    fm_sincos(cam[2], &s, &c);
+    double g5[5], gdd[4][4], guu[4][4];
+    foort::metric_dd_rt(metric, R.M, R.cam_u, s, c, g5);
+    expand(g5, gdd);
+    foort::metric_uu_rt(metric, R.M, R.cam_u, s, c, g5);
+    expand(g5, guu);
+    double V[4][4] = {};
+    double sg[4] = {0, 0, 0, 0};
+    double n;
+    // e_t
+    for (int i = 0; i < 4; ++i)
+        V[0][i] = guu[0][i];
+    n = inner(gdd, V[0], V[0]);
+    sg[0] = (n < 0) ? -1 : 1;
+    {
+        const double t = 1 / std::sqrt(sg[0] * n);
+        for (int i = 0; i < 4; ++i)
+            V[0][i] = (V[0][i] * sg[0]) * t;
+    }
+    // e_phi
+    {
+        const double c0 = sg[0] * inner(gdd, guu[3], V[0]);
+        for (int i = 0; i < 4; ++i)
+            V[3][i] = guu[3][i] - V[0][i] * c0;
+        n = inner(gdd, V[3], V[3]);
+        sg[3] = (n < 0) ? -1 : 1;
+        const double sc = sg[3] * cam[1] * fm_sin(cam[2]);
+        const double t = 1 / std::sqrt(sg[3] * n);
+        for (int i = 0; i < 4; ++i)
+            V[3][i] = (V[3][i] * sc) * t;
+    }
+    // e_r
+    {
+        const double c0 = sg[0] * inner(gdd, guu[1], V[0]);
+        const double c3 = sg[3] * inner(gdd, guu[1], V[3]);
+        for (int i = 0; i < 4; ++i)
+            V[1][i] = (guu[1][i] - V[0][i] * c0) - V[3][i] * c3;
+        n = inner(gdd, V[1], V[1]);
+        sg[1] = (n < 0) ? -1 : 1;
+        const double t = 1 / std::sqrt(sg[1] * n);
+        for (int i = 0; i < 4; ++i)
+            V[1][i] = (V[1][i] * sg[1]) * t;
+    }
+    // e_theta
+    {
+        const double c0 = sg[0] * inner(gdd, guu[2], V[0]);
+        const double c3 = sg[3] * inner(gdd, guu[2], V[3]);
+        double row[4];
+        for (int i = 0; i < 4; ++i)
+            row[i] = (guu[2][i] - V[0][i] * c0) - V[2][i] * c3;
+        for (int i = 0; i < 4; ++i)
+            V[2][i] = row[i];
+        n = inner(gdd, V[2], V[2]);
+        sg[2] = (n < 0) ? -1 : 1;
+        const double sc = sg[2] * cam[1];
+        const double t = 1 / std::sqrt(sg[2] * n);
+        for (int i = 0; i < 4; ++i)
+            V[2][i] = (V[2][i] * sc) * t;
+    }
+    std::memcpy(R.V, V, sizeof(V));
+    for (int i = 0; i < 4; ++i)
+        R.g0[i] = gdd[0][i];
+}
+
+} // namespace
+
+struct foort_gpu_ctx {
+    foort_run_desc desc;
+    RunConst rc;
+    KernelFn kernel;
+    int device = 0;
+    std::string err;
+    // grow-only device staging for the host entry points
+    void *d_buf = nullptr;
+    size_t d_cap = 0;
+    unsigned long long *d_steps = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace {
+
+int fail(foort_gpu_ctx *ctx, int code, const std::string &msg)
+{
+    if (ctx)
+        ctx->err = msg;
+    else
+        g_open_error = msg;
+    return code;
+}
+
+int hip_fail(foort_gpu_ctx *ctx, hipError_t e, const char *what)
+{
+    return fail(ctx, FOORT_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(ctx, expr)                         \
+    do {                                           \
+        hipError_t e_ = (expr);                    \
+        if (e_ != hipSuccess)                      \
+            return hip_fail((ctx), e_, #expr);     \
+    } while (0)
+
+int ensure_buffer(foort_gpu_ctx *ctx, size_t bytes)
+{
+    if (bytes <= ctx->d_cap)
+        return FOORT_OK;
+    if (ctx->d_buf)
+        (void)hipFree(ctx->d_buf);
+    ctx->d_buf = nullptr;
+    ctx->d_cap = 0;
+    hipError_t e = hipMalloc(&ctx->d_buf, bytes);
+    if (e != hipSuccess)
+        return fail(ctx, FOORT_ENOMEM, std::string("hipMalloc of staging buffer: ") + hipGetErrorString(e));
+    ctx->d_cap = bytes;
+    return FOORT_OK;
+}
+
+int launch(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, uint64_t count, const uint64_t *si,
+           const DevResults &out, hipStream_t stream)
+{
+    if (count == 0)
+        return FOORT_OK;
+    const int block = ctx->desc.block_size > 0 ? ctx->desc.block_size : 256;
+    if (block > 256 || block % 64 != 0)
+        return fail(ctx, FOORT_EINVAL, "block_size must be a multiple of 64 and at most 256");
+    const uint64_t grid = (count + block - 1) / block;
+    if (grid > 0x7fffffffull)
+        return fail(ctx, FOORT_EINVAL, "batch too large");
+    hipLaunchKernelGGL(ctx->kernel, dim3(static_cast<unsigned>(grid)), dim3(block), 0, stream, ctx->rc, side, first,
+                       count, si, out, ctx->d_steps);
+    HIP_TRY(ctx, hipGetLastError());
+    return FOORT_OK;
+}
+
+int check_batch(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, uint64_t count, bool grid)
+{
+    if (side < 2)
+        return fail(ctx, FOORT_EINVAL, "row_column_size must be at least 2");
+    if (grid && (first > side * side || count > side * side - first))
+        return fail(ctx, FOORT_EINVAL, "grid indices exceed row_column_size^2");
+    return FOORT_OK;
+}
+
+// Host-pointer entry points: stage through the context buffer, one launch, copy back.
+int trace_host(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, uint64_t count, const uint64_t *si,
+               const foort_geodesic_results *out, foort_stats *stats)
+{
+    if (!out)
+        return fail(ctx, FOORT_EINVAL, "results pointer is NULL");
+    // layout of the staging buffer: [screen index | term | quadrant | passes | steps | closest | lambda | pos | vel]
+    auto align = [](size_t v) { return (v + 255) & ~size_t(255); };
+    size_t off = 0;
+    const size_t o_si = off;
+    off = align(off + (si ? count * 16 : 0));
+    const size_t o_term = off;
+    off = align(off + count * 4);
+    const size_t o_quad = off;
+    off = align(off + count * 4);
+    const size_t o_pass = off;
+    off = align(off + count * 4);
+    const size_t o_steps = off;
+    off = align(off + count * 4);
+    const size_t o_close = off;
+    off = align(off + count * 8);
+    const size_t o_lam = off;
+    off = align(off + count * 8);
+    const size_t o_pos = off;
+    off = align(off + count * 32);
+    const size_t o_vel = off;
+    off = align(off + count * 32);
+    int rc = ensure_buffer(ctx, off);
+    if (rc)
+        return rc;
+    char *base = static_cast<char *>(ctx->d_buf);
+    const uint64_t *d_si = nullptr;
+    if (si) {
+        HIP_TRY(ctx, hipMemcpy(base + o_si, si, count * 16, hipMemcpyHostToDevice));
+        d_si = reinterpret_cast<const uint64_t *>(base + o_si);
+    }
+    DevResults d{};
+    d.term = out->term ? reinterpret_cast<int32_t *>(base + o_term) : nullptr;
+    d.quadrant = out->quadrant ? reinterpret_cast<int32_t *>(base + o_quad) : nullptr;
+    d.passes = out->equat_passes ? reinterpret_cast<int32_t *>(base + o_pass) : nullptr;
+    d.steps = out->steps ? reinterpret_cast<uint32_t *>(base + o_steps) : nullptr;
+    d.closest = out->closest_radius ? reinterpret_cast<double *>(base + o_close) : nullptr;
+    d.lambda = out->lambda ? reinterpret_cast<double *>(base + o_lam) : nullptr;
+    d.pos = out->final_pos ? reinterpret_cast<double *>(base + o_pos) : nullptr;
+    d.vel = out->final_vel ? reinterpret_cast<double *>(base + o_vel) : nullptr;
+    HIP_TRY(ctx, hipMemset(ctx->d_steps, 0, sizeof(unsigned long long)));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, nullptr));
+    rc = launch(ctx, side, first, count, d_si, d, nullptr);
+    if (rc)
+        return rc;
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, nullptr));
+    HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+    auto back = [&](void *dst, const void *src, size_t bytes) -> hipError_t {
+        if (!dst || !src)
+            return hipSuccess;
+        return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+    };
+    HIP_TRY(ctx, back(out->term, d.term, count * 4));
+    HIP_TRY(ctx, back(out->quadrant, d.quadrant, count * 4));
+    HIP_TRY(ctx, back(out->equat_passes, d.passes, count * 4));
+    HIP_TRY(ctx, back(out->steps, d.steps, count * 4));
+    HIP_TRY(ctx, back(out->closest_radius, d.closest, count * 8));
+    HIP_TRY(ctx, back(out->lambda, d.lambda, count * 8));
+    HIP_TRY(ctx, back(out->final_pos, d.pos, count * 32));
+    HIP_TRY(ctx, back(out->final_vel, d.vel, count * 32));
+    if (stats) {
+        unsigned long long steps = 0;
+        HIP_TRY(ctx, hipMemcpy(&steps, ctx->d_steps, sizeof(steps), hipMemcpyDeviceToHost));
+        float ms = 0;
+        HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        stats->geodesics = count;
+        stats->steps = steps;
+        stats->launches = 1;
+        stats->device_ms = ms;
+    }
+    return FOORT_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+void foort_desc_init(foort_run_desc *d)
+{
+    if (!d)
+        return;
+    std::memset(d, 0, sizeof(*d));
+    d->metric = FOORT_METRIC_KERR;
+    d->metric_params[0] = 0.5;
+    d->integrator = FOORT_INTEGRATOR_RK4;
+    d->christoffel = FOORT_CHRISTOFFEL_FD;
+    d->step_size = 0.03;
+    d->smallest_step = 1e-12;
+    d->derivative_h = 1e-7;
+    d->verlet_tolerance = 0.001;
+    d->term_flags = FOORT_TERMFLAG_BOUNDARYSPHERE | FOORT_TERMFLAG_TIMEOUT;
+    d->precision = FOORT_PRECISION_FP64;
+    d->horizon_epsilon = 0.01;
+    d->horizon_update_n = 1;
+    d->sphere_radius = 1000;
+    d->sphere_update_n = 1;
+    d->timeout_max_steps = 10000;
+    d->timeout_update_n = 1;
+    d->theta_epsilon = 1e-5;
+    d->theta_update_n = 1;
+    d->diag_flags = FOORT_DIAGFLAG_FOURCOLORSCREEN;
+    d->value_diag = FOORT_DIAGFLAG_FOURCOLORSCREEN;
+    d->ep_threshold = 0.01;
+    d->ep_update = foort_update_frequency{1, 1, 1};
+    d->cr_update = foort_update_frequency{1, 1, 1};
+    d->camera_position[1] = 1000;
+    d->camera_position[2] = foort::kPi / 2.0;
+    d->screen_size[0] = 10;
+    d->screen_size[1] = 10;
+    d->schedule = FOORT_SCHEDULE_ONESHOT;
+    d->mixed_switch_radius = 10;
+    d->mixed_pole_margin = 0.1;
+}
+
+int foort_desc_size(void)
+{
+    return static_cast<int>(sizeof(foort_run_desc));
+}
+
+int foort_gpu_open(const foort_run_desc *desc, foort_gpu_ctx **out)
+{
+    if (!desc || !out)
+        return fail(nullptr, FOORT_EINVAL, "NULL descriptor or context pointer");
+    *out = nullptr;
+    const foort_run_desc &d = *desc;
+    if (d.integrator != FOORT_INTEGRATOR_RK4 && d.integrator != FOORT_INTEGRATOR_VERLET)
+        return fail(nullptr, FOORT_EINVAL, "unknown integrator");
+    if (d.christoffel != FOORT_CHRISTOFFEL_FD)
+        return fail(nullptr, FOORT_EINVAL, "only the finite-difference Christoffel policy is built");
+    if (d.precision != FOORT_PRECISION_FP64)
+        return fail(nullptr, FOORT_EINVAL, "only FP64 is built");
+    if (d.schedule != FOORT_SCHEDULE_ONESHOT)
+        return fail(nullptr, FOORT_EINVAL, "only the one-shot schedule is built");
+    if (d.diag_flags & FOORT_DIAGFLAG_GEODESICPOSITION)
+        return fail(nullptr, FOORT_EINVAL, "GeodesicPosition is not supported by the GPU path");
+    if (!(d.derivative_h > 0))
+        return fail(nullptr, FOORT_EINVAL, "DerivativeH must be positive");
+    if (d.step_cap > 0x7fffffffull)
+        return fail(nullptr, FOORT_EINVAL, "step_cap above 2^31-1");
+    if (!(d.camera_position[1] > 0))
+        return fail(nullptr, FOORT_EINVAL, "camera radius must be positive");
+
+    foort_gpu_ctx *ctx = new foort_gpu_ctx();
+    ctx->desc = d;
+    RunConst &R = ctx->rc;
+    std::memset(&R, 0, sizeof(R));
+    std::string err;
+    if (!init_metric(d, R.M, err)) {
+        delete ctx;
+        return fail(nullptr, FOORT_EINVAL, err);
+    }
+    R.eps = d.step_size;
+    R.minstep = d.smallest_step;
+    R.h_fd = d.derivative_h;
+    R.inv2h = 1 / (2 * d.derivative_h); // FOORT/Metric.cpp:30: / (2 * h) is * (1 / (2 * h))
+    R.verlet_tol = d.verlet_tolerance;
+    R.terms = d.term_flags & 0xfu;
+    if (!R.M.has_horizon) // Config turns Horizon off for metrics without one (FOORT/Config.cpp:652-658)
+        R.terms &= ~FOORT_TERMFLAG_HORIZON;
+    R.diags = d.diag_flags & (FOORT_DIAGFLAG_FOURCOLORSCREEN | FOORT_DIAGFLAG_EQUATORIALPASSES | FOORT_DIAGFLAG_CLOSESTRADIUS);
+    R.h_eps = d.horizon_epsilon;
+    R.sphere_R = d.sphere_radius;
+    R.theta_eps = d.theta_epsilon;
+    R.h_n = clamp_u32(d.horizon_update_n);
+    R.s_n = clamp_u32(d.sphere_update_n);
+    R.t_n = clamp_u32(d.timeout_update_n);
+    R.th_n = clamp_u32(d.theta_update_n);
+    R.max_steps = clamp_u32(d.timeout_max_steps);
+    R.step_cap = d.step_cap ? static_cast<uint32_t>(d.step_cap) : 0x7fffffffu;
+    R.ep_thr = d.ep_threshold;
+    R.ep_n = clamp_u32(d.ep_update.update_n_steps);
+    R.ep_start = d.ep_update.update_start;
+    R.ep_finish = d.ep_update.update_finish;
+    R.cr_n = clamp_u32(d.cr_update.update_n_steps);
+    R.cr_start = d.cr_update.update_start;
+    R.cr_finish = d.cr_update.update_finish;
+    R.cam_t = d.camera_position[0];
+    R.cam_r = d.camera_position[1];
+    R.cam_u = R.M.rlog ? std::log(d.camera_position[1]) : d.camera_position[1];
+    R.cam_theta = d.camera_position[2];
+    R.cam_phi = d.camera_position[3];
+    R.sin_theta0 = fm_sin(d.camera_position[2]);
+    R.size_x = d.screen_size[0];
+    R.size_y = d.screen_size[1];
+    R.center_x = d.screen_center[0];
+    R.center_y = d.screen_center[1];
+    build_camera(d.metric, R, d.camera_position);
+    ctx->kernel = pick_kernel(d.metric, d.integrator);
+
+    hipError_t e = hipGetDevice(&ctx->device);
+    if (e == hipSuccess)
+        e = hipMalloc(reinterpret_cast<void **>(&ctx->d_steps), sizeof(unsigned long long));
+    if (e == hipSuccess)
+        e = hipEventCreate(&ctx->ev0);
+    if (e == hipSuccess)
+        e = hipEventCreate(&ctx->ev1);
+    if (e != hipSuccess) {
+        std::string msg = std::string("HIP initialisation: ") + hipGetErrorString(e);
+        foort_gpu_close(ctx);
+        return fail(nullptr, FOORT_EDEVICE, msg);
+    }
+    *out = ctx;
+    return FOORT_OK;
+}
+
+void foort_gpu_close(foort_gpu_ctx *ctx)
+{
+    if (!ctx)
+        return;
+    if (ctx->d_buf)
+        (void)hipFree(ctx->d_buf);
+    if (ctx->d_steps)
+        (void)hipFree(ctx->d_steps);
+    if (ctx->ev0)
+        (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1)
+        (void)hipEventDestroy(ctx->ev1);
+    delete ctx;
+}
+
+const char *foort_gpu_last_error(const foort_gpu_ctx *ctx)
+{
+    return ctx ? ctx->err.c_str() : g_open_error.c_str();
+}
+
+int foort_gpu_trace_grid(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, uint64_t count,
+                         const foort_geodesic_results *out, foort_stats *stats)
+{
+    if (!ctx)
+        return fail(nullptr, FOORT_EINVAL, "NULL context");
+    int rc = check_batch(ctx, side, first, count, true);
+    if (rc)
+        return rc;
+    return trace_host(ctx, side, first, count, nullptr, out, stats);
+}
+
+int foort_gpu_trace_pixels(foort_gpu_ctx *ctx, uint64_t side, uint64_t count, const uint64_t *screen_index,
+                           const foort_geodesic_results *out, foort_stats *stats)
+{
+    if (!ctx)
+        return fail(nullptr, FOORT_EINVAL, "NULL context");
+    if (count && !screen_index)
+        return fail(ctx, FOORT_EINVAL, "screen_index is NULL");
+    int rc = check_batch(ctx, side, 0, count, false);
+    if (rc)
+        return rc;
+    for (uint64_t i = 0; i < 2 * count; ++i)
+        if (screen_index[i] >= side)
+            return fail(ctx, FOORT_EINVAL, "screen index outside the row_column_size grid");
+    return trace_host(ctx, side, 0, count, screen_index, out, stats);
+}
+
+int foort_gpu_trace_device(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, uint64_t count,
+                           const uint64_t *screen_index, const foort_geodesic_results *out, void *stream,
+                           foort_stats *stats)
+{
+    if (!ctx)
+        return fail(nullptr, FOORT_EINVAL, "NULL context");
+    if (!out)
+        return fail(ctx, FOORT_EINVAL, "results pointer is NULL");
+    int rc = check_batch(ctx, side, first, count, screen_index == nullptr);
+    if (rc)
+        return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    DevResults d{out->term, out->quadrant, out->equat_passes, out->closest_radius,
+                 out->final_pos, out->final_vel, out->lambda, out->steps};
+    if (stats) {
+        HIP_TRY(ctx, hipMemsetAsync(ctx->d_steps, 0, sizeof(unsigned long long), s));
+        HIP_TRY(ctx, hipEventRecord(ctx->ev0, s));
+    }
+    rc = launch(ctx, side, first, count, screen_index, d, s);
+    if (rc)
+        return rc;
+    if (stats) {
+        HIP_TRY(ctx, hipEventRecord(ctx->ev1, s));
+        HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+        unsigned long long steps = 0;
+        HIP_TRY(ctx, hipMemcpy(&steps, ctx->d_steps, sizeof(steps), hipMemcpyDeviceToHost));
+        float ms = 0;
+        HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        stats->geodesics = count;
+        stats->steps = steps;
+        stats->launches = 1;
+        stats->device_ms = ms;
+    }
+    return FOORT_OK;
+}
+
+int foort_gpu_camera(const foort_gpu_ctx *ctx, double vielbein[16], double row0[4])
+{
+    if (!ctx || !vielbein || !row0)
+        return FOORT_EINVAL;
+    std::memcpy(vielbein, ctx->rc.V, sizeof(double) * 16);
+    std::memcpy(row0, ctx->rc.g0, sizeof(double) * 4);
+    return FOORT_OK;
+}
+
+} // extern "C"

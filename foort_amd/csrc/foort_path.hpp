@@ -1,0 +1,647 @@
+// foort_path.hpp -- the per-pixel geodesic path, compiled for gfx950 kernels
+// and (for the camera set-up) for the host.
+//
+// Everything here follows the reference's arithmetic term by term so that,
+// compiled with floating-point contraction off, the GPU reproduces the CPU
+// oracle bit for bit:
+//   metrics            FOORT/Metric.cpp:131-191, :218-229, :273-361, :409-473, :511-626
+//   Christoffel (FD)   FOORT/Metric.cpp:14-55   (sparse form: contraction.inc)
+//   adaptive step      FOORT/Integrators.cpp:34-47
+//   RK4 / Verlet       FOORT/Integrators.cpp:51-94, :99-152
+//   Geodesic::Update   FOORT/Geodesic.cpp:61-88
+//   terminations       FOORT/Terminations.cpp:71-214
+//   diagnostics        FOORT/Diagnostics.cpp:113-140, :159-200, :351-380, :427-446
+//   initial conditions FOORT/ViewScreen.cpp:75-119
+// The metric expressions are written exactly as the reference writes them
+// (same operands, same association) because every rounding matters.
+#pragma once
+
+#include <stdint.h>
+
+#include "fm_math.h"
+#include "foort_gpu.h"
+
+#if defined(__HIPCC__)
+#define FM_HD __host__ __device__
+#define FM_HD_INLINE __host__ __device__ __attribute__((always_inline)) inline
+#else
+#define FM_HD
+#define FM_HD_INLINE inline
+#endif
+
+namespace foort {
+
+constexpr double kPi = 3.1415926535; // FOORT/Geometry.h:44
+
+// ---------------------------------------------------------------------------
+// Run constants (filled on the host by foort_gpu_open, passed by value)
+// ---------------------------------------------------------------------------
+struct MetricParams {
+    double a;               // Kerr / Johannsen / MN spin, RL rescaled a
+    double m, p, q;         // Rasheed-Larsen (rescaled by (p+q)/4)
+    double a13, a22, a52, eps3;  // Johannsen
+    double alpha3, mn_alpha, mn_k;  // Manko-Novikov
+    double horizon;         // SphericalHorizonMetric::m_HorizonRadius
+    int32_t rlog;
+    int32_t has_horizon;
+};
+
+struct RunConst {
+    MetricParams M;
+    // integrator
+    double eps, minstep, h_fd, inv2h, verlet_tol;
+    // terminations
+    uint32_t terms;
+    uint32_t diags;
+    double h_eps, sphere_R, theta_eps;
+    uint32_t h_n, s_n, t_n, th_n;   // update every n steps (0 = always)
+    uint32_t max_steps;
+    uint32_t step_cap;
+    // diagnostics
+    double ep_thr;
+    uint32_t ep_n, cr_n;
+    int32_t ep_start, ep_finish, cr_start, cr_finish;
+    // camera
+    double V[4][4];        // vielbein, row a = e_a
+    double g0[4];          // g_dd[0][i] at the camera
+    double cam_r;          // true camera radius
+    double cam_u;          // radial coordinate of the camera (log r with RLogScale)
+    double cam_t, cam_theta, cam_phi;
+    double sin_theta0;     // sin(camera theta)
+    double size_x, size_y, center_x, center_y;
+};
+
+// ---------------------------------------------------------------------------
+// Metrics: g_dd and g^uu at (u, theta) given sin/cos of theta.  u is the
+// radial coordinate (r, or log r with RLogScale).  Output order g00 g03 g11 g22 g33.
+// ---------------------------------------------------------------------------
+struct G5 {
+    double g[5];
+};
+
+FM_HD_INLINE double true_r(const MetricParams &M, double u)
+{
+    return M.rlog ? fm_exp(u) : u;
+}
+
+// FOORT/Metric.cpp:131-159
+FM_HD_INLINE void kerr_dd(const MetricParams &M, double u, double sint, double cost, double *g)
+{
+    const double a = M.a;
+    double r = M.rlog ? fm_exp(u) : u;
+    double sigma = r * r + a * a * cost * cost;
+    double delta = r * r + a * a - 2. * r;
+    double A_ = (r * r + a * a) * (r * r + a * a) - delta * a * a * sint * sint;
+    double g00 = -(1. - 2. * r / sigma);
+    double g11 = sigma / delta;
+    double g22 = sigma;
+    double g33 = A_ / sigma * sint * sint;
+    double g03 = -2. * a * r * sint * sint / sigma;
+    if (M.rlog)
+        g11 *= (r * r);
+    g[0] = g00; g[1] = g03; g[2] = g11; g[3] = g22; g[4] = g33;
+}
+
+// FOORT/Metric.cpp:162-191
+FM_HD_INLINE void kerr_uu(const MetricParams &M, double u, double sint, double cost, double *g)
+{
+    const double a = M.a;
+    double r = M.rlog ? fm_exp(u) : u;
+    double sigma = r * r + a * a * cost * cost;
+    double delta = r * r + a * a - 2. * r;
+    double A_ = (r * r + a * a) * (r * r + a * a) - delta * a * a * sint * sint;
+    double g00 = -A_ / (sigma * delta);
+    double g11 = delta / sigma;
+    double g22 = 1. / sigma;
+    double g33 = (delta - a * a * sint * sint) / (sigma * delta * sint * sint);
+    double g03 = -2. * a * r / (sigma * delta);
+    if (M.rlog)
+        g11 *= 1.0 / (r * r);
+    g[0] = g00; g[1] = g03; g[2] = g11; g[3] = g22; g[4] = g33;
+}
+
+// FOORT/Metric.cpp:218-229 (no RLogScale: FlatSpace is not a SphericalHorizonMetric)
+FM_HD_INLINE void flat_dd(const MetricParams &, double u, double sint, double, double *g)
+{
+    g[0] = -1; g[1] = 0; g[2] = 1; g[3] = u * u; g[4] = u * u * sint * sint;
+}
+
+FM_HD_INLINE void flat_uu(const MetricParams &, double u, double sint, double, double *g)
+{
+    g[0] = -1; g[1] = 0; g[2] = 1; g[3] = 1 / (u * u); g[4] = 1 / (u * u * sint * sint);
+}
+
+// FOORT/Metric.cpp:276-300 / :321-345
+struct RLCommon {
+    double r, sint2, delta, H3, Bp, H1, H2, sqH1H2;
+};
+
+FM_HD_INLINE void rl_common(const MetricParams &M, double u, double sint, double cost, RLCommon &c)
+{
+    const double m = M.m, a = M.a, pp = M.p, qq = M.q;
+    double r = M.rlog ? fm_exp(u) : u;
+    double r2 = r * r;
+    double sint2 = sint * sint;
+    double cost2 = cost * cost;
+    double delta = r2 + a * a - 2. * r * m;
+    double H3 = r2 - 2. * r * m + a * a * cost2;
+    double Bp = __builtin_sqrt(pp * qq) * a * sint2 *
+                (((pp * qq + 4. * m * m) * r - m * (pp - 2. * m) * (qq - 2. * m)) / (2. * m * (pp + qq) * H3));
+    double H1 = r2 + a * a * cost2 + r * (pp - 2. * m) + (pp / (pp + qq)) * ((pp - 2. * m) * (qq - 2. * m) / 2.) -
+                (pp / (2. * m * (pp + qq))) * __builtin_sqrt((pp * pp - 4. * m * m) * (qq * qq - 4. * m * m)) * a * cost;
+    double H2 = r2 + a * a * cost2 + r * (qq - 2. * m) + (qq / (pp + qq)) * ((pp - 2. * m) * (qq - 2. * m) / 2.) +
+                (qq / (2. * m * (pp + qq))) * __builtin_sqrt((pp * pp - 4. * m * m) * (qq * qq - 4. * m * m)) * a * cost;
+    c.r = r; c.sint2 = sint2; c.delta = delta; c.H3 = H3; c.Bp = Bp; c.H1 = H1; c.H2 = H2;
+    c.sqH1H2 = __builtin_sqrt(H1 * H2);
+}
+
+FM_HD_INLINE void rl_dd(const MetricParams &M, double u, double sint, double cost, double *g)
+{
+    RLCommon c;
+    rl_common(M, u, sint, cost, c);
+    double g00 = -c.H3 / c.sqH1H2;
+    double g11 = c.sqH1H2 / c.delta;
+    double g22 = c.sqH1H2;
+    double g33 = -(c.H3 * c.Bp * c.Bp) / c.sqH1H2 + (c.sqH1H2 * c.delta * c.sint2) / c.H3;
+    double g03 = -(c.H3 * c.Bp) / c.sqH1H2;
+    if (M.rlog)
+        g11 *= (c.r * c.r);
+    g[0] = g00; g[1] = g03; g[2] = g11; g[3] = g22; g[4] = g33;
+}
+
+FM_HD_INLINE void rl_uu(const MetricParams &M, double u, double sint, double cost, double *g)
+{
+    RLCommon c;
+    rl_common(M, u, sint, cost, c);
+    double g00 = ((c.H3 * c.H3 * c.Bp * c.Bp) / c.sint2 - c.H1 * c.H2 * c.delta) / (c.sqH1H2 * c.H3 * c.delta);
+    double g11 = c.delta / c.sqH1H2;
+    double g22 = 1. / c.sqH1H2;
+    double g33 = c.H3 / (c.sqH1H2 * c.delta * c.sint2);
+    double g03 = -(c.H3 * c.Bp) / (c.sqH1H2 * c.delta * c.sint2);
+    if (M.rlog)
+        g11 *= 1.0 / (c.r * c.r);
+    g[0] = g00; g[1] = g03; g[2] = g11; g[3] = g22; g[4] = g33;
+}
+
+// FOORT/Metric.cpp:409-440
+FM_HD_INLINE void joh_dd(const MetricParams &M, double u, double sint, double cost, double *g)
+{
+    const double a = M.a;
+    double r = M.rlog ? fm_exp(u) : u;
+    double A1 = 1. + M.a13 / (r * r * r);
+    double A2 = 1. + M.a22 / (r * r);
+    double A5 = 1. + M.a52 / (r * r);
+    double eps_f = M.eps3 / r;
+    double rho2 = r * r + a * a * cost * cost + eps_f;
+    double delta = r * r + a * a - 2. * r;
+    double den = fm_pow2((r * r + a * a) * A1 - a * a * A2 * sint * sint);
+    double g00 = -rho2 * (delta - a * a * A2 * A2 * sint * sint) / den;
+    double g11 = rho2 / (delta * A5);
+    double g22 = rho2;
+    double g33 = rho2 * sint * sint * (fm_pow2((r * r + a * a) * A1) - a * a * delta * sint * sint) / den;
+    double g03 = -a * ((r * r + a * a) * A1 * A2 - delta) * rho2 * sint * sint / den;
+    if (M.rlog)
+        g11 *= (r * r);
+    g[0] = g00; g[1] = g03; g[2] = g11; g[3] = g22; g[4] = g33;
+}
+
+// FOORT/Metric.cpp:442-473
+FM_HD_INLINE void joh_uu(const MetricParams &M, double u, double sint, double cost, double *g)
+{
+    const double a = M.a;
+    double r = M.rlog ? fm_exp(u) : u;
+    double A1 = 1. + M.a13 / (r * r * r);
+    double A2 = 1. + M.a22 / (r * r);
+    double A5 = 1. + M.a52 / (r * r);
+    double eps_f = M.eps3 / r;
+    double rho2 = r * r + a * a * cost * cost + eps_f;
+    double delta = r * r + a * a - 2. * r;
+    double g00 = (-1. * fm_pow2((r * r + a * a) * A1) + a * a * delta * sint * sint) / (delta * rho2);
+    double g11 = delta * A5 / rho2;
+    double g22 = 1. / rho2;
+    double g33 = (-a * a * A2 * A2 * sint * sint + delta) / (delta * rho2 * sint * sint);
+    double g03 = -a * (A2 * A1 * (r * r + a * a) - delta) / (delta * rho2);
+    if (M.rlog)
+        g11 *= 1.0 / (r * r);
+    g[0] = g00; g[1] = g03; g[2] = g11; g[3] = g22; g[4] = g33;
+}
+
+// FOORT/Metric.cpp:516-554 / :572-610
+struct MNCommon {
+    double r, sint, f, exp_2gamma, omega, rho_sq, delta;
+};
+
+FM_HD_INLINE void mn_common(const MetricParams &M, double u, double sint, double cost, MNCommon &c)
+{
+    const double al = M.mn_alpha, kk = M.mn_k, a3 = M.alpha3;
+    double r = M.rlog ? fm_exp(u) : u;
+    double xx = (r - 1.) / kk;
+    double yy = cost;
+    double R = __builtin_sqrt(xx * xx + yy * yy - 1.);
+    double P1 = xx * yy / R;
+    double P2 = 0.5 * (3. * xx * xx * yy * yy / (R * R) - 1.);
+    double P3 = 0.5 * (5. * fm_pow3(xx * yy / R) - 3. * xx * yy / R);
+    double P4 = 0.125 * (35. * fm_pow4(xx * yy / R) - 30. * xx * xx * yy * yy / (R * R) + 3.);
+    double aa = -al * fm_exp(2. * a3 * (1. - (xx - yy) * (1. / R + P1 / (R * R) + P2 / (R * R * R) + P3 / (R * R * R * R))));
+    double bb = al * fm_exp(2. * a3 * (-1. + (xx + yy) * (1. / R - P1 / (R * R) + P2 / (R * R * R) - P3 / (R * R * R * R))));
+    double AA = (xx * xx - 1.) * (1. + aa * bb) * (1. + aa * bb) - (1. - yy * yy) * (bb - aa) * (bb - aa);
+    double BB = fm_pow2((xx + 1. + (xx - 1.) * aa * bb)) + fm_pow2(((1. + yy) * aa + (1. - yy) * bb));
+    double CC = (xx * xx - 1.) * (1. + aa * bb) * (bb - aa - yy * (aa + bb)) +
+                (1. - yy * yy) * (bb - aa) * (1. + aa * bb + xx * (1. - aa * bb));
+    double psi = a3 * P3 / (R * R * R * R);
+    double gamma_prime = 2. * a3 * a3 * (P4 * P4 - P3 * P3) / (fm_pow8(R)) +
+                         2. * a3 * (-yy + xx * P1 / R - yy * P2 / (R * R) + xx * P3 / (R * R * R)) / R;
+    c.f = fm_exp(2. * psi) * AA / BB;
+    c.exp_2gamma = fm_exp(2. * gamma_prime) * AA / ((xx * xx - yy * yy) * (1. - al * al) * (1. - al * al));
+    c.omega = 2. * kk * fm_exp(-2. * psi) * CC / AA - 4. * kk * al / (1. - al * al);
+    c.rho_sq = (r - 1.) * (r - 1.) - kk * kk * cost * cost;
+    c.delta = (r - 1.) * (r - 1.) - kk * kk;
+    c.r = r;
+    c.sint = sint;
+}
+
+FM_HD_INLINE void mn_dd(const MetricParams &M, double u, double sint, double cost, double *g)
+{
+    MNCommon c;
+    mn_common(M, u, sint, cost, c);
+    double g00 = -c.f;
+    double g22 = c.exp_2gamma * c.rho_sq / c.f;
+    double g11 = g22 / c.delta;
+    double g33 = -c.f * c.omega * c.omega + c.delta * sint * sint / c.f;
+    double g03 = c.omega * c.f;
+    if (M.rlog)
+        g11 *= (c.r * c.r);
+    g[0] = g00; g[1] = g03; g[2] = g11; g[3] = g22; g[4] = g33;
+}
+
+FM_HD_INLINE void mn_uu(const MetricParams &M, double u, double sint, double cost, double *g)
+{
+    MNCommon c;
+    mn_common(M, u, sint, cost, c);
+    double beta_d = c.omega * c.f;
+    double gamma_dd = -c.f * c.omega * c.omega + c.delta * sint * sint / c.f;
+    double beta_u = beta_d / gamma_dd;
+    double alpha_sq = beta_u * beta_d + c.f;
+    double g00 = -1. / alpha_sq;
+    double g22 = c.f / (c.exp_2gamma * c.rho_sq);
+    double g11 = c.delta * g22;
+    double g33 = 1. / gamma_dd - beta_u * beta_u / alpha_sq;
+    double g03 = beta_u / alpha_sq;
+    if (M.rlog)
+        g11 *= 1.0 / (c.r * c.r);
+    g[0] = g00; g[1] = g03; g[2] = g11; g[3] = g22; g[4] = g33;
+}
+
+template <int METRIC>
+FM_HD_INLINE void metric_dd(const MetricParams &M, double u, double sint, double cost, double *g)
+{
+    if constexpr (METRIC == FOORT_METRIC_KERR) kerr_dd(M, u, sint, cost, g);
+    else if constexpr (METRIC == FOORT_METRIC_FLATSPACE) flat_dd(M, u, sint, cost, g);
+    else if constexpr (METRIC == FOORT_METRIC_RASHEEDLARSEN) rl_dd(M, u, sint, cost, g);
+    else if constexpr (METRIC == FOORT_METRIC_JOHANNSEN) joh_dd(M, u, sint, cost, g);
+    else mn_dd(M, u, sint, cost, g);
+}
+
+template <int METRIC>
+FM_HD_INLINE void metric_uu(const MetricParams &M, double u, double sint, double cost, double *g)
+{
+    if constexpr (METRIC == FOORT_METRIC_KERR) kerr_uu(M, u, sint, cost, g);
+    else if constexpr (METRIC == FOORT_METRIC_FLATSPACE) flat_uu(M, u, sint, cost, g);
+    else if constexpr (METRIC == FOORT_METRIC_RASHEEDLARSEN) rl_uu(M, u, sint, cost, g);
+    else if constexpr (METRIC == FOORT_METRIC_JOHANNSEN) joh_uu(M, u, sint, cost, g);
+    else mn_uu(M, u, sint, cost, g);
+}
+
+// runtime dispatch (host camera set-up only)
+FM_HD_INLINE void metric_dd_rt(int id, const MetricParams &M, double u, double sint, double cost, double *g)
+{
+    switch (id) {
+    case FOORT_METRIC_KERR: kerr_dd(M, u, sint, cost, g); break;
+    case FOORT_METRIC_FLATSPACE: flat_dd(M, u, sint, cost, g); break;
+    case FOORT_METRIC_RASHEEDLARSEN: rl_dd(M, u, sint, cost, g); break;
+    case FOORT_METRIC_JOHANNSEN: joh_dd(M, u, sint, cost, g); break;
+    default: mn_dd(M, u, sint, cost, g); break;
+    }
+}
+
+FM_HD_INLINE void metric_uu_rt(int id, const MetricParams &M, double u, double sint, double cost, double *g)
+{
+    switch (id) {
+    case FOORT_METRIC_KERR: kerr_uu(M, u, sint, cost, g); break;
+    case FOORT_METRIC_FLATSPACE: flat_uu(M, u, sint, cost, g); break;
+    case FOORT_METRIC_RASHEEDLARSEN: rl_uu(M, u, sint, cost, g); break;
+    case FOORT_METRIC_JOHANNSEN: joh_uu(M, u, sint, cost, g); break;
+    default: mn_uu(M, u, sint, cost, g); break;
+    }
+}
+
+#include "contraction.inc"
+
+// ---------------------------------------------------------------------------
+// Geodesic right-hand side with finite-difference Christoffels
+// (Metric::getChristoffel_udd + geoRHS): d/dlambda v = -Gamma v v
+// ---------------------------------------------------------------------------
+template <int METRIC>
+FM_HD_INLINE void rhs_fd(const RunConst &R, const double *p, const double *v, double *out)
+{
+    const double u = p[1], th = p[2], h = R.h_fd;
+    double s0, c0, sp, cp, sm, cm;
+    fm_sincos(th, &s0, &c0);
+    fm_sincos(th + h, &sp, &cp);
+    fm_sincos(th - h, &sm, &cm);
+    double gp[5], gm[5], d1[5], d2[5], uu[5];
+    metric_dd<METRIC>(R.M, u + h, s0, c0, gp);
+    metric_dd<METRIC>(R.M, u - h, s0, c0, gm);
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+        d1[k] = (gp[k] - gm[k]) * R.inv2h;
+    metric_dd<METRIC>(R.M, u, sp, cp, gp);
+    metric_dd<METRIC>(R.M, u, sm, cm, gm);
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+        d2[k] = (gp[k] - gm[k]) * R.inv2h;
+    metric_uu<METRIC>(R.M, u, s0, c0, uu);
+    geo_contract<double>(d1, d2, uu, v, out);
+}
+
+// std::min / std::max semantics
+FM_HD_INLINE double ref_min(double a, double b) { return (b < a) ? b : a; }
+FM_HD_INLINE double ref_max(double a, double b) { return (a < b) ? b : a; }
+
+// Integrators::GetAdaptiveStep, FOORT/Integrators.cpp:34-47
+FM_HD_INLINE double adaptive_step(const RunConst &R, const double *pos, const double *vel)
+{
+    double d1 = R.eps / (__builtin_fabs(vel[1]) + 1e-20);
+    double d2 = R.eps * ref_min(pos[2], kPi - pos[2]) / (__builtin_fabs(vel[2]) + 1e-20);
+    double d3 = R.eps / (__builtin_fabs(vel[3] + 1e-20));
+    double h = 1 / (1 / __builtin_fabs(d1) + 1 / __builtin_fabs(d2) + 1 / __builtin_fabs(d3));
+    return ref_max(h, R.minstep);
+}
+
+// IntegrateGeodesicStep_RK4, FOORT/Integrators.cpp:51-94
+template <int METRIC>
+FM_HD_INLINE void rk4_step(const RunConst &R, const double *x, const double *v, double *nx, double *nv, double &hout)
+{
+    const double h = adaptive_step(R, x, v);
+    const double hh = 0.5 * h;
+    double k1[4], k2[4], k3[4], k4[4], l2[4], l3[4], l4[4], tp[4];
+    rhs_fd<METRIC>(R, x, v, k1);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        tp[i] = x[i] + v[i] * hh;
+        l2[i] = v[i] + k1[i] * hh;
+    }
+    rhs_fd<METRIC>(R, tp, l2, k2);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        tp[i] = x[i] + l2[i] * hh;
+        l3[i] = v[i] + k2[i] * hh;
+    }
+    rhs_fd<METRIC>(R, tp, l3, k3);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        tp[i] = x[i] + l3[i] * h;
+        l4[i] = v[i] + k3[i] * h;
+    }
+    rhs_fd<METRIC>(R, tp, l4, k4);
+    const double h6 = h / 6.0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        nv[i] = v[i] + (((k1[i] + k2[i] * 2) + k3[i] * 2) + k4[i]) * h6;
+        nx[i] = x[i] + (((v[i] + l2[i] * 2) + l3[i] * 2) + l4[i]) * h6;
+    }
+    hout = h;
+}
+
+// Verlet iterations are capped (the reference loops without bound); the CPU
+// oracle uses the same cap.
+constexpr int kVerletMaxIter = 1000;
+
+// IntegrateGeodesicStep_Verlet, FOORT/Integrators.cpp:99-152
+template <int METRIC>
+FM_HD_INLINE void verlet_step(const RunConst &R, const double *x, const double *v, double *nx, double *nv, double &hout)
+{
+    const double h = adaptive_step(R, x, v);
+    double ac[4], as[4], vi[4];
+    rhs_fd<METRIC>(R, x, v, ac);
+    const double hh2 = h * h / 2.0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        nx[i] = (x[i] + v[i] * h) + ac[i] * hh2;
+        vi[i] = v[i] + ac[i] * h;
+    }
+    rhs_fd<METRIC>(R, nx, vi, as);
+    const double h2 = h / 2.0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        nv[i] = v[i] + (ac[i] + as[i]) * h2;
+    if (R.verlet_tol > 0.0) {
+        const double tol2 = R.verlet_tol * R.verlet_tol;
+        for (int it = 0; it < kVerletMaxIter; ++it) {
+            double n1 = 0, n2 = 0;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                double dd = nv[i] - vi[i];
+                n1 += dd * dd;
+                n2 += nv[i] * nv[i];
+            }
+            if (!(n1 / n2 > tol2))
+                break;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                vi[i] = nv[i];
+            rhs_fd<METRIC>(R, nx, vi, as);
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                nv[i] = v[i] + (ac[i] + as[i]) * h2;
+        }
+    }
+    hout = h;
+}
+
+// ---------------------------------------------------------------------------
+// Geodesic state, terminations and diagnostics
+// ---------------------------------------------------------------------------
+struct GeoState {
+    double pos[4], vel[4], lambda;
+    double prevtheta, closest;
+    int32_t term;
+    uint32_t steps;
+    uint32_t hc, bc, tc, thc, tsteps;   // termination counters
+    uint32_t epc, crc;                  // diagnostic counters
+    int32_t quadrant, passes;
+};
+
+// Termination::DecideUpdate, FOORT/Terminations.cpp:71-90
+FM_HD_INLINE bool term_decide(uint32_t n, uint32_t &cnt)
+{
+    if (n == 0)
+        return true;
+    ++cnt;
+    if (cnt >= n) {
+        cnt = 0;
+        return true;
+    }
+    return false;
+}
+
+// Diagnostic::DecideUpdate, FOORT/Diagnostics.cpp:113-140
+FM_HD_INLINE bool diag_decide(uint32_t n, int32_t start, int32_t finish, uint32_t &cnt, const GeoState &g)
+{
+    if (n > 0) {
+        ++cnt;
+        if (cnt >= n) {
+            cnt = 0;
+            return true;
+        }
+        return false;
+    } else if (start && g.lambda == 0.0) {
+        return true;
+    } else if (finish && g.term != FOORT_TERM_CONTINUE) {
+        return true;
+    }
+    return false;
+}
+
+// FourColorScreenDiagnostic::UpdateData, FOORT/Diagnostics.cpp:159-200
+FM_HD_INLINE int32_t four_color_quadrant(const double *pos)
+{
+    double p3 = pos[3];
+    if (__builtin_fabs(p3) < 2 * kPi * 1e5) {
+        while (p3 > 2 * kPi)
+            p3 -= 2 * kPi;
+        while (p3 < 0)
+            p3 += 2 * kPi;
+    }
+    if (pos[2] < kPi / 2)
+        return (p3 < kPi) ? 1 : 2;
+    return (p3 < kPi) ? 3 : 4;
+}
+
+FM_HD_INLINE void diag_update(const RunConst &R, GeoState &g)
+{
+    if (R.diags & FOORT_DIAGFLAG_FOURCOLORSCREEN) {
+        if (g.term == FOORT_TERM_BOUNDARYSPHERE)
+            g.quadrant = four_color_quadrant(g.pos);
+    }
+    if (R.diags & FOORT_DIAGFLAG_EQUATORIALPASSES) { // FOORT/Diagnostics.cpp:351-380
+        if (diag_decide(R.ep_n, R.ep_start, R.ep_finish, g.epc, g)) {
+            double cur = g.pos[2];
+            if (__builtin_fabs(cur - kPi / 2.0) > kPi / 2.0 * R.ep_thr) {
+                if (g.prevtheta > 0 && ((g.prevtheta - kPi / 2.0) * (cur - kPi / 2.0)) < 0.0)
+                    ++g.passes;
+                g.prevtheta = cur;
+            }
+        }
+        if (g.term == FOORT_TERM_HORIZON)
+            g.passes = -g.passes;
+    }
+    if (R.diags & FOORT_DIAGFLAG_CLOSESTRADIUS) { // FOORT/Diagnostics.cpp:427-446
+        if (diag_decide(R.cr_n, R.cr_start, R.cr_finish, g.crc, g)) {
+            double cur = true_r(R.M, g.pos[1]);
+            if (g.closest < 0 || cur < g.closest)
+                g.closest = cur;
+        }
+        if (g.term == FOORT_TERM_HORIZON)
+            g.closest = 0.0;
+    }
+}
+
+// Terminations in the order of CreateTerminationVector, FOORT/Terminations.cpp:22-40
+FM_HD_INLINE int32_t check_terms(const RunConst &R, GeoState &g)
+{
+    if (R.terms & FOORT_TERMFLAG_HORIZON) { // :96-117
+        if (term_decide(R.h_n, g.hc)) {
+            double r = true_r(R.M, g.pos[1]);
+            if ((r < R.M.horizon * (1 + R.h_eps)) || (R.M.horizon == 0.0 && r < R.h_eps))
+                return FOORT_TERM_HORIZON;
+        }
+    }
+    if (R.terms & FOORT_TERMFLAG_BOUNDARYSPHERE) { // :130-146
+        if (term_decide(R.s_n, g.bc)) {
+            double r = true_r(R.M, g.pos[1]);
+            if (r > R.sphere_R)
+                return FOORT_TERM_BOUNDARYSPHERE;
+        }
+    }
+    if (R.terms & FOORT_TERMFLAG_TIMEOUT) { // :168-186
+        if (term_decide(R.t_n, g.tc)) {
+            if (g.tsteps >= R.max_steps)
+                return FOORT_TERM_TIMEOUT;
+            ++g.tsteps;
+        }
+    }
+    if (R.terms & FOORT_TERMFLAG_THETASINGULARITY) { // :200-214
+        if (term_decide(R.th_n, g.thc)) {
+            double th = g.pos[2];
+            if (__builtin_fabs(th) < R.theta_eps || __builtin_fabs(kPi - th) < R.theta_eps)
+                return FOORT_TERM_THETASINGULARITY;
+        }
+    }
+    return FOORT_TERM_CONTINUE;
+}
+
+// ViewScreen::SetNewInitialConditions + Geodesic::Reset for the unit screen point (ux, uy)
+FM_HD_INLINE void geo_init(const RunConst &R, double ux, double uy, GeoState &g)
+{
+    g.pos[0] = R.cam_t;
+    g.pos[1] = R.cam_u;
+    g.pos[2] = R.cam_theta;
+    g.pos[3] = R.cam_phi;
+    const double alpha = R.center_x + R.size_x * (ux - 0.5);
+    const double beta = R.center_y + R.size_y * (uy - 0.5);
+    const double rc = R.cam_r;
+    const double densqrt = __builtin_sqrt(rc * rc + alpha * alpha + beta * beta);
+    const double pf[4] = {-1.0, -rc / densqrt, -beta / rc / densqrt, alpha / rc / R.sin_theta0 / densqrt};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        double s = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            s += R.V[j][i] * pf[j];
+        g.vel[i] = s;
+    }
+    double cur = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        cur += R.g0[i] * g.vel[i];
+    const double inv = 1 / cur;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        g.vel[i] = (g.vel[i] * 1.0) * inv;
+    g.lambda = 0.0;
+    g.term = FOORT_TERM_CONTINUE;
+    g.steps = 0;
+    g.hc = g.bc = g.tc = g.thc = g.tsteps = 0;
+    g.epc = g.crc = 0;
+    g.quadrant = 0;
+    g.passes = 0;
+    g.prevtheta = -1;
+    g.closest = -1;
+    diag_update(R, g);
+}
+
+// Geodesic::Update, FOORT/Geodesic.cpp:61-88
+template <int METRIC, int INTEG>
+FM_HD_INLINE void geo_update(const RunConst &R, GeoState &g)
+{
+    double np[4], nv[4], h;
+    if constexpr (INTEG == FOORT_INTEGRATOR_VERLET)
+        verlet_step<METRIC>(R, g.pos, g.vel, np, nv, h);
+    else
+        rk4_step<METRIC>(R, g.pos, g.vel, np, nv, h);
+    g.lambda += h;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        g.pos[i] = np[i];
+        g.vel[i] = nv[i];
+    }
+    ++g.steps;
+    int32_t t = check_terms(R, g);
+    if (t == FOORT_TERM_CONTINUE && g.steps >= R.step_cap)
+        t = FOORT_TERM_STEPCAP;
+    g.term = t;
+    diag_update(R, g);
+}
+
+} // namespace foort

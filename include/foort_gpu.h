@@ -1,0 +1,205 @@
+/*
+ * foort_gpu.h -- C ABI of the MI355X (gfx950) per-pixel null-geodesic
+ * integrator.  It drops in for the OpenMP block of the reference driver,
+ * FOORT/Main.cpp:320-414: for every pixel index of one mesh iteration that
+ * block calls ViewScreen::SetNewInitialConditions (FOORT/ViewScreen.cpp:75-119),
+ * Geodesic::Reset (FOORT/Geodesic.cpp:33-58), Geodesic::Update until the
+ * termination condition is no longer Term::Continue (FOORT/Geodesic.cpp:61-88),
+ * then hands the diagnostics to Mesh::GeodesicFinished and
+ * GeodesicOutputHandler::NewGeodesicOutput.  One foort_gpu_trace_* call does
+ * the same for a whole batch of pixels on the GPU and returns, per pixel, the
+ * termination, the diagnostic values and the final state.
+ *
+ * Plain C types only; no exceptions cross the boundary; every entry point
+ * returns FOORT_OK (0) or a negative FOORT_E* code and records a message that
+ * foort_gpu_last_error() returns.  A context is used by one host thread at a
+ * time; several contexts may run concurrently on different streams.
+ */
+#ifndef FOORT_GPU_H
+#define FOORT_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- return codes ---- */
+#define FOORT_OK 0
+#define FOORT_EINVAL (-1)    /* invalid descriptor or argument */
+#define FOORT_EDEVICE (-2)   /* HIP runtime error */
+#define FOORT_ENOMEM (-3)    /* device allocation failed */
+
+/* ---- metrics: the Metric descendants of FOORT/Metric.h:78-189 ---- */
+#define FOORT_METRIC_KERR 0          /* params {a}                          FOORT/Metric.cpp:111-197 */
+#define FOORT_METRIC_FLATSPACE 1     /* params {}                           FOORT/Metric.cpp:205-235 */
+#define FOORT_METRIC_RASHEEDLARSEN 2 /* params {m, a, p, q}                 FOORT/Metric.cpp:244-369 */
+#define FOORT_METRIC_JOHANNSEN 3     /* params {a, alpha13, alpha22, alpha52, epsilon3}  FOORT/Metric.cpp:378-481 */
+#define FOORT_METRIC_MANKONOVIKOV 4  /* params {a, alpha3}                  FOORT/Metric.cpp:490-633 */
+
+/* ---- integrators: FOORT/Integrators.h:54-62 ---- */
+#define FOORT_INTEGRATOR_RK4 0
+#define FOORT_INTEGRATOR_VERLET 1
+
+/* ---- Christoffel policy ---- */
+#define FOORT_CHRISTOFFEL_FD 0       /* central differences of g_dd, FOORT/Metric.cpp:14-55 (the reference) */
+#define FOORT_CHRISTOFFEL_ANALYTIC 1 /* exact derivatives of the same g_dd (forward-mode differentiation) */
+
+/* ---- arithmetic of the per-step right-hand side ---- */
+#define FOORT_PRECISION_FP64 0
+#define FOORT_PRECISION_MIXED 1      /* binary32 right-hand side where r > mixed_switch_radius
+                                        and theta is away from the poles; state, step size and
+                                        the update stay binary64 (needs FOORT_CHRISTOFFEL_ANALYTIC) */
+
+/* ---- work distribution ---- */
+#define FOORT_SCHEDULE_ONESHOT 0     /* one lane per pixel, integrated to termination */
+#define FOORT_SCHEDULE_PERSISTENT 1  /* resident lanes refill from a pixel counter as geodesics end */
+
+/* ---- termination bitflags: TermBitflag, FOORT/Terminations.h:29-33 ---- */
+#define FOORT_TERMFLAG_BOUNDARYSPHERE 0x1u
+#define FOORT_TERMFLAG_TIMEOUT 0x2u
+#define FOORT_TERMFLAG_HORIZON 0x4u
+#define FOORT_TERMFLAG_THETASINGULARITY 0x8u
+
+/* ---- termination conditions: enum class Term, FOORT/Terminations.h:48-67 ---- */
+#define FOORT_TERM_CONTINUE 0
+#define FOORT_TERM_HORIZON 1
+#define FOORT_TERM_BOUNDARYSPHERE 2
+#define FOORT_TERM_TIMEOUT 3
+#define FOORT_TERM_THETASINGULARITY 4
+#define FOORT_TERM_STEPCAP 5         /* no counterpart: the step_cap safety ceiling was reached */
+
+/* ---- diagnostic bitflags: DiagBitflag, FOORT/Diagnostics.h:33-37 ---- */
+#define FOORT_DIAGFLAG_GEODESICPOSITION 0x1u
+#define FOORT_DIAGFLAG_FOURCOLORSCREEN 0x2u
+#define FOORT_DIAGFLAG_EQUATORIALPASSES 0x4u
+#define FOORT_DIAGFLAG_CLOSESTRADIUS 0x8u
+
+/* UpdateFrequency, FOORT/Diagnostics.h:50-55 */
+typedef struct foort_update_frequency {
+    uint64_t update_n_steps;
+    int32_t update_start;
+    int32_t update_finish;
+} foort_update_frequency;
+
+/*
+ * Everything the reference reads from its configuration file for one run
+ * (Config::GetMetric ... Config::GetGeodesicIntegrator, FOORT/Config.cpp:123-1071),
+ * plus execution options.  foort_desc_init() fills the reference defaults.
+ */
+typedef struct foort_run_desc {
+    /* Metric */
+    int32_t metric;              /* FOORT_METRIC_* */
+    int32_t rlog_scale;          /* Metric.RLogScale: the radial coordinate is u = log r */
+    double metric_params[6];     /* constructor arguments, in the order listed above */
+    /* Integrator */
+    int32_t integrator;          /* FOORT_INTEGRATOR_* */
+    int32_t christoffel;         /* FOORT_CHRISTOFFEL_* */
+    double step_size;            /* Integrator.StepSize (epsilon, default 0.03) */
+    double smallest_step;        /* Integrator.SmallestPossibleStepsize (1e-12) */
+    double derivative_h;         /* Integrator.DerivativeH (1e-7) */
+    double verlet_tolerance;     /* Integrator.VerletVelocityTolerance (0.001) */
+    /* Terminations */
+    uint32_t term_flags;         /* FOORT_TERMFLAG_* */
+    uint32_t precision;          /* FOORT_PRECISION_* */
+    double horizon_epsilon;      /* Horizon.Epsilon_Horizon (0.01) */
+    uint64_t horizon_update_n;   /* Horizon.UpdateFrequency (1) */
+    double sphere_radius;        /* BoundarySphere.SphereRadius (1000) */
+    uint64_t sphere_update_n;
+    uint64_t timeout_max_steps;  /* TimeOut.MaxSteps (10000) */
+    uint64_t timeout_update_n;
+    double theta_epsilon;        /* ThetaSingularity.Epsilon (1e-5) */
+    uint64_t theta_update_n;
+    /* Diagnostics */
+    uint32_t diag_flags;         /* FOORT_DIAGFLAG_* */
+    uint32_t value_diag;         /* the UseForMesh diagnostic (one FOORT_DIAGFLAG_*) */
+    double ep_threshold;         /* EquatorialPasses.Threshold (0.01) */
+    foort_update_frequency ep_update;  /* (1, start, finish) */
+    foort_update_frequency cr_update;  /* ClosestRadius (1, start, finish) */
+    /* ViewScreen */
+    double camera_position[4];   /* (t, r, theta, phi); r is the true radius even with RLogScale */
+    double screen_size[2];       /* ScreenSize {x, y} (10, 10) */
+    double screen_center[2];     /* ScreenCenter {x, y} (0, 0) */
+    /* Execution (no counterpart in the reference) */
+    uint64_t step_cap;           /* per-geodesic ceiling on integration steps; 0 = 2^31-1 */
+    int32_t schedule;            /* FOORT_SCHEDULE_* */
+    int32_t block_size;          /* threads per workgroup; 0 = tuned default */
+    uint32_t steps_per_launch;   /* persistent schedule: steps per kernel launch; 0 = default */
+    uint32_t lanes_per_cu;       /* persistent schedule: resident lanes per CU; 0 = default */
+    double mixed_switch_radius;  /* FOORT_PRECISION_MIXED: binary32 steps only where r > this (10) */
+    double mixed_pole_margin;    /* ... and where min(theta, pi - theta) > this (0.1) */
+} foort_run_desc;
+
+/*
+ * Per-geodesic results of one batch, structure of arrays, `count` entries
+ * each (final_pos and final_vel: 4 doubles per geodesic, t r theta phi).
+ * Any pointer may be NULL and is then not written.  For the *_device entry
+ * points all pointers are device pointers.
+ */
+typedef struct foort_geodesic_results {
+    int32_t *term;            /* Term at termination (FOORT_TERM_*) */
+    int32_t *quadrant;        /* FourColorScreen value, FOORT/Diagnostics.cpp:159-200 */
+    int32_t *equat_passes;    /* EquatorialPasses value, FOORT/Diagnostics.cpp:351-380 */
+    double *closest_radius;   /* ClosestRadius value, FOORT/Diagnostics.cpp:427-446 */
+    double *final_pos;        /* Geodesic::getCurrentPos at termination */
+    double *final_vel;        /* Geodesic::getCurrentVel */
+    double *lambda;           /* Geodesic::getCurrentLambda */
+    uint32_t *steps;          /* number of Geodesic::Update calls */
+} foort_geodesic_results;
+
+typedef struct foort_stats {
+    uint64_t geodesics;       /* geodesics integrated by the call */
+    uint64_t steps;           /* sum of their integration steps */
+    uint64_t launches;        /* kernel launches */
+    double device_ms;         /* GPU time of the integration kernels */
+} foort_stats;
+
+typedef struct foort_gpu_ctx foort_gpu_ctx;
+
+/* Fill *d with the reference defaults (FOORT/Config.cpp:123-1155, FOORT/Integrators.h:31-57). */
+void foort_desc_init(foort_run_desc *d);
+
+/* sizeof(foort_run_desc) as compiled into the library (lets a binding check its layout). */
+int foort_desc_size(void);
+
+/* Validate *desc, set up the camera frame (FOORT/ViewScreen.cpp:10-73) on the
+ * current HIP device and return a context.  Replaces the object construction
+ * of FOORT/Main.cpp:205-256. */
+int foort_gpu_open(const foort_run_desc *desc, foort_gpu_ctx **ctx);
+
+void foort_gpu_close(foort_gpu_ctx *ctx);
+
+/* Message of the last failure on ctx (or of the last failed open when ctx is NULL). */
+const char *foort_gpu_last_error(const foort_gpu_ctx *ctx);
+
+/* SimpleSquareMesh batch (FOORT/Mesh.cpp:31-50): pixel indices first .. first+count-1 of a
+ * row_column_size x row_column_size grid, row = index / size, column = index % size.
+ * Host result pointers; synchronous. */
+int foort_gpu_trace_grid(foort_gpu_ctx *ctx, uint64_t row_column_size, uint64_t first,
+                         uint64_t count, const foort_geodesic_results *out, foort_stats *stats);
+
+/* Explicit pixel list, as the subdivision meshes queue them (FOORT/Mesh.cpp:199-206, :666-673):
+ * screen_index holds (row, column) pairs, unit screen coordinates are index / (size - 1).
+ * Host pointers; synchronous. */
+int foort_gpu_trace_pixels(foort_gpu_ctx *ctx, uint64_t row_column_size, uint64_t count,
+                           const uint64_t *screen_index, const foort_geodesic_results *out,
+                           foort_stats *stats);
+
+/* Device-resident variant: screen_index (or NULL for grid indices first .. first+count-1)
+ * and every result pointer are device memory; work is queued on `stream` (a hipStream_t,
+ * NULL for the default stream) and the call returns without waiting.  stats, if given,
+ * is filled when the work has completed (the call then synchronizes the stream). */
+int foort_gpu_trace_device(foort_gpu_ctx *ctx, uint64_t row_column_size, uint64_t first,
+                           uint64_t count, const uint64_t *screen_index,
+                           const foort_geodesic_results *out, void *stream, foort_stats *stats);
+
+/* The camera frame the context uses: vielbein[4][4] (row a = frame vector e_a,
+ * FOORT/ViewScreen.h:88) and row 0 of g_dd at the camera (FOORT/ViewScreen.h:86). */
+int foort_gpu_camera(const foort_gpu_ctx *ctx, double vielbein[16], double metric_dd_row0[4]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FOORT_GPU_H */

@@ -1,0 +1,75 @@
+"""Accuracy of the portable binary64 functions (foort_amd/csrc/fm_math.h)
+against mpmath at 200 bits, on the argument ranges the path uses, plus the
+special values.  The same code runs in the GPU kernels; GPU/CPU identity is
+covered by the parity tests."""
+import ctypes as C
+import math
+
+import mpmath as mp
+import numpy as np
+import pytest
+
+import oracle as O
+
+mp.mp.prec = 200
+
+
+def _eval(which, x):
+    L = O.lib("fm")
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    a, b = np.zeros_like(x), np.zeros_like(x)
+    L.oracle_fm_eval.argtypes = [C.c_void_p, C.c_uint64, C.c_int, C.c_void_p, C.c_void_p]
+    L.oracle_fm_eval(x.ctypes.data_as(C.c_void_p), len(x), which, a.ctypes.data_as(C.c_void_p),
+                     b.ctypes.data_as(C.c_void_p))
+    return a, b
+
+
+def _ulp_err(v, ref):
+    ref = mp.mpf(ref)
+    if ref == 0:
+        return 0.0 if v == 0 else math.inf
+    e = mp.floor(mp.log(abs(ref), 2))
+    return float(abs(mp.mpf(float(v)) - ref) / mp.power(2, e - 52))
+
+
+def _max_ulp(vals, xs, f):
+    return max(_ulp_err(v, f(mp.mpf(float(x)))) for v, x in zip(vals, xs))
+
+
+RNG = np.random.default_rng(11)
+
+
+def test_sincos_theta_range():
+    x = RNG.uniform(-0.1, 3.3, 3000)
+    s, c = _eval(0, x)
+    assert _max_ulp(s, x, mp.sin) < 0.8
+    assert _max_ulp(c, x, mp.cos) < 0.8
+
+
+def test_sincos_wide_range_and_specials():
+    x = np.concatenate([RNG.uniform(-1e6, 1e6, 500), [0.0, -0.0, 1e-300, 1.5707963267948966, 3.141592653589793]])
+    s, c = _eval(0, x)
+    assert _max_ulp(s[:-5], x[:-5], mp.sin) < 1.0
+    assert s[-5] == 0.0 and c[-5] == 1.0 and s[-3] == 1e-300
+    s, c = _eval(0, np.array([np.inf, -np.inf, np.nan]))
+    assert np.isnan(s).all() and np.isnan(c).all()
+
+
+def test_exp_accuracy_and_limits():
+    x = np.concatenate([RNG.uniform(-10, 10, 2000), RNG.uniform(-700, 700, 500)])
+    e, _ = _eval(1, x)
+    assert _max_ulp(e, x, mp.exp) < 0.55
+    e, _ = _eval(1, np.array([0.0, 710.0, -746.0, np.inf, -np.inf, np.nan, 709.7, -740.0]))
+    assert e[0] == 1.0 and e[1] == np.inf and e[2] == 0.0 and e[3] == np.inf and e[4] == 0.0
+    assert np.isnan(e[5]) and np.isfinite(e[6])
+    assert 0 < e[7] < 1e-300  # subnormal range
+    assert abs(e[7] - float(mp.exp(-740))) <= 2 ** -1074
+
+
+@pytest.mark.parametrize("which,n", [(2, 3), (3, 4), (4, 8)])
+def test_integer_powers_correctly_rounded(which, n):
+    x = np.concatenate([RNG.uniform(0.05, 30.0, 2000), -RNG.uniform(0.05, 5.0, 200)])
+    p, _ = _eval(which, x)
+    ref = [float(mp.mpf(float(v)) ** n) for v in x]
+    assert np.mean(p == np.array(ref)) > 0.999  # double-double: correctly rounded but for rare midpoint cases
+    assert _max_ulp(p, x, lambda X: X ** n) < 0.501

@@ -1,0 +1,110 @@
+"""GPU path vs the CPU oracle (portable-math build), bit for bit, through the C ABI.
+
+The GPU kernels and oracle/liboracle_fm.so evaluate the same reference
+arithmetic with the same elementary functions (foort_amd/csrc/fm_math.h), so
+every field -- termination, the three diagnostics, final position, velocity,
+affine parameter and step count -- must agree exactly.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from foort_amd import abi as A
+from tests.cases import FIELDS, PARITY_CASES, bitwise_equal, c1
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu():
+    import foort_amd
+    return foort_amd
+
+
+@pytest.mark.parametrize("name", sorted(PARITY_CASES))
+def test_parity_bitexact(name):
+    mk, side, pixels = PARITY_CASES[name]
+    desc = mk()
+    print(f"[parity] case={name} side={side} n={'grid' if pixels is None else len(pixels)}", flush=True)
+    F = _gpu()
+    with F.Context(desc) as ctx:
+        if pixels is None:
+            g = ctx.trace_grid(side)
+            o = O.trace("fm", desc, side)
+        else:
+            g = ctx.trace_pixels(side, pixels)
+            o = O.trace("fm", desc, side, screen_index=pixels)
+    bad = [k for k in FIELDS if not bitwise_equal(g[k], o[k])]
+    if bad:
+        k = bad[0]
+        gv = np.asarray(g[k]).reshape(len(g["term"]), -1)
+        ov = np.asarray(o[k]).reshape(len(o["term"]), -1)
+        i = int(np.nonzero((gv != ov).any(1))[0][0])
+        pytest.fail(f"{name}: fields {bad} differ; first pixel {i}: gpu {gv[i]} oracle {ov[i]}")
+    assert int(g["stats"].steps) == int(o["steps"].sum())
+
+
+def test_camera_matches_oracle():
+    desc = c1(rlog=True)
+    F = _gpu()
+    with F.Context(desc) as ctx:
+        V, g0 = ctx.camera()
+    Vo, g0o = O.camera("fm", desc)
+    assert bitwise_equal(V, Vo) and bitwise_equal(g0, g0o)
+
+
+def test_grid_offset_and_pixel_list_agree():
+    """A grid batch starting at `first` equals the same pixels given as a list."""
+    desc = c1(rlog=True, max_steps=5000)
+    side, first, count = 20, 37, 50
+    F = _gpu()
+    with F.Context(desc) as ctx:
+        a = ctx.trace_grid(side, first, count)
+        idx = np.arange(first, first + count)
+        b = ctx.trace_pixels(side, np.stack([idx // side, idx % side], 1))
+    for k in FIELDS:
+        assert bitwise_equal(a[k], b[k]), k
+
+
+def test_empty_and_single():
+    desc = c1(rlog=True, max_steps=5000)
+    F = _gpu()
+    with F.Context(desc) as ctx:
+        e = ctx.trace_grid(16, 0, 0)
+        assert len(e["term"]) == 0 and e["stats"].steps == 0
+        s = ctx.trace_pixels(16, [[15, 15]])
+    o = O.trace("fm", desc, 16, screen_index=np.array([[15, 15]], np.uint64))
+    for k in FIELDS:
+        assert bitwise_equal(s[k], o[k]), k
+
+
+def test_invalid_arguments_raise():
+    F = _gpu()
+    d = c1()
+    d.integrator = 7
+    with pytest.raises(F.FoortError):
+        F.Context(d)
+    with F.Context(c1(rlog=True)) as ctx:
+        with pytest.raises(F.FoortError):
+            ctx.trace_grid(1)
+        with pytest.raises(F.FoortError):
+            ctx.trace_grid(4, 10, 10)
+        with pytest.raises(F.FoortError):
+            ctx.trace_pixels(4, [[4, 0]])
+
+
+def test_device_entry_point_matches_host():
+    import torch
+    desc = c1(rlog=True, max_steps=5000)
+    side = 24
+    F = _gpu()
+    with F.Context(desc) as ctx:
+        h = ctx.trace_grid(side)
+        n = side * side
+        d = F.device_results(n)
+        st = ctx.trace_device(side, 0, n, d)
+        torch.cuda.synchronize()
+    for k in FIELDS:
+        assert bitwise_equal(d[k].cpu().numpy(), h[k]), k
+    assert st.steps == h["stats"].steps and st.geodesics == n
