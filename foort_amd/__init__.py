@@ -21,6 +21,8 @@ _lib = None
 EXPORTS = [
     "foort_desc_init", "foort_desc_size", "foort_gpu_open", "foort_gpu_close", "foort_gpu_last_error",
     "foort_gpu_trace_grid", "foort_gpu_trace_pixels", "foort_gpu_trace_device", "foort_gpu_camera",
+    "foort_gpu_device_alloc", "foort_gpu_device_free", "foort_gpu_copy_to_device", "foort_gpu_copy_to_host",
+    "foort_gpu_synchronize",
 ]
 
 
@@ -47,6 +49,10 @@ def lib():
         L.foort_gpu_trace_device.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_void_p,
                                              C.POINTER(GeodesicResults), C.c_void_p, C.POINTER(Stats)]
         L.foort_gpu_camera.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.foort_gpu_device_alloc.argtypes = [C.c_size_t, C.POINTER(C.c_void_p)]
+        L.foort_gpu_device_free.argtypes = [C.c_void_p]
+        L.foort_gpu_copy_to_device.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+        L.foort_gpu_copy_to_host.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
         if L.foort_desc_size() != C.sizeof(RunDesc):
             raise FoortError(f"foort_run_desc layout mismatch: library {L.foort_desc_size()} bytes, "
                              f"binding {C.sizeof(RunDesc)}")
@@ -129,12 +135,13 @@ class Context:
         return out
 
     def trace_device(self, side, first, count, results, screen_index=None, stream=None, stats=True):
-        """Device-resident batch: `results` maps field name -> torch CUDA tensor (or None)."""
-        res = _results_struct(results, lambda t: C.c_void_p(t.data_ptr()))
-        si = C.c_void_p(screen_index.data_ptr()) if screen_index is not None else None
+        """Device-resident batch: `results` maps field name -> DeviceArray (or None);
+        `screen_index` is a DeviceArray of (row, column) pairs or None for grid indices."""
+        res = _results_struct(results, lambda t: C.c_void_p(t.ptr))
+        si = C.c_void_p(screen_index.ptr) if screen_index is not None else None
         st = Stats() if stats else None
-        s = C.c_void_p(stream.cuda_stream) if stream is not None else None
-        self._check(self._L.foort_gpu_trace_device(self._h, side, first, count, si, C.byref(res), s,
+        self._check(self._L.foort_gpu_trace_device(self._h, side, first, count, si, C.byref(res),
+                                                   C.c_void_p(stream) if stream else None,
                                                    C.byref(st) if st is not None else None),
                     "foort_gpu_trace_device")
         return st
@@ -146,12 +153,55 @@ class Context:
         return V.reshape(4, 4), g0
 
 
-def device_results(count, device="cuda", fields=("term", "quadrant", "equat_passes", "closest_radius",
-                                                  "final_pos", "final_vel", "lambda", "steps")):
-    """Allocate device result tensors for trace_device."""
-    import torch
-    shapes = {"term": ((count,), torch.int32), "quadrant": ((count,), torch.int32),
-              "equat_passes": ((count,), torch.int32), "closest_radius": ((count,), torch.float64),
-              "final_pos": ((count, 4), torch.float64), "final_vel": ((count, 4), torch.float64),
-              "lambda": ((count,), torch.float64), "steps": ((count,), torch.int32)}
-    return {k: torch.empty(*shapes[k][0], dtype=shapes[k][1], device=device) for k in fields}
+class DeviceArray:
+    """A device allocation made through the C ABI (foort_gpu_device_alloc)."""
+
+    def __init__(self, shape, dtype):
+        self.shape = tuple(shape) if hasattr(shape, "__len__") else (shape,)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        L = lib()
+        p = C.c_void_p()
+        if L.foort_gpu_device_alloc(self.nbytes, C.byref(p)) != 0:
+            raise FoortError(L.foort_gpu_last_error(None).decode())
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        if lib().foort_gpu_copy_to_device(C.c_void_p(d.ptr), a.ctypes.data_as(C.c_void_p), d.nbytes) != 0:
+            raise FoortError(lib().foort_gpu_last_error(None).decode())
+        return d
+
+    def numpy(self):
+        out = np.empty(self.shape, self.dtype)
+        if lib().foort_gpu_copy_to_host(out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr), self.nbytes) != 0:
+            raise FoortError(lib().foort_gpu_last_error(None).decode())
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().foort_gpu_device_free(C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+RESULT_LAYOUT = {"term": ((), np.int32), "quadrant": ((), np.int32), "equat_passes": ((), np.int32),
+                 "closest_radius": ((), np.float64), "final_pos": ((4,), np.float64),
+                 "final_vel": ((4,), np.float64), "lambda": ((), np.float64), "steps": ((), np.uint32)}
+
+
+def device_results(count, fields=tuple(RESULT_LAYOUT)):
+    """Allocate device result arrays for Context.trace_device."""
+    return {k: DeviceArray((count,) + RESULT_LAYOUT[k][0], RESULT_LAYOUT[k][1]) for k in fields}
+
+
+def synchronize():
+    if lib().foort_gpu_synchronize() != 0:
+        raise FoortError(lib().foort_gpu_last_error(None).decode())

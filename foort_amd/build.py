@@ -14,14 +14,16 @@ REPO = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libfoort_gpu.so")
 
-HIPCC_FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
-    "-ffp-contract=off", "-fno-fast-math",
+COMMON = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-fno-fast-math",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I" + os.path.join(REPO, "include"), "-I" + CSRC,
 ]
 
-SOURCES = ["foort_gpu.hip"]
+# (source, contraction): the finite-difference unit must round exactly like the
+# CPU oracle; the analytic unit may fuse multiply-adds.
+SOURCES = [("foort_gpu.hip", "off"), ("foort_gpu_an.hip", "fast")]
+BUILD_DIR = os.path.join(PKG, "_build")
 
 
 def hipcc():
@@ -31,11 +33,26 @@ def hipcc():
     return exe
 
 
-def build_gpu(extra=(), out=LIB, verbose=False):
-    cmd = [hipcc()] + HIPCC_FLAGS + list(extra) + [os.path.join(CSRC, s) for s in SOURCES] + ["-o", out + ".tmp"]
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
+
+
+def build_gpu(extra=(), out=LIB, verbose=False):
+    from concurrent.futures import ThreadPoolExecutor
+    os.makedirs(BUILD_DIR, exist_ok=True)
+    jobs = []
+    objs = []
+    for src, contract in SOURCES:
+        obj = os.path.join(BUILD_DIR, os.path.splitext(src)[0] + ".o")
+        objs.append(obj)
+        jobs.append([hipcc()] + COMMON + ["-ffp-contract=" + contract] + list(extra)
+                    + ["-c", os.path.join(CSRC, src), "-o", obj])
+    with ThreadPoolExecutor(len(jobs)) as ex:
+        for f in [ex.submit(_run, j, verbose) for j in jobs]:
+            f.result()
+    _run([hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC"] + objs + ["-o", out + ".tmp"], verbose)
     os.replace(out + ".tmp", out)
     return out
 

@@ -13,120 +13,36 @@
 #include <string>
 
 #include "foort_gpu.h"
-#include "foort_path.hpp"
+#include "foort_kernels.hpp"
 
 #pragma clang fp contract(off)
 
+using foort::DevResults;
 using foort::GeoState;
+using foort::KernelPair;
+using foort::LaneSlots;
 using foort::RunConst;
 
 namespace {
 
-struct DevResults {
-    int32_t *term;
-    int32_t *quadrant;
-    int32_t *passes;
-    double *closest;
-    double *pos;
-    double *vel;
-    double *lambda;
-    uint32_t *steps;
-};
-
-__device__ __forceinline__ void store_result(const DevResults &out, uint64_t i, const GeoState &g)
+// Reset the lane slots and counters of the persistent schedule (a kernel, not
+// a memset, so the sequence can be captured in a graph).
+__global__ void __launch_bounds__(256) reset_lanes(foort::LaneSlots S, unsigned long long *total_steps)
 {
-    if (out.term)
-        out.term[i] = g.term;
-    if (out.quadrant)
-        out.quadrant[i] = g.quadrant;
-    if (out.passes)
-        out.passes[i] = g.passes;
-    if (out.closest)
-        out.closest[i] = g.closest;
-    if (out.lambda)
-        out.lambda[i] = g.lambda;
-    if (out.steps)
-        out.steps[i] = g.steps;
-    if (out.pos) {
-        double2 *p = reinterpret_cast<double2 *>(out.pos + 4 * i);
-        p[0] = make_double2(g.pos[0], g.pos[1]);
-        p[1] = make_double2(g.pos[2], g.pos[3]);
-    }
-    if (out.vel) {
-        double2 *p = reinterpret_cast<double2 *>(out.vel + 4 * i);
-        p[0] = make_double2(g.vel[0], g.vel[1]);
-        p[1] = make_double2(g.vel[2], g.vel[3]);
+    const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l < S.L)
+        S.pix[l] = -1;
+    if (l == 0) {
+        *S.next = 0;
+        *S.done = 0;
+        *total_steps = 0;
     }
 }
 
-// Unit screen point of a pixel, as the meshes compute it (FOORT/Mesh.cpp:42-49, :204-205)
-__device__ __forceinline__ void pixel_unit(uint64_t side, uint64_t first, const uint64_t *si, uint64_t i,
-                                           double &ux, double &uy)
+__global__ void __launch_bounds__(64) zero_counters(unsigned long long *p, int n)
 {
-    uint64_t row, col;
-    if (si) {
-        row = si[2 * i];
-        col = si[2 * i + 1];
-    } else {
-        const uint64_t index = first + i;
-        row = index / side;
-        col = index - row * side;
-    }
-    const double den = static_cast<double>(side - 1);
-    ux = static_cast<double>(row) * 1.0 / den;
-    uy = static_cast<double>(col) * 1.0 / den;
-}
-
-__device__ __forceinline__ void add_steps(unsigned long long *total, unsigned long long mine)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1)
-        mine += __shfl_xor(mine, off);
-    if ((threadIdx.x & 63) == 0)
-        atomicAdd(total, mine);
-}
-
-// One lane per pixel, integrated to termination (the drop-in for the
-// per-thread loop of FOORT/Main.cpp:373-400).
-template <int METRIC, int INTEG>
-__global__ void __launch_bounds__(256) trace_oneshot(const RunConst R, uint64_t side, uint64_t first, uint64_t count,
-                                                     const uint64_t *__restrict__ si, DevResults out,
-                                                     unsigned long long *__restrict__ total_steps)
-{
-    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    unsigned long long mine = 0;
-    if (i < count) {
-        double ux, uy;
-        pixel_unit(side, first, si, i, ux, uy);
-        GeoState g;
-        foort::geo_init(R, ux, uy, g);
-        while (g.term == FOORT_TERM_CONTINUE)
-            foort::geo_update<METRIC, INTEG>(R, g);
-        store_result(out, i, g);
-        mine = g.steps;
-    }
-    add_steps(total_steps, mine);
-}
-
-using KernelFn = void (*)(const RunConst, uint64_t, uint64_t, uint64_t, const uint64_t *, DevResults,
-                          unsigned long long *);
-
-template <int M>
-KernelFn pick_integ(int integ)
-{
-    return integ == FOORT_INTEGRATOR_VERLET ? trace_oneshot<M, FOORT_INTEGRATOR_VERLET>
-                                            : trace_oneshot<M, FOORT_INTEGRATOR_RK4>;
-}
-
-KernelFn pick_kernel(int metric, int integ)
-{
-    switch (metric) {
-    case FOORT_METRIC_KERR: return pick_integ<FOORT_METRIC_KERR>(integ);
-    case FOORT_METRIC_FLATSPACE: return pick_integ<FOORT_METRIC_FLATSPACE>(integ);
-    case FOORT_METRIC_RASHEEDLARSEN: return pick_integ<FOORT_METRIC_RASHEEDLARSEN>(integ);
-    case FOORT_METRIC_JOHANNSEN: return pick_integ<FOORT_METRIC_JOHANNSEN>(integ);
-    default: return pick_integ<FOORT_METRIC_MANKONOVIKOV>(integ);
-    }
+    if (static_cast<int>(threadIdx.x) < n)
+        p[threadIdx.x] = 0;
 }
 
 thread_local std::string g_open_error;
@@ -283,13 +199,20 @@ void build_camera(int metric, RunConst &R, const double cam[4])
 struct foort_gpu_ctx {
     foort_run_desc desc;
     RunConst rc;
-    KernelFn kernel;
+    KernelPair kernels{};
     int device = 0;
+    int num_cu = 0;
     std::string err;
     // grow-only device staging for the host entry points
     void *d_buf = nullptr;
     size_t d_cap = 0;
+    unsigned long long *d_counters = nullptr; // [0] steps, [1] next pixel, [2] pixels done
     unsigned long long *d_steps = nullptr;
+    // persistent schedule
+    LaneSlots slots{};
+    void *d_slots = nullptr;
+    uint32_t budget = 0;
+    uint64_t last_launches = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
@@ -331,20 +254,81 @@ int ensure_buffer(foort_gpu_ctx *ctx, size_t bytes)
     return FOORT_OK;
 }
 
+int setup_persistent(foort_gpu_ctx *ctx)
+{
+    if (ctx->d_slots)
+        return FOORT_OK;
+    uint32_t lanes_per_cu = ctx->desc.lanes_per_cu;
+    if (lanes_per_cu == 0) {
+        int blocks = 0;
+        HIP_TRY(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                         &blocks, reinterpret_cast<const void *>(ctx->kernels.persistent), 256, 0));
+        lanes_per_cu = static_cast<uint32_t>(std::max(1, blocks)) * 256u;
+    }
+    uint64_t L = static_cast<uint64_t>(ctx->num_cu) * lanes_per_cu;
+    L = (L + 255) / 256 * 256;
+    if (L == 0 || L > (1u << 24))
+        return fail(ctx, FOORT_EINVAL, "persistent lane count out of range");
+    const size_t bytes = L * (foort::kSlotD * 8 + foort::kSlotU * 4 + 8);
+    hipError_t e = hipMalloc(&ctx->d_slots, bytes);
+    if (e != hipSuccess)
+        return fail(ctx, FOORT_ENOMEM, std::string("hipMalloc of lane slots: ") + hipGetErrorString(e));
+    char *b = static_cast<char *>(ctx->d_slots);
+    ctx->slots.d = reinterpret_cast<double *>(b);
+    ctx->slots.pix = reinterpret_cast<long long *>(b + L * foort::kSlotD * 8);
+    ctx->slots.u = reinterpret_cast<uint32_t *>(b + L * (foort::kSlotD * 8 + 8));
+    ctx->slots.next = ctx->d_counters + 1;
+    ctx->slots.done = ctx->d_counters + 2;
+    ctx->slots.L = static_cast<uint32_t>(L);
+    return FOORT_OK;
+}
+
 int launch(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, uint64_t count, const uint64_t *si,
            const DevResults &out, hipStream_t stream)
 {
+    ctx->last_launches = 0;
+    if (ctx->desc.schedule == FOORT_SCHEDULE_ONESHOT) {
+        hipLaunchKernelGGL(zero_counters, dim3(1), dim3(64), 0, stream, ctx->d_counters, 3);
+        HIP_TRY(ctx, hipGetLastError());
+        if (count == 0)
+            return FOORT_OK;
+        const int block = ctx->desc.block_size > 0 ? ctx->desc.block_size : 256;
+        if (block > 256 || block % 64 != 0)
+            return fail(ctx, FOORT_EINVAL, "block_size must be a multiple of 64 and at most 256");
+        const uint64_t grid = (count + block - 1) / block;
+        if (grid > 0x7fffffffull)
+            return fail(ctx, FOORT_EINVAL, "batch too large");
+        hipLaunchKernelGGL(ctx->kernels.oneshot, dim3(static_cast<unsigned>(grid)), dim3(block), 0, stream, ctx->rc,
+                           side, first, count, si, out, ctx->d_steps, ctx->slots, 0u);
+        HIP_TRY(ctx, hipGetLastError());
+        ctx->last_launches = 1;
+        return FOORT_OK;
+    }
+    int rc = setup_persistent(ctx);
+    if (rc)
+        return rc;
+    const LaneSlots &S = ctx->slots;
+    hipLaunchKernelGGL(reset_lanes, dim3(S.L / 256), dim3(256), 0, stream, S, ctx->d_steps);
+    HIP_TRY(ctx, hipGetLastError());
     if (count == 0)
         return FOORT_OK;
-    const int block = ctx->desc.block_size > 0 ? ctx->desc.block_size : 256;
-    if (block > 256 || block % 64 != 0)
-        return fail(ctx, FOORT_EINVAL, "block_size must be a multiple of 64 and at most 256");
-    const uint64_t grid = (count + block - 1) / block;
-    if (grid > 0x7fffffffull)
-        return fail(ctx, FOORT_EINVAL, "batch too large");
-    hipLaunchKernelGGL(ctx->kernel, dim3(static_cast<unsigned>(grid)), dim3(block), 0, stream, ctx->rc, side, first,
-                       count, si, out, ctx->d_steps);
-    HIP_TRY(ctx, hipGetLastError());
+    // every launch advances each busy lane by `budget` steps; no geodesic is longer than
+    // step_cap, so the batch needs at most (count / L + 1) * (step_cap / budget + 1) launches
+    const uint64_t max_launches =
+        (count / S.L + 2) * (static_cast<uint64_t>(ctx->rc.step_cap) / ctx->budget + 2) + 16;
+    unsigned long long done = 0;
+    uint64_t n = 0;
+    while (done < count) {
+        if (n >= max_launches)
+            return fail(ctx, FOORT_EDEVICE, "persistent schedule did not finish within its launch bound");
+        hipLaunchKernelGGL(ctx->kernels.persistent, dim3(S.L / 256), dim3(256), 0, stream, ctx->rc, side, first,
+                           count, si, out, ctx->d_steps, S, ctx->budget);
+        HIP_TRY(ctx, hipGetLastError());
+        ++n;
+        HIP_TRY(ctx, hipMemcpyAsync(&done, S.done, sizeof(done), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(ctx, hipStreamSynchronize(stream));
+    }
+    ctx->last_launches = n;
     return FOORT_OK;
 }
 
@@ -402,7 +386,6 @@ int trace_host(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, uint64_t count
     d.lambda = out->lambda ? reinterpret_cast<double *>(base + o_lam) : nullptr;
     d.pos = out->final_pos ? reinterpret_cast<double *>(base + o_pos) : nullptr;
     d.vel = out->final_vel ? reinterpret_cast<double *>(base + o_vel) : nullptr;
-    HIP_TRY(ctx, hipMemset(ctx->d_steps, 0, sizeof(unsigned long long)));
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, nullptr));
     rc = launch(ctx, side, first, count, d_si, d, nullptr);
     if (rc)
@@ -429,7 +412,7 @@ int trace_host(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, uint64_t count
         HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
         stats->geodesics = count;
         stats->steps = steps;
-        stats->launches = 1;
+        stats->launches = ctx->last_launches;
         stats->device_ms = ms;
     }
     return FOORT_OK;
@@ -489,12 +472,14 @@ int foort_gpu_open(const foort_run_desc *desc, foort_gpu_ctx **out)
     const foort_run_desc &d = *desc;
     if (d.integrator != FOORT_INTEGRATOR_RK4 && d.integrator != FOORT_INTEGRATOR_VERLET)
         return fail(nullptr, FOORT_EINVAL, "unknown integrator");
-    if (d.christoffel != FOORT_CHRISTOFFEL_FD)
-        return fail(nullptr, FOORT_EINVAL, "only the finite-difference Christoffel policy is built");
-    if (d.precision != FOORT_PRECISION_FP64)
-        return fail(nullptr, FOORT_EINVAL, "only FP64 is built");
-    if (d.schedule != FOORT_SCHEDULE_ONESHOT)
-        return fail(nullptr, FOORT_EINVAL, "only the one-shot schedule is built");
+    if (d.christoffel != FOORT_CHRISTOFFEL_FD && d.christoffel != FOORT_CHRISTOFFEL_ANALYTIC)
+        return fail(nullptr, FOORT_EINVAL, "unknown Christoffel policy");
+    if (d.precision != FOORT_PRECISION_FP64 && d.precision != FOORT_PRECISION_MIXED)
+        return fail(nullptr, FOORT_EINVAL, "unknown precision");
+    if (d.precision == FOORT_PRECISION_MIXED && d.christoffel != FOORT_CHRISTOFFEL_ANALYTIC)
+        return fail(nullptr, FOORT_EINVAL, "mixed precision needs the analytic Christoffel policy");
+    if (d.schedule != FOORT_SCHEDULE_ONESHOT && d.schedule != FOORT_SCHEDULE_PERSISTENT)
+        return fail(nullptr, FOORT_EINVAL, "unknown schedule");
     if (d.diag_flags & FOORT_DIAGFLAG_GEODESICPOSITION)
         return fail(nullptr, FOORT_EINVAL, "GeodesicPosition is not supported by the GPU path");
     if (!(d.derivative_h > 0))
@@ -548,12 +533,21 @@ int foort_gpu_open(const foort_run_desc *desc, foort_gpu_ctx **out)
     R.size_y = d.screen_size[1];
     R.center_x = d.screen_center[0];
     R.center_y = d.screen_center[1];
+    R.mixed_u = R.M.rlog ? std::log(d.mixed_switch_radius) : d.mixed_switch_radius;
+    R.mixed_pole = d.mixed_pole_margin;
     build_camera(d.metric, R, d.camera_position);
-    ctx->kernel = pick_kernel(d.metric, d.integrator);
+    ctx->kernels = d.christoffel == FOORT_CHRISTOFFEL_FD
+                       ? foort::pick_policy<foort::RhsFD>(d.metric, d.integrator)
+                       : foort::pick_analytic(d.metric, d.integrator, static_cast<int>(d.precision));
+    ctx->budget = d.steps_per_launch ? d.steps_per_launch : 8192u;
 
     hipError_t e = hipGetDevice(&ctx->device);
     if (e == hipSuccess)
-        e = hipMalloc(reinterpret_cast<void **>(&ctx->d_steps), sizeof(unsigned long long));
+        e = hipDeviceGetAttribute(&ctx->num_cu, hipDeviceAttributeMultiprocessorCount, ctx->device);
+    if (e == hipSuccess)
+        e = hipMalloc(reinterpret_cast<void **>(&ctx->d_counters), 4 * sizeof(unsigned long long));
+    if (e == hipSuccess)
+        ctx->d_steps = ctx->d_counters;
     if (e == hipSuccess)
         e = hipEventCreate(&ctx->ev0);
     if (e == hipSuccess)
@@ -573,8 +567,10 @@ void foort_gpu_close(foort_gpu_ctx *ctx)
         return;
     if (ctx->d_buf)
         (void)hipFree(ctx->d_buf);
-    if (ctx->d_steps)
-        (void)hipFree(ctx->d_steps);
+    if (ctx->d_counters)
+        (void)hipFree(ctx->d_counters);
+    if (ctx->d_slots)
+        (void)hipFree(ctx->d_slots);
     if (ctx->ev0)
         (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1)
@@ -628,10 +624,8 @@ int foort_gpu_trace_device(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, ui
     hipStream_t s = static_cast<hipStream_t>(stream);
     DevResults d{out->term, out->quadrant, out->equat_passes, out->closest_radius,
                  out->final_pos, out->final_vel, out->lambda, out->steps};
-    if (stats) {
-        HIP_TRY(ctx, hipMemsetAsync(ctx->d_steps, 0, sizeof(unsigned long long), s));
+    if (stats)
         HIP_TRY(ctx, hipEventRecord(ctx->ev0, s));
-    }
     rc = launch(ctx, side, first, count, screen_index, d, s);
     if (rc)
         return rc;
@@ -644,10 +638,44 @@ int foort_gpu_trace_device(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, ui
         HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
         stats->geodesics = count;
         stats->steps = steps;
-        stats->launches = 1;
+        stats->launches = ctx->last_launches;
         stats->device_ms = ms;
     }
     return FOORT_OK;
+}
+
+int foort_gpu_device_alloc(size_t bytes, void **ptr)
+{
+    if (!ptr)
+        return fail(nullptr, FOORT_EINVAL, "NULL pointer");
+    hipError_t e = hipMalloc(ptr, bytes ? bytes : 1);
+    if (e != hipSuccess)
+        return fail(nullptr, FOORT_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return FOORT_OK;
+}
+
+int foort_gpu_device_free(void *ptr)
+{
+    hipError_t e = hipFree(ptr);
+    return e == hipSuccess ? FOORT_OK : hip_fail(nullptr, e, "hipFree");
+}
+
+int foort_gpu_copy_to_device(void *dst, const void *src, size_t bytes)
+{
+    hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+    return e == hipSuccess ? FOORT_OK : hip_fail(nullptr, e, "hipMemcpy to device");
+}
+
+int foort_gpu_copy_to_host(void *dst, const void *src, size_t bytes)
+{
+    hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+    return e == hipSuccess ? FOORT_OK : hip_fail(nullptr, e, "hipMemcpy to host");
+}
+
+int foort_gpu_synchronize(void)
+{
+    hipError_t e = hipDeviceSynchronize();
+    return e == hipSuccess ? FOORT_OK : hip_fail(nullptr, e, "hipDeviceSynchronize");
 }
 
 int foort_gpu_camera(const foort_gpu_ctx *ctx, double vielbein[16], double row0[4])

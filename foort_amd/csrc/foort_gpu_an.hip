@@ -1,0 +1,17 @@
+// foort_gpu_an.hip -- kernels with analytic Christoffel symbols (exact
+// derivatives of the reference metrics by forward-mode dual numbers), in
+// binary64 or mixed binary32/binary64.  These are not bit-compatible with the
+// reference's finite differences, so this unit is compiled with contraction
+// allowed (fused multiply-add).
+#include "foort_kernels.hpp"
+
+namespace foort {
+
+KernelPair pick_analytic(int metric, int integ, int precision)
+{
+    if (precision == FOORT_PRECISION_MIXED)
+        return pick_policy<RhsMixed>(metric, integ);
+    return pick_policy<RhsAn64>(metric, integ);
+}
+
+} // namespace foort

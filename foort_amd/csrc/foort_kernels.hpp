@@ -1,0 +1,260 @@
+// foort_kernels.hpp -- kernel templates over a right-hand-side policy and an
+// integrator.  Instantiated by foort_gpu.hip (finite differences, contraction
+// off: bit-exact with the CPU oracle) and foort_gpu_an.hip (analytic and
+// mixed-precision right-hand sides).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "foort_path.hpp"
+
+namespace foort {
+
+struct DevResults {
+    int32_t *term;
+    int32_t *quadrant;
+    int32_t *passes;
+    double *closest;
+    double *pos;
+    double *vel;
+    double *lambda;
+    uint32_t *steps;
+};
+
+// Per-lane state of the persistent schedule, structure of arrays over L lanes,
+// kept in HBM between launches.
+constexpr int kSlotD = 11; // pos[4] vel[4] lambda prevtheta closest
+constexpr int kSlotU = 11; // term steps hc bc tc thc tsteps epc crc quadrant passes
+struct LaneSlots {
+    double *d;                 // kSlotD * L
+    uint32_t *u;               // kSlotU * L
+    long long *pix;            // L; -1 = idle
+    unsigned long long *next;  // next pixel of the batch to hand out
+    unsigned long long *done;  // pixels finished
+    uint32_t L;
+};
+
+__device__ __forceinline__ void store_result(const DevResults &out, uint64_t i, const GeoState &g)
+{
+    if (out.term)
+        out.term[i] = g.term;
+    if (out.quadrant)
+        out.quadrant[i] = g.quadrant;
+    if (out.passes)
+        out.passes[i] = g.passes;
+    if (out.closest)
+        out.closest[i] = g.closest;
+    if (out.lambda)
+        out.lambda[i] = g.lambda;
+    if (out.steps)
+        out.steps[i] = g.steps;
+    if (out.pos) {
+        double2 *p = reinterpret_cast<double2 *>(out.pos + 4 * i);
+        p[0] = make_double2(g.pos[0], g.pos[1]);
+        p[1] = make_double2(g.pos[2], g.pos[3]);
+    }
+    if (out.vel) {
+        double2 *p = reinterpret_cast<double2 *>(out.vel + 4 * i);
+        p[0] = make_double2(g.vel[0], g.vel[1]);
+        p[1] = make_double2(g.vel[2], g.vel[3]);
+    }
+}
+
+// Unit screen point of a pixel, as the meshes compute it (FOORT/Mesh.cpp:42-49, :204-205)
+__device__ __forceinline__ void pixel_unit(uint64_t side, uint64_t first, const uint64_t *si, uint64_t i,
+                                           double &ux, double &uy)
+{
+    uint64_t row, col;
+    if (si) {
+        row = si[2 * i];
+        col = si[2 * i + 1];
+    } else {
+        const uint64_t index = first + i;
+        row = index / side;
+        col = index - row * side;
+    }
+    const double den = static_cast<double>(side - 1);
+    ux = static_cast<double>(row) * 1.0 / den;
+    uy = static_cast<double>(col) * 1.0 / den;
+}
+
+__device__ __forceinline__ void add_steps(unsigned long long *total, unsigned long long mine)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        mine += __shfl_xor(mine, off);
+    if ((threadIdx.x & 63) == 0)
+        atomicAdd(total, mine);
+}
+
+// One lane per pixel, integrated to termination: the drop-in for the
+// per-thread loop of FOORT/Main.cpp:373-400.
+template <class RHS, int INTEG>
+__global__ void __launch_bounds__(256) trace_oneshot(const RunConst R, uint64_t side, uint64_t first, uint64_t count,
+                                                     const uint64_t *__restrict__ si, DevResults out,
+                                                     unsigned long long *__restrict__ total_steps, LaneSlots,
+                                                     uint32_t)
+{
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    unsigned long long mine = 0;
+    if (i < count) {
+        double ux, uy;
+        pixel_unit(side, first, si, i, ux, uy);
+        GeoState g;
+        geo_init(R, ux, uy, g);
+        while (g.term == FOORT_TERM_CONTINUE)
+            geo_update<RHS, INTEG>(R, g);
+        store_result(out, i, g);
+        mine = g.steps;
+    }
+    add_steps(total_steps, mine);
+}
+
+__device__ __forceinline__ void slot_load(const LaneSlots &S, uint32_t l, GeoState &g)
+{
+    const uint32_t L = S.L;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        g.pos[k] = S.d[k * L + l];
+        g.vel[k] = S.d[(4 + k) * L + l];
+    }
+    g.lambda = S.d[8 * L + l];
+    g.prevtheta = S.d[9 * L + l];
+    g.closest = S.d[10 * L + l];
+    g.term = static_cast<int32_t>(S.u[0 * L + l]);
+    g.steps = S.u[1 * L + l];
+    g.hc = S.u[2 * L + l];
+    g.bc = S.u[3 * L + l];
+    g.tc = S.u[4 * L + l];
+    g.thc = S.u[5 * L + l];
+    g.tsteps = S.u[6 * L + l];
+    g.epc = S.u[7 * L + l];
+    g.crc = S.u[8 * L + l];
+    g.quadrant = static_cast<int32_t>(S.u[9 * L + l]);
+    g.passes = static_cast<int32_t>(S.u[10 * L + l]);
+}
+
+__device__ __forceinline__ void slot_save(const LaneSlots &S, uint32_t l, const GeoState &g)
+{
+    const uint32_t L = S.L;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        S.d[k * L + l] = g.pos[k];
+        S.d[(4 + k) * L + l] = g.vel[k];
+    }
+    S.d[8 * L + l] = g.lambda;
+    S.d[9 * L + l] = g.prevtheta;
+    S.d[10 * L + l] = g.closest;
+    S.u[0 * L + l] = static_cast<uint32_t>(g.term);
+    S.u[1 * L + l] = g.steps;
+    S.u[2 * L + l] = g.hc;
+    S.u[3 * L + l] = g.bc;
+    S.u[4 * L + l] = g.tc;
+    S.u[5 * L + l] = g.thc;
+    S.u[6 * L + l] = g.tsteps;
+    S.u[7 * L + l] = g.epc;
+    S.u[8 * L + l] = g.crc;
+    S.u[9 * L + l] = static_cast<uint32_t>(g.quadrant);
+    S.u[10 * L + l] = static_cast<uint32_t>(g.passes);
+}
+
+// Persistent work-stealing schedule.  L resident lanes; a lane whose geodesic
+// terminates writes its result and takes the next pixel of the batch, the
+// lanes of a wavefront claiming their pixels with one atomic (ballot + prefix
+// count).  A launch advances every busy lane by at most `budget` steps and
+// parks the lane state in HBM, so every launch is bounded; a wavefront leaves
+// as soon as none of its lanes has work.  The host relaunches until `done`
+// reaches the batch size.
+template <class RHS, int INTEG>
+__global__ void __launch_bounds__(256) trace_persistent(const RunConst R, uint64_t side, uint64_t first,
+                                                        uint64_t count, const uint64_t *__restrict__ si,
+                                                        DevResults out, unsigned long long *__restrict__ total_steps,
+                                                        LaneSlots S, uint32_t budget)
+{
+    const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x; // grid covers exactly S.L lanes
+    const uint32_t lane_id = threadIdx.x & 63;
+    GeoState g;
+    long long pix = S.pix[l];
+    if (pix >= 0)
+        slot_load(S, l, g);
+    bool exhausted = false;
+    unsigned long long mine = 0;
+    for (uint32_t it = 0; it < budget; ++it) {
+        const unsigned long long need = __ballot(pix < 0 && !exhausted);
+        if (need) {
+            const int leader = __ffsll(static_cast<long long>(need)) - 1;
+            unsigned long long base = 0;
+            if (static_cast<int>(lane_id) == leader)
+                base = atomicAdd(S.next, static_cast<unsigned long long>(__popcll(need)));
+            base = __shfl(base, leader);
+            if (pix < 0 && !exhausted) {
+                const unsigned long long k = base + __popcll(need & ((1ull << lane_id) - 1ull));
+                if (k < count) {
+                    pix = static_cast<long long>(k);
+                    double ux, uy;
+                    pixel_unit(side, first, si, k, ux, uy);
+                    geo_init(R, ux, uy, g);
+                } else {
+                    exhausted = true;
+                }
+            }
+        }
+        if (!__any(pix >= 0))
+            break;
+        bool finished = false;
+        if (pix >= 0) {
+            geo_update<RHS, INTEG>(R, g);
+            ++mine;
+            if (g.term != FOORT_TERM_CONTINUE) {
+                store_result(out, static_cast<uint64_t>(pix), g);
+                pix = -1;
+                finished = true;
+            }
+        }
+        const unsigned long long fin = __ballot(finished);
+        if (fin && static_cast<int>(lane_id) == __ffsll(static_cast<long long>(fin)) - 1)
+            atomicAdd(S.done, static_cast<unsigned long long>(__popcll(fin)));
+    }
+    S.pix[l] = pix;
+    if (pix >= 0)
+        slot_save(S, l, g);
+    add_steps(total_steps, mine);
+}
+
+using KernelFn = void (*)(const RunConst, uint64_t, uint64_t, uint64_t, const uint64_t *, DevResults,
+                          unsigned long long *, LaneSlots, uint32_t);
+
+struct KernelPair {
+    KernelFn oneshot;
+    KernelFn persistent;
+};
+
+template <class RHS, int INTEG>
+KernelPair kernel_pair()
+{
+    return KernelPair{trace_oneshot<RHS, INTEG>, trace_persistent<RHS, INTEG>};
+}
+
+template <template <int> class POLICY, int M>
+KernelPair pick_integ(int integ)
+{
+    return integ == FOORT_INTEGRATOR_VERLET ? kernel_pair<POLICY<M>, FOORT_INTEGRATOR_VERLET>()
+                                            : kernel_pair<POLICY<M>, FOORT_INTEGRATOR_RK4>();
+}
+
+template <template <int> class POLICY>
+KernelPair pick_policy(int metric, int integ)
+{
+    switch (metric) {
+    case FOORT_METRIC_KERR: return pick_integ<POLICY, FOORT_METRIC_KERR>(integ);
+    case FOORT_METRIC_FLATSPACE: return pick_integ<POLICY, FOORT_METRIC_FLATSPACE>(integ);
+    case FOORT_METRIC_RASHEEDLARSEN: return pick_integ<POLICY, FOORT_METRIC_RASHEEDLARSEN>(integ);
+    case FOORT_METRIC_JOHANNSEN: return pick_integ<POLICY, FOORT_METRIC_JOHANNSEN>(integ);
+    default: return pick_integ<POLICY, FOORT_METRIC_MANKONOVIKOV>(integ);
+    }
+}
+
+// defined in foort_gpu_an.hip (analytic binary64 and mixed right-hand sides)
+KernelPair pick_analytic(int metric, int integ, int precision);
+
+} // namespace foort

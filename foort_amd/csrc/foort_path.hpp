@@ -18,6 +18,7 @@
 
 #include <stdint.h>
 
+#include "dual.hpp"
 #include "fm_math.h"
 #include "foort_gpu.h"
 
@@ -32,6 +33,70 @@
 namespace foort {
 
 constexpr double kPi = 3.1415926535; // FOORT/Geometry.h:44
+
+// ---------------------------------------------------------------------------
+// Elementary functions for the scalar types of the path.  double is the
+// reference-exact type (the portable fm_math.h functions); D2<S> carries
+// derivatives (analytic Christoffel symbols), S = double or float.
+// ---------------------------------------------------------------------------
+FM_HD_INLINE double fexp(double x) { return fm_exp(x); }
+FM_HD_INLINE double fsqrt(double x) { return __builtin_sqrt(x); }
+FM_HD_INLINE double fpow2(double x) { return fm_pow2(x); }
+FM_HD_INLINE double fpow3(double x) { return fm_pow3(x); }
+FM_HD_INLINE double fpow4(double x) { return fm_pow4(x); }
+FM_HD_INLINE double fpow8(double x) { return fm_pow8(x); }
+
+FM_HD_INLINE float sexp(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __expf(x);
+#else
+    return __builtin_expf(x);
+#endif
+}
+FM_HD_INLINE double sexp(double x) { return fm_exp(x); }
+FM_HD_INLINE float ssqrt(float x) { return __builtin_sqrtf(x); }
+FM_HD_INLINE double ssqrt(double x) { return __builtin_sqrt(x); }
+FM_HD_INLINE void ssincos(double x, double *s, double *c) { fm_sincos(x, s, c); }
+FM_HD_INLINE void ssincos(double x, float *s, float *c)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    __sincosf(static_cast<float>(x), s, c);
+#else
+    *s = __builtin_sinf(static_cast<float>(x));
+    *c = __builtin_cosf(static_cast<float>(x));
+#endif
+}
+
+template <class S>
+FM_HD_INLINE D2<S> fexp(const D2<S> &a)
+{
+    const S e = sexp(a.v);
+    return D2<S>(e, e * a.du, e * a.dt);
+}
+template <class S>
+FM_HD_INLINE D2<S> fsqrt(const D2<S> &a)
+{
+    const S r = ssqrt(a.v);
+    const S k = S(0.5) / r;
+    return D2<S>(r, a.du * k, a.dt * k);
+}
+template <class S>
+FM_HD_INLINE D2<S> fpow2(const D2<S> &a) { return a * a; }
+template <class S>
+FM_HD_INLINE D2<S> fpow3(const D2<S> &a) { return a * a * a; }
+template <class S>
+FM_HD_INLINE D2<S> fpow4(const D2<S> &a)
+{
+    const D2<S> b = a * a;
+    return b * b;
+}
+template <class S>
+FM_HD_INLINE D2<S> fpow8(const D2<S> &a)
+{
+    const D2<S> b = fpow4(a);
+    return b * b;
+}
 
 // ---------------------------------------------------------------------------
 // Run constants (filled on the host by foort_gpu_open, passed by value)
@@ -69,231 +134,244 @@ struct RunConst {
     double cam_t, cam_theta, cam_phi;
     double sin_theta0;     // sin(camera theta)
     double size_x, size_y, center_x, center_y;
+    // mixed precision: binary32 right-hand side where u > mixed_u and theta is
+    // farther than mixed_pole from both poles
+    double mixed_u, mixed_pole;
 };
 
 // ---------------------------------------------------------------------------
 // Metrics: g_dd and g^uu at (u, theta) given sin/cos of theta.  u is the
 // radial coordinate (r, or log r with RLogScale).  Output order g00 g03 g11 g22 g33.
 // ---------------------------------------------------------------------------
-struct G5 {
-    double g[5];
-};
-
 FM_HD_INLINE double true_r(const MetricParams &M, double u)
 {
     return M.rlog ? fm_exp(u) : u;
 }
 
 // FOORT/Metric.cpp:131-159
-FM_HD_INLINE void kerr_dd(const MetricParams &M, double u, double sint, double cost, double *g)
+template <class T>
+FM_HD_INLINE void kerr_dd(const MetricParams &M, T u, T sint, T cost, T *g)
 {
     const double a = M.a;
-    double r = M.rlog ? fm_exp(u) : u;
-    double sigma = r * r + a * a * cost * cost;
-    double delta = r * r + a * a - 2. * r;
-    double A_ = (r * r + a * a) * (r * r + a * a) - delta * a * a * sint * sint;
-    double g00 = -(1. - 2. * r / sigma);
-    double g11 = sigma / delta;
-    double g22 = sigma;
-    double g33 = A_ / sigma * sint * sint;
-    double g03 = -2. * a * r * sint * sint / sigma;
+    T r = M.rlog ? fexp(u) : u;
+    T sigma = r * r + a * a * cost * cost;
+    T delta = r * r + a * a - 2. * r;
+    T A_ = (r * r + a * a) * (r * r + a * a) - delta * a * a * sint * sint;
+    T g00 = -(1. - 2. * r / sigma);
+    T g11 = sigma / delta;
+    T g22 = sigma;
+    T g33 = A_ / sigma * sint * sint;
+    T g03 = -2. * a * r * sint * sint / sigma;
     if (M.rlog)
         g11 *= (r * r);
     g[0] = g00; g[1] = g03; g[2] = g11; g[3] = g22; g[4] = g33;
 }
 
 // FOORT/Metric.cpp:162-191
-FM_HD_INLINE void kerr_uu(const MetricParams &M, double u, double sint, double cost, double *g)
+template <class T>
+FM_HD_INLINE void kerr_uu(const MetricParams &M, T u, T sint, T cost, T *g)
 {
     const double a = M.a;
-    double r = M.rlog ? fm_exp(u) : u;
-    double sigma = r * r + a * a * cost * cost;
-    double delta = r * r + a * a - 2. * r;
-    double A_ = (r * r + a * a) * (r * r + a * a) - delta * a * a * sint * sint;
-    double g00 = -A_ / (sigma * delta);
-    double g11 = delta / sigma;
-    double g22 = 1. / sigma;
-    double g33 = (delta - a * a * sint * sint) / (sigma * delta * sint * sint);
-    double g03 = -2. * a * r / (sigma * delta);
+    T r = M.rlog ? fexp(u) : u;
+    T sigma = r * r + a * a * cost * cost;
+    T delta = r * r + a * a - 2. * r;
+    T A_ = (r * r + a * a) * (r * r + a * a) - delta * a * a * sint * sint;
+    T g00 = -A_ / (sigma * delta);
+    T g11 = delta / sigma;
+    T g22 = 1. / sigma;
+    T g33 = (delta - a * a * sint * sint) / (sigma * delta * sint * sint);
+    T g03 = -2. * a * r / (sigma * delta);
     if (M.rlog)
         g11 *= 1.0 / (r * r);
     g[0] = g00; g[1] = g03; g[2] = g11; g[3] = g22; g[4] = g33;
 }
 
 // FOORT/Metric.cpp:218-229 (no RLogScale: FlatSpace is not a SphericalHorizonMetric)
-FM_HD_INLINE void flat_dd(const MetricParams &, double u, double sint, double, double *g)
+template <class T>
+FM_HD_INLINE void flat_dd(const MetricParams &, T u, T sint, T, T *g)
 {
     g[0] = -1; g[1] = 0; g[2] = 1; g[3] = u * u; g[4] = u * u * sint * sint;
 }
 
-FM_HD_INLINE void flat_uu(const MetricParams &, double u, double sint, double, double *g)
+template <class T>
+FM_HD_INLINE void flat_uu(const MetricParams &, T u, T sint, T, T *g)
 {
     g[0] = -1; g[1] = 0; g[2] = 1; g[3] = 1 / (u * u); g[4] = 1 / (u * u * sint * sint);
 }
 
 // FOORT/Metric.cpp:276-300 / :321-345
+template <class T>
 struct RLCommon {
-    double r, sint2, delta, H3, Bp, H1, H2, sqH1H2;
+    T r, sint2, delta, H3, Bp, H1, H2, sqH1H2;
 };
 
-FM_HD_INLINE void rl_common(const MetricParams &M, double u, double sint, double cost, RLCommon &c)
+template <class T>
+FM_HD_INLINE void rl_common(const MetricParams &M, T u, T sint, T cost, RLCommon<T> &c)
 {
     const double m = M.m, a = M.a, pp = M.p, qq = M.q;
-    double r = M.rlog ? fm_exp(u) : u;
-    double r2 = r * r;
-    double sint2 = sint * sint;
-    double cost2 = cost * cost;
-    double delta = r2 + a * a - 2. * r * m;
-    double H3 = r2 - 2. * r * m + a * a * cost2;
-    double Bp = __builtin_sqrt(pp * qq) * a * sint2 *
+    T r = M.rlog ? fexp(u) : u;
+    T r2 = r * r;
+    T sint2 = sint * sint;
+    T cost2 = cost * cost;
+    T delta = r2 + a * a - 2. * r * m;
+    T H3 = r2 - 2. * r * m + a * a * cost2;
+    T Bp = __builtin_sqrt(pp * qq) * a * sint2 *
                 (((pp * qq + 4. * m * m) * r - m * (pp - 2. * m) * (qq - 2. * m)) / (2. * m * (pp + qq) * H3));
-    double H1 = r2 + a * a * cost2 + r * (pp - 2. * m) + (pp / (pp + qq)) * ((pp - 2. * m) * (qq - 2. * m) / 2.) -
+    T H1 = r2 + a * a * cost2 + r * (pp - 2. * m) + (pp / (pp + qq)) * ((pp - 2. * m) * (qq - 2. * m) / 2.) -
                 (pp / (2. * m * (pp + qq))) * __builtin_sqrt((pp * pp - 4. * m * m) * (qq * qq - 4. * m * m)) * a * cost;
-    double H2 = r2 + a * a * cost2 + r * (qq - 2. * m) + (qq / (pp + qq)) * ((pp - 2. * m) * (qq - 2. * m) / 2.) +
+    T H2 = r2 + a * a * cost2 + r * (qq - 2. * m) + (qq / (pp + qq)) * ((pp - 2. * m) * (qq - 2. * m) / 2.) +
                 (qq / (2. * m * (pp + qq))) * __builtin_sqrt((pp * pp - 4. * m * m) * (qq * qq - 4. * m * m)) * a * cost;
     c.r = r; c.sint2 = sint2; c.delta = delta; c.H3 = H3; c.Bp = Bp; c.H1 = H1; c.H2 = H2;
-    c.sqH1H2 = __builtin_sqrt(H1 * H2);
+    c.sqH1H2 = fsqrt(H1 * H2);
 }
 
-FM_HD_INLINE void rl_dd(const MetricParams &M, double u, double sint, double cost, double *g)
+template <class T>
+FM_HD_INLINE void rl_dd(const MetricParams &M, T u, T sint, T cost, T *g)
 {
-    RLCommon c;
+    RLCommon<T> c;
     rl_common(M, u, sint, cost, c);
-    double g00 = -c.H3 / c.sqH1H2;
-    double g11 = c.sqH1H2 / c.delta;
-    double g22 = c.sqH1H2;
-    double g33 = -(c.H3 * c.Bp * c.Bp) / c.sqH1H2 + (c.sqH1H2 * c.delta * c.sint2) / c.H3;
-    double g03 = -(c.H3 * c.Bp) / c.sqH1H2;
+    T g00 = -c.H3 / c.sqH1H2;
+    T g11 = c.sqH1H2 / c.delta;
+    T g22 = c.sqH1H2;
+    T g33 = -(c.H3 * c.Bp * c.Bp) / c.sqH1H2 + (c.sqH1H2 * c.delta * c.sint2) / c.H3;
+    T g03 = -(c.H3 * c.Bp) / c.sqH1H2;
     if (M.rlog)
         g11 *= (c.r * c.r);
     g[0] = g00; g[1] = g03; g[2] = g11; g[3] = g22; g[4] = g33;
 }
 
-FM_HD_INLINE void rl_uu(const MetricParams &M, double u, double sint, double cost, double *g)
+template <class T>
+FM_HD_INLINE void rl_uu(const MetricParams &M, T u, T sint, T cost, T *g)
 {
-    RLCommon c;
+    RLCommon<T> c;
     rl_common(M, u, sint, cost, c);
-    double g00 = ((c.H3 * c.H3 * c.Bp * c.Bp) / c.sint2 - c.H1 * c.H2 * c.delta) / (c.sqH1H2 * c.H3 * c.delta);
-    double g11 = c.delta / c.sqH1H2;
-    double g22 = 1. / c.sqH1H2;
-    double g33 = c.H3 / (c.sqH1H2 * c.delta * c.sint2);
-    double g03 = -(c.H3 * c.Bp) / (c.sqH1H2 * c.delta * c.sint2);
+    T g00 = ((c.H3 * c.H3 * c.Bp * c.Bp) / c.sint2 - c.H1 * c.H2 * c.delta) / (c.sqH1H2 * c.H3 * c.delta);
+    T g11 = c.delta / c.sqH1H2;
+    T g22 = 1. / c.sqH1H2;
+    T g33 = c.H3 / (c.sqH1H2 * c.delta * c.sint2);
+    T g03 = -(c.H3 * c.Bp) / (c.sqH1H2 * c.delta * c.sint2);
     if (M.rlog)
         g11 *= 1.0 / (c.r * c.r);
     g[0] = g00; g[1] = g03; g[2] = g11; g[3] = g22; g[4] = g33;
 }
 
 // FOORT/Metric.cpp:409-440
-FM_HD_INLINE void joh_dd(const MetricParams &M, double u, double sint, double cost, double *g)
+template <class T>
+FM_HD_INLINE void joh_dd(const MetricParams &M, T u, T sint, T cost, T *g)
 {
     const double a = M.a;
-    double r = M.rlog ? fm_exp(u) : u;
-    double A1 = 1. + M.a13 / (r * r * r);
-    double A2 = 1. + M.a22 / (r * r);
-    double A5 = 1. + M.a52 / (r * r);
-    double eps_f = M.eps3 / r;
-    double rho2 = r * r + a * a * cost * cost + eps_f;
-    double delta = r * r + a * a - 2. * r;
-    double den = fm_pow2((r * r + a * a) * A1 - a * a * A2 * sint * sint);
-    double g00 = -rho2 * (delta - a * a * A2 * A2 * sint * sint) / den;
-    double g11 = rho2 / (delta * A5);
-    double g22 = rho2;
-    double g33 = rho2 * sint * sint * (fm_pow2((r * r + a * a) * A1) - a * a * delta * sint * sint) / den;
-    double g03 = -a * ((r * r + a * a) * A1 * A2 - delta) * rho2 * sint * sint / den;
+    T r = M.rlog ? fexp(u) : u;
+    T A1 = 1. + M.a13 / (r * r * r);
+    T A2 = 1. + M.a22 / (r * r);
+    T A5 = 1. + M.a52 / (r * r);
+    T eps_f = M.eps3 / r;
+    T rho2 = r * r + a * a * cost * cost + eps_f;
+    T delta = r * r + a * a - 2. * r;
+    T den = fpow2((r * r + a * a) * A1 - a * a * A2 * sint * sint);
+    T g00 = -rho2 * (delta - a * a * A2 * A2 * sint * sint) / den;
+    T g11 = rho2 / (delta * A5);
+    T g22 = rho2;
+    T g33 = rho2 * sint * sint * (fpow2((r * r + a * a) * A1) - a * a * delta * sint * sint) / den;
+    T g03 = -a * ((r * r + a * a) * A1 * A2 - delta) * rho2 * sint * sint / den;
     if (M.rlog)
         g11 *= (r * r);
     g[0] = g00; g[1] = g03; g[2] = g11; g[3] = g22; g[4] = g33;
 }
 
 // FOORT/Metric.cpp:442-473
-FM_HD_INLINE void joh_uu(const MetricParams &M, double u, double sint, double cost, double *g)
+template <class T>
+FM_HD_INLINE void joh_uu(const MetricParams &M, T u, T sint, T cost, T *g)
 {
     const double a = M.a;
-    double r = M.rlog ? fm_exp(u) : u;
-    double A1 = 1. + M.a13 / (r * r * r);
-    double A2 = 1. + M.a22 / (r * r);
-    double A5 = 1. + M.a52 / (r * r);
-    double eps_f = M.eps3 / r;
-    double rho2 = r * r + a * a * cost * cost + eps_f;
-    double delta = r * r + a * a - 2. * r;
-    double g00 = (-1. * fm_pow2((r * r + a * a) * A1) + a * a * delta * sint * sint) / (delta * rho2);
-    double g11 = delta * A5 / rho2;
-    double g22 = 1. / rho2;
-    double g33 = (-a * a * A2 * A2 * sint * sint + delta) / (delta * rho2 * sint * sint);
-    double g03 = -a * (A2 * A1 * (r * r + a * a) - delta) / (delta * rho2);
+    T r = M.rlog ? fexp(u) : u;
+    T A1 = 1. + M.a13 / (r * r * r);
+    T A2 = 1. + M.a22 / (r * r);
+    T A5 = 1. + M.a52 / (r * r);
+    T eps_f = M.eps3 / r;
+    T rho2 = r * r + a * a * cost * cost + eps_f;
+    T delta = r * r + a * a - 2. * r;
+    T g00 = (-1. * fpow2((r * r + a * a) * A1) + a * a * delta * sint * sint) / (delta * rho2);
+    T g11 = delta * A5 / rho2;
+    T g22 = 1. / rho2;
+    T g33 = (-a * a * A2 * A2 * sint * sint + delta) / (delta * rho2 * sint * sint);
+    T g03 = -a * (A2 * A1 * (r * r + a * a) - delta) / (delta * rho2);
     if (M.rlog)
         g11 *= 1.0 / (r * r);
     g[0] = g00; g[1] = g03; g[2] = g11; g[3] = g22; g[4] = g33;
 }
 
 // FOORT/Metric.cpp:516-554 / :572-610
+template <class T>
 struct MNCommon {
-    double r, sint, f, exp_2gamma, omega, rho_sq, delta;
+    T r, sint, f, exp_2gamma, omega, rho_sq, delta;
 };
 
-FM_HD_INLINE void mn_common(const MetricParams &M, double u, double sint, double cost, MNCommon &c)
+template <class T>
+FM_HD_INLINE void mn_common(const MetricParams &M, T u, T sint, T cost, MNCommon<T> &c)
 {
     const double al = M.mn_alpha, kk = M.mn_k, a3 = M.alpha3;
-    double r = M.rlog ? fm_exp(u) : u;
-    double xx = (r - 1.) / kk;
-    double yy = cost;
-    double R = __builtin_sqrt(xx * xx + yy * yy - 1.);
-    double P1 = xx * yy / R;
-    double P2 = 0.5 * (3. * xx * xx * yy * yy / (R * R) - 1.);
-    double P3 = 0.5 * (5. * fm_pow3(xx * yy / R) - 3. * xx * yy / R);
-    double P4 = 0.125 * (35. * fm_pow4(xx * yy / R) - 30. * xx * xx * yy * yy / (R * R) + 3.);
-    double aa = -al * fm_exp(2. * a3 * (1. - (xx - yy) * (1. / R + P1 / (R * R) + P2 / (R * R * R) + P3 / (R * R * R * R))));
-    double bb = al * fm_exp(2. * a3 * (-1. + (xx + yy) * (1. / R - P1 / (R * R) + P2 / (R * R * R) - P3 / (R * R * R * R))));
-    double AA = (xx * xx - 1.) * (1. + aa * bb) * (1. + aa * bb) - (1. - yy * yy) * (bb - aa) * (bb - aa);
-    double BB = fm_pow2((xx + 1. + (xx - 1.) * aa * bb)) + fm_pow2(((1. + yy) * aa + (1. - yy) * bb));
-    double CC = (xx * xx - 1.) * (1. + aa * bb) * (bb - aa - yy * (aa + bb)) +
+    T r = M.rlog ? fexp(u) : u;
+    T xx = (r - 1.) / kk;
+    T yy = cost;
+    T R = fsqrt(xx * xx + yy * yy - 1.);
+    T P1 = xx * yy / R;
+    T P2 = 0.5 * (3. * xx * xx * yy * yy / (R * R) - 1.);
+    T P3 = 0.5 * (5. * fpow3(xx * yy / R) - 3. * xx * yy / R);
+    T P4 = 0.125 * (35. * fpow4(xx * yy / R) - 30. * xx * xx * yy * yy / (R * R) + 3.);
+    T aa = -al * fexp(2. * a3 * (1. - (xx - yy) * (1. / R + P1 / (R * R) + P2 / (R * R * R) + P3 / (R * R * R * R))));
+    T bb = al * fexp(2. * a3 * (-1. + (xx + yy) * (1. / R - P1 / (R * R) + P2 / (R * R * R) - P3 / (R * R * R * R))));
+    T AA = (xx * xx - 1.) * (1. + aa * bb) * (1. + aa * bb) - (1. - yy * yy) * (bb - aa) * (bb - aa);
+    T BB = fpow2((xx + 1. + (xx - 1.) * aa * bb)) + fpow2(((1. + yy) * aa + (1. - yy) * bb));
+    T CC = (xx * xx - 1.) * (1. + aa * bb) * (bb - aa - yy * (aa + bb)) +
                 (1. - yy * yy) * (bb - aa) * (1. + aa * bb + xx * (1. - aa * bb));
-    double psi = a3 * P3 / (R * R * R * R);
-    double gamma_prime = 2. * a3 * a3 * (P4 * P4 - P3 * P3) / (fm_pow8(R)) +
+    T psi = a3 * P3 / (R * R * R * R);
+    T gamma_prime = 2. * a3 * a3 * (P4 * P4 - P3 * P3) / (fpow8(R)) +
                          2. * a3 * (-yy + xx * P1 / R - yy * P2 / (R * R) + xx * P3 / (R * R * R)) / R;
-    c.f = fm_exp(2. * psi) * AA / BB;
-    c.exp_2gamma = fm_exp(2. * gamma_prime) * AA / ((xx * xx - yy * yy) * (1. - al * al) * (1. - al * al));
-    c.omega = 2. * kk * fm_exp(-2. * psi) * CC / AA - 4. * kk * al / (1. - al * al);
+    c.f = fexp(2. * psi) * AA / BB;
+    c.exp_2gamma = fexp(2. * gamma_prime) * AA / ((xx * xx - yy * yy) * (1. - al * al) * (1. - al * al));
+    c.omega = 2. * kk * fexp(-2. * psi) * CC / AA - 4. * kk * al / (1. - al * al);
     c.rho_sq = (r - 1.) * (r - 1.) - kk * kk * cost * cost;
     c.delta = (r - 1.) * (r - 1.) - kk * kk;
     c.r = r;
     c.sint = sint;
 }
 
-FM_HD_INLINE void mn_dd(const MetricParams &M, double u, double sint, double cost, double *g)
+template <class T>
+FM_HD_INLINE void mn_dd(const MetricParams &M, T u, T sint, T cost, T *g)
 {
-    MNCommon c;
+    MNCommon<T> c;
     mn_common(M, u, sint, cost, c);
-    double g00 = -c.f;
-    double g22 = c.exp_2gamma * c.rho_sq / c.f;
-    double g11 = g22 / c.delta;
-    double g33 = -c.f * c.omega * c.omega + c.delta * sint * sint / c.f;
-    double g03 = c.omega * c.f;
+    T g00 = -c.f;
+    T g22 = c.exp_2gamma * c.rho_sq / c.f;
+    T g11 = g22 / c.delta;
+    T g33 = -c.f * c.omega * c.omega + c.delta * sint * sint / c.f;
+    T g03 = c.omega * c.f;
     if (M.rlog)
         g11 *= (c.r * c.r);
     g[0] = g00; g[1] = g03; g[2] = g11; g[3] = g22; g[4] = g33;
 }
 
-FM_HD_INLINE void mn_uu(const MetricParams &M, double u, double sint, double cost, double *g)
+template <class T>
+FM_HD_INLINE void mn_uu(const MetricParams &M, T u, T sint, T cost, T *g)
 {
-    MNCommon c;
+    MNCommon<T> c;
     mn_common(M, u, sint, cost, c);
-    double beta_d = c.omega * c.f;
-    double gamma_dd = -c.f * c.omega * c.omega + c.delta * sint * sint / c.f;
-    double beta_u = beta_d / gamma_dd;
-    double alpha_sq = beta_u * beta_d + c.f;
-    double g00 = -1. / alpha_sq;
-    double g22 = c.f / (c.exp_2gamma * c.rho_sq);
-    double g11 = c.delta * g22;
-    double g33 = 1. / gamma_dd - beta_u * beta_u / alpha_sq;
-    double g03 = beta_u / alpha_sq;
+    T beta_d = c.omega * c.f;
+    T gamma_dd = -c.f * c.omega * c.omega + c.delta * sint * sint / c.f;
+    T beta_u = beta_d / gamma_dd;
+    T alpha_sq = beta_u * beta_d + c.f;
+    T g00 = -1. / alpha_sq;
+    T g22 = c.f / (c.exp_2gamma * c.rho_sq);
+    T g11 = c.delta * g22;
+    T g33 = 1. / gamma_dd - beta_u * beta_u / alpha_sq;
+    T g03 = beta_u / alpha_sq;
     if (M.rlog)
         g11 *= 1.0 / (c.r * c.r);
     g[0] = g00; g[1] = g03; g[2] = g11; g[3] = g22; g[4] = g33;
 }
 
-template <int METRIC>
-FM_HD_INLINE void metric_dd(const MetricParams &M, double u, double sint, double cost, double *g)
+template <int METRIC, class T>
+FM_HD_INLINE void metric_dd(const MetricParams &M, T u, T sint, T cost, T *g)
 {
     if constexpr (METRIC == FOORT_METRIC_KERR) kerr_dd(M, u, sint, cost, g);
     else if constexpr (METRIC == FOORT_METRIC_FLATSPACE) flat_dd(M, u, sint, cost, g);
@@ -302,8 +380,8 @@ FM_HD_INLINE void metric_dd(const MetricParams &M, double u, double sint, double
     else mn_dd(M, u, sint, cost, g);
 }
 
-template <int METRIC>
-FM_HD_INLINE void metric_uu(const MetricParams &M, double u, double sint, double cost, double *g)
+template <int METRIC, class T>
+FM_HD_INLINE void metric_uu(const MetricParams &M, T u, T sint, T cost, T *g)
 {
     if constexpr (METRIC == FOORT_METRIC_KERR) kerr_uu(M, u, sint, cost, g);
     else if constexpr (METRIC == FOORT_METRIC_FLATSPACE) flat_uu(M, u, sint, cost, g);
@@ -350,19 +428,82 @@ FM_HD_INLINE void rhs_fd(const RunConst &R, const double *p, const double *v, do
     fm_sincos(th + h, &sp, &cp);
     fm_sincos(th - h, &sm, &cm);
     double gp[5], gm[5], d1[5], d2[5], uu[5];
-    metric_dd<METRIC>(R.M, u + h, s0, c0, gp);
-    metric_dd<METRIC>(R.M, u - h, s0, c0, gm);
+    metric_dd<METRIC, double>(R.M, u + h, s0, c0, gp);
+    metric_dd<METRIC, double>(R.M, u - h, s0, c0, gm);
 #pragma unroll
     for (int k = 0; k < 5; ++k)
         d1[k] = (gp[k] - gm[k]) * R.inv2h;
-    metric_dd<METRIC>(R.M, u, sp, cp, gp);
-    metric_dd<METRIC>(R.M, u, sm, cm, gm);
+    metric_dd<METRIC, double>(R.M, u, sp, cp, gp);
+    metric_dd<METRIC, double>(R.M, u, sm, cm, gm);
 #pragma unroll
     for (int k = 0; k < 5; ++k)
         d2[k] = (gp[k] - gm[k]) * R.inv2h;
-    metric_uu<METRIC>(R.M, u, s0, c0, uu);
+    metric_uu<METRIC, double>(R.M, u, s0, c0, uu);
     geo_contract<double>(d1, d2, uu, v, out);
 }
+
+// Analytic right-hand side: the metric and its exact derivatives in one
+// dual-number pass (scalar type S), then the same sparse contraction.
+template <int METRIC, class S>
+FM_HD_INLINE void rhs_an(const RunConst &R, const double *p, const double *v, double *out)
+{
+    S s, c;
+    ssincos(p[2], &s, &c);
+    const D2<S> U(static_cast<S>(p[1]), S(1), S(0));
+    const D2<S> ST(s, S(0), c), CT(c, S(0), -s);
+    D2<S> g[5], gu[5];
+    metric_dd<METRIC, D2<S>>(R.M, U, ST, CT, g);
+    metric_uu<METRIC, D2<S>>(R.M, U, ST, CT, gu);
+    S d1[5], d2[5], uu[5], vv[4], o[4];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        d1[k] = g[k].du;
+        d2[k] = g[k].dt;
+        uu[k] = gu[k].v;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        vv[k] = static_cast<S>(v[k]);
+    geo_contract<S>(d1, d2, uu, vv, o);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        out[k] = static_cast<double>(o[k]);
+}
+
+// Right-hand-side policies: FD = the reference's finite differences (exact
+// parity), AN64 = analytic binary64, MIXED = analytic, binary32 away from the
+// strong-field region (u > mixed_u) and the poles, binary64 elsewhere.
+template <int METRIC>
+struct RhsFD {
+    static constexpr int metric = METRIC;
+    FM_HD_INLINE static void eval(const RunConst &R, const double *p, const double *v, double *out)
+    {
+        rhs_fd<METRIC>(R, p, v, out);
+    }
+};
+
+template <int METRIC>
+struct RhsAn64 {
+    static constexpr int metric = METRIC;
+    FM_HD_INLINE static void eval(const RunConst &R, const double *p, const double *v, double *out)
+    {
+        rhs_an<METRIC, double>(R, p, v, out);
+    }
+};
+
+template <int METRIC>
+struct RhsMixed {
+    static constexpr int metric = METRIC;
+    FM_HD_INLINE static void eval(const RunConst &R, const double *p, const double *v, double *out)
+    {
+        const double th = p[2];
+        const bool far = p[1] > R.mixed_u && th > R.mixed_pole && (kPi - th) > R.mixed_pole;
+        if (far)
+            rhs_an<METRIC, float>(R, p, v, out);
+        else
+            rhs_an<METRIC, double>(R, p, v, out);
+    }
+};
 
 // std::min / std::max semantics
 FM_HD_INLINE double ref_min(double a, double b) { return (b < a) ? b : a; }
@@ -379,31 +520,31 @@ FM_HD_INLINE double adaptive_step(const RunConst &R, const double *pos, const do
 }
 
 // IntegrateGeodesicStep_RK4, FOORT/Integrators.cpp:51-94
-template <int METRIC>
+template <class RHS>
 FM_HD_INLINE void rk4_step(const RunConst &R, const double *x, const double *v, double *nx, double *nv, double &hout)
 {
     const double h = adaptive_step(R, x, v);
     const double hh = 0.5 * h;
     double k1[4], k2[4], k3[4], k4[4], l2[4], l3[4], l4[4], tp[4];
-    rhs_fd<METRIC>(R, x, v, k1);
+    RHS::eval(R, x, v, k1);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         tp[i] = x[i] + v[i] * hh;
         l2[i] = v[i] + k1[i] * hh;
     }
-    rhs_fd<METRIC>(R, tp, l2, k2);
+    RHS::eval(R, tp, l2, k2);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         tp[i] = x[i] + l2[i] * hh;
         l3[i] = v[i] + k2[i] * hh;
     }
-    rhs_fd<METRIC>(R, tp, l3, k3);
+    RHS::eval(R, tp, l3, k3);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         tp[i] = x[i] + l3[i] * h;
         l4[i] = v[i] + k3[i] * h;
     }
-    rhs_fd<METRIC>(R, tp, l4, k4);
+    RHS::eval(R, tp, l4, k4);
     const double h6 = h / 6.0;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -418,19 +559,19 @@ FM_HD_INLINE void rk4_step(const RunConst &R, const double *x, const double *v, 
 constexpr int kVerletMaxIter = 1000;
 
 // IntegrateGeodesicStep_Verlet, FOORT/Integrators.cpp:99-152
-template <int METRIC>
+template <class RHS>
 FM_HD_INLINE void verlet_step(const RunConst &R, const double *x, const double *v, double *nx, double *nv, double &hout)
 {
     const double h = adaptive_step(R, x, v);
     double ac[4], as[4], vi[4];
-    rhs_fd<METRIC>(R, x, v, ac);
+    RHS::eval(R, x, v, ac);
     const double hh2 = h * h / 2.0;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         nx[i] = (x[i] + v[i] * h) + ac[i] * hh2;
         vi[i] = v[i] + ac[i] * h;
     }
-    rhs_fd<METRIC>(R, nx, vi, as);
+    RHS::eval(R, nx, vi, as);
     const double h2 = h / 2.0;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
@@ -450,7 +591,7 @@ FM_HD_INLINE void verlet_step(const RunConst &R, const double *x, const double *
 #pragma unroll
             for (int i = 0; i < 4; ++i)
                 vi[i] = nv[i];
-            rhs_fd<METRIC>(R, nx, vi, as);
+            RHS::eval(R, nx, vi, as);
 #pragma unroll
             for (int i = 0; i < 4; ++i)
                 nv[i] = v[i] + (ac[i] + as[i]) * h2;
@@ -622,14 +763,14 @@ FM_HD_INLINE void geo_init(const RunConst &R, double ux, double uy, GeoState &g)
 }
 
 // Geodesic::Update, FOORT/Geodesic.cpp:61-88
-template <int METRIC, int INTEG>
+template <class RHS, int INTEG>
 FM_HD_INLINE void geo_update(const RunConst &R, GeoState &g)
 {
     double np[4], nv[4], h;
     if constexpr (INTEG == FOORT_INTEGRATOR_VERLET)
-        verlet_step<METRIC>(R, g.pos, g.vel, np, nv, h);
+        verlet_step<RHS>(R, g.pos, g.vel, np, nv, h);
     else
-        rk4_step<METRIC>(R, g.pos, g.vel, np, nv, h);
+        rk4_step<RHS>(R, g.pos, g.vel, np, nv, h);
     g.lambda += h;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {

@@ -194,6 +194,15 @@ int foort_gpu_trace_device(foort_gpu_ctx *ctx, uint64_t row_column_size, uint64_
                            uint64_t count, const uint64_t *screen_index,
                            const foort_geodesic_results *out, void *stream, foort_stats *stats);
 
+/* Device memory helpers for callers that keep batches resident on the GPU
+ * (no counterpart in the reference): allocate, free, copy, and wait for all
+ * work queued on the device. */
+int foort_gpu_device_alloc(size_t bytes, void **ptr);
+int foort_gpu_device_free(void *ptr);
+int foort_gpu_copy_to_device(void *dst, const void *src, size_t bytes);
+int foort_gpu_copy_to_host(void *dst, const void *src, size_t bytes);
+int foort_gpu_synchronize(void);
+
 /* The camera frame the context uses: vielbein[4][4] (row a = frame vector e_a,
  * FOORT/ViewScreen.h:88) and row 0 of g_dd at the camera (FOORT/ViewScreen.h:86). */
 int foort_gpu_camera(const foort_gpu_ctx *ctx, double vielbein[16], double metric_dd_row0[4]);

@@ -95,7 +95,6 @@ def test_invalid_arguments_raise():
 
 
 def test_device_entry_point_matches_host():
-    import torch
     desc = c1(rlog=True, max_steps=5000)
     side = 24
     F = _gpu()
@@ -104,7 +103,44 @@ def test_device_entry_point_matches_host():
         n = side * side
         d = F.device_results(n)
         st = ctx.trace_device(side, 0, n, d)
-        torch.cuda.synchronize()
+        F.synchronize()
+        idx = np.arange(n)
+        si = F.DeviceArray.from_host(np.stack([idx // side, idx % side], 1).astype(np.uint64))
+        d2 = F.device_results(n)
+        ctx.trace_device(side, 0, n, d2, screen_index=si)
     for k in FIELDS:
-        assert bitwise_equal(d[k].cpu().numpy(), h[k]), k
+        assert bitwise_equal(d[k].numpy(), h[k]), k
+        assert bitwise_equal(d2[k].numpy(), h[k]), k
     assert st.steps == h["stats"].steps and st.geodesics == n
+
+
+@pytest.mark.parametrize("budget", [37, 4096])
+def test_persistent_schedule_bitexact(budget):
+    """Work-stealing lanes (256 lanes for 2304 pixels, so every lane refills
+    several times) with lane state parked in HBM between launches: results
+    must equal the oracle bit for bit."""
+    desc = c1(rlog=True, max_steps=10000, schedule=A.SCHEDULE_PERSISTENT, lanes_per_cu=1, steps_per_launch=budget)
+    side = 48
+    print(f"[persistent] budget={budget}", flush=True)
+    F = _gpu()
+    with F.Context(desc) as ctx:
+        g = ctx.trace_grid(side)
+    o = O.trace("fm", desc, side)
+    for k in FIELDS:
+        assert bitwise_equal(g[k], o[k]), k
+    assert g["stats"].launches > (1 if budget == 37 else 0)
+    assert g["stats"].steps == o["steps"].sum()
+
+
+def test_persistent_pixel_list_verlet():
+    from tests.cases import test_cfg
+    desc = test_cfg(
+        schedule=A.SCHEDULE_PERSISTENT, lanes_per_cu=1, steps_per_launch=100)
+    from tests.cases import sample_pixels
+    px = sample_pixels(250, 700, 9)
+    F = _gpu()
+    with F.Context(desc) as ctx:
+        g = ctx.trace_pixels(250, px)
+    o = O.trace("fm", desc, 250, screen_index=px)
+    for k in FIELDS:
+        assert bitwise_equal(g[k], o[k]), k
