@@ -1,0 +1,226 @@
+#!/usr/bin/env python3
+"""bench.py -- throughput of the per-pixel null-geodesic integration on one MI355X.
+
+Workload (the drop-in configuration): Kerr a=0.99, observer r=1000M,
+inclination 85 deg, 1024x1024 SimpleSquareMesh over a 30M x 30M screen, RK4
+with the adaptive step, FP64, the reference's finite-difference Christoffel
+symbols, horizon / boundary sphere / time-out (1e5) terminations and the
+FourColorScreen, EquatorialPasses and ClosestRadius diagnostics.  One
+benchmark step = one full 1024^2 image (all pixels in one batch through the C
+ABI, foort_gpu_trace_device), results resident in HBM; the pixel indices are
+generated on the device, so no input crosses PCIe.
+
+Reported: pixels/s (value) and geodesic steps/s of that path, whose results
+are bit-identical to the CPU oracle; the roofline of its kernel; the other
+formulations of the same image (persistent schedule, analytic Christoffels in
+FP64 and mixed precision) with their agreement with it; and the reference's
+own CPU build (oracle/_ref, its -Ofast flags) timed on a sample of the same
+image on this host's cores.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+if REPO not in sys.path:
+    sys.path.insert(0, REPO)
+
+METRIC = "pixels/s & geodesic steps/s, Kerr a=0.99 1024² (FP64 vs mixed); p50/p99 latency"
+PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector peak
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def flops_per_step(metric="kerr"):
+    """Algorithmic FLOPs of one RK4 step: the closed-form (analytic) formulation
+    counted by foort_amd/csrc/opcount.cpp (foort_amd/flops_per_step.json)."""
+    with open(os.path.join(REPO, "foort_amd", "flops_per_step.json")) as f:
+        d = json.load(f)
+    closed = [c for c in d["counts"] if c["metric"] == metric and c["formulation"] != "fd"]
+    best = min(closed, key=lambda c: c["rk4_step_flops"])
+    fd = [c for c in d["counts"] if c["metric"] == metric and c["formulation"] == "fd"][0]
+    return best, fd
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def cpu_baseline(desc, side, stride):
+    """The reference's own path (oracle/_ref, built from its sources with its
+    -Ofast flags) on every `stride`-th row and column of the image."""
+    import numpy as np
+    import oracle as O
+    rows = np.arange(0, side, stride, dtype=np.uint64)
+    px = np.stack(np.meshgrid(rows, rows, indexing="ij"), -1).reshape(-1, 2)
+    threads = min(16, os.cpu_count() or 1)
+    kind = "reference" if O.have("ref_ofast") else "port"
+    if kind == "reference":
+        steps, sec = O.time_reference("ref_ofast", desc, side, px, threads)
+    else:
+        t = time.perf_counter()
+        o = O.trace("libm", desc, side, screen_index=px, nthreads=threads)
+        sec = time.perf_counter() - t
+        steps = int(o["steps"].sum())
+    return {
+        "value": len(px) / sec, "unit": "pixels/s", "cores": threads, "kind": kind,
+        "sample": f"{len(px)} pixels (every {stride}th row and column of the 1024x1024 image), "
+                  f"{steps} geodesic steps in {sec:.1f} s on {threads} threads of {cpu_model()}",
+        "geodesic_steps_per_s": steps / sec,
+    }
+
+
+def agreement(ref, out):
+    import numpy as np
+    same = (ref["term"] == out["term"]) & (ref["quadrant"] == out["quadrant"]) & \
+           (ref["equat_passes"] == out["equat_passes"])
+    rel = np.abs(out["final_pos"] - ref["final_pos"]) / np.maximum(np.abs(ref["final_pos"]), 1e-300)
+    relmax = rel.max(axis=1)[same]
+    return {
+        "integer_diag_mismatch_rate": float(1 - same.mean()),
+        "pos_rel_err_median": float(np.median(relmax)) if relmax.size else None,
+        "pos_rel_err_p99": float(np.quantile(relmax, 0.99)) if relmax.size else None,
+        "frac_pos_within_1e-9": float((relmax <= 1e-9).mean()) if relmax.size else None,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--side", type=int, default=1024, help="image side (1024 is the workload)")
+    ap.add_argument("--cpu-stride", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--christoffel", choices=["fd", "analytic"], default="fd",
+                    help="timed path: the reference's finite differences (default) or analytic")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--schedule", choices=["oneshot", "persistent"], default="oneshot")
+    ap.add_argument("--steps-per-launch", type=int, default=0, help="persistent schedule budget (0 = default)")
+    ap.add_argument("--block", type=int, default=0, help="one-shot block size (0 = default)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
+              f"the sharding design is described in DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    import foort_amd as F
+    from foort_amd import workloads as W
+
+    import foort_amd.abi as A
+    side = args.side
+    n = side * side
+    path = dict(christoffel=A.CHRISTOFFEL_FD if args.christoffel == "fd" else A.CHRISTOFFEL_ANALYTIC,
+                precision=A.PRECISION_FP64 if args.precision == "fp64" else A.PRECISION_MIXED,
+                schedule=A.SCHEDULE_ONESHOT if args.schedule == "oneshot" else A.SCHEDULE_PERSISTENT,
+                steps_per_launch=args.steps_per_launch, block_size=args.block)
+    desc = W.c1(**path)
+    ref_desc = W.c1()
+    res = F.device_results(n)
+    ctx = F.Context(desc)
+    for i in range(args.warmup):
+        st = ctx.trace_device(side, 0, n, res)
+        log(f"warmup {i}: {st.device_ms:.1f} ms, {st.steps} steps")
+    F.synchronize()
+    kernel_ms, steps = [], []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        st = ctx.trace_device(side, 0, n, res)  # waits for completion and reads the HIP events
+        kernel_ms.append(st.device_ms)
+        steps.append(st.steps)
+        log(f"step {i}: {st.device_ms:.1f} ms, {st.steps} steps")
+    F.synchronize()
+    wall = time.perf_counter() - t0
+    base = {k: v.numpy() for k, v in res.items()}
+    ctx.close()
+
+    value = args.steps * n / wall
+    closed, fd = flops_per_step("kerr")
+    avg_ms = sum(kernel_ms) / len(kernel_ms)
+    kname = {"fd": "RhsFD", "analytic": "RhsAn64"}[args.christoffel] if args.precision == "fp64" else "RhsMixed"
+    kernel = f"trace_{args.schedule}<{kname}<Kerr>, RK4>"
+    steps_per_launch = sum(steps) / len(steps)
+    achieved = closed["rk4_step_flops"] * steps_per_launch / (avg_ms * 1e-3) / 1e12
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "pixels/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": "C1: " + W.C1_TEXT, "pixels": n,
+                   "christoffel": "finite differences (reference)" if args.christoffel == "fd" else "analytic",
+                   "precision": args.precision, "schedule": args.schedule,
+                   "parity": "bit-identical to the CPU oracle" if args.christoffel == "fd"
+                   else "statistical (see variants)"},
+        "geodesic_steps_per_s": sum(steps) / wall,
+        "roofline": {
+            "bound": "valu_fp64",
+            "achieved": achieved,
+            "peak": PEAK_FP64_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP64_TFLOPS,
+            "traffic": None,
+            "kernel": kernel,
+            "flops_per_step": closed["rk4_step_flops"],
+            "flops_per_step_formulation": closed["formulation"],
+            "steps_per_launch": steps_per_launch,
+            "avg_launch_ms": avg_ms,
+            "fd_formulation_flops_per_step": fd["rk4_step_flops"],
+            "fd_formulation_transcendentals_per_step": fd["rk4_step_transcendental"],
+        },
+    }
+
+    if not args.no_variants:
+        import foort_amd.abi as A
+        variants = {}
+        for name, kw in [("fd_persistent", dict(schedule=A.SCHEDULE_PERSISTENT)),
+                         ("analytic_fp64_oneshot", dict(christoffel=A.CHRISTOFFEL_ANALYTIC)),
+                         ("analytic_fp64_persistent", dict(christoffel=A.CHRISTOFFEL_ANALYTIC,
+                                                           schedule=A.SCHEDULE_PERSISTENT)),
+                         ("mixed_oneshot", dict(christoffel=A.CHRISTOFFEL_ANALYTIC, precision=A.PRECISION_MIXED)),
+                         ("mixed_persistent", dict(christoffel=A.CHRISTOFFEL_ANALYTIC, precision=A.PRECISION_MIXED,
+                                                   schedule=A.SCHEDULE_PERSISTENT))]:
+            with F.Context(W.c1(**kw)) as vctx:
+                vctx.trace_device(side, 0, n, res)  # warm-up
+                F.synchronize()
+                t = time.perf_counter()
+                st = vctx.trace_device(side, 0, n, res)
+                F.synchronize()
+                dt = time.perf_counter() - t
+            out = {k: v.numpy() for k, v in res.items()}
+            variants[name] = {"pixels_per_s": n / dt, "geodesic_steps_per_s": st.steps / dt,
+                              "kernel_ms": st.device_ms, "launches": st.launches,
+                              "vs_fd_oneshot": agreement(base, out)}
+            log(f"{name}: {n / dt:.3e} px/s")
+        line["variants"] = variants
+
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        line["cpu_baseline"] = cpu_baseline(ref_desc, side, args.cpu_stride)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -15,7 +15,7 @@ from . import abi
 from .abi import RunDesc, GeodesicResults, Stats, make_desc, default_desc  # noqa: F401
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libfoort_gpu.so")
+LIB_PATH = os.environ.get("FOORT_GPU_LIB") or os.path.join(_HERE, "libfoort_gpu.so")
 _lib = None
 
 EXPORTS = [

@@ -8,6 +8,15 @@
 
 #include "foort_path.hpp"
 
+// Occupancy targets (waves per SIMD) of the two schedules; they bound the
+// registers the compiler may use.  Chosen by measurement on MI355X.
+#ifndef FOORT_WPE_ONESHOT
+#define FOORT_WPE_ONESHOT 2
+#endif
+#ifndef FOORT_WPE_PERSIST
+#define FOORT_WPE_PERSIST 2
+#endif
+
 namespace foort {
 
 struct DevResults {
@@ -90,7 +99,8 @@ __device__ __forceinline__ void add_steps(unsigned long long *total, unsigned lo
 // One lane per pixel, integrated to termination: the drop-in for the
 // per-thread loop of FOORT/Main.cpp:373-400.
 template <class RHS, int INTEG>
-__global__ void __launch_bounds__(256) trace_oneshot(const RunConst R, uint64_t side, uint64_t first, uint64_t count,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FOORT_WPE_ONESHOT, FOORT_WPE_ONESHOT)))
+trace_oneshot(const RunConst R, uint64_t side, uint64_t first, uint64_t count,
                                                      const uint64_t *__restrict__ si, DevResults out,
                                                      unsigned long long *__restrict__ total_steps, LaneSlots,
                                                      uint32_t)
@@ -166,7 +176,8 @@ __device__ __forceinline__ void slot_save(const LaneSlots &S, uint32_t l, const 
 // as soon as none of its lanes has work.  The host relaunches until `done`
 // reaches the batch size.
 template <class RHS, int INTEG>
-__global__ void __launch_bounds__(256) trace_persistent(const RunConst R, uint64_t side, uint64_t first,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FOORT_WPE_PERSIST, FOORT_WPE_PERSIST)))
+trace_persistent(const RunConst R, uint64_t side, uint64_t first,
                                                         uint64_t count, const uint64_t *__restrict__ si,
                                                         DevResults out, unsigned long long *__restrict__ total_steps,
                                                         LaneSlots S, uint32_t budget)

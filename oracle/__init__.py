@@ -125,3 +125,16 @@ def horizon_radius(kind, desc):
     L = lib(kind)
     L.oracle_horizon_radius.restype = C.c_double
     return L.oracle_horizon_radius(C.byref(desc))
+
+
+def time_reference(kind, desc, size, screen_index, nthreads):
+    """Run the reference's per-pixel loop once per pixel (oracle/_ref, `kind`
+    'ref' or 'ref_ofast'); returns (total steps, wall seconds)."""
+    import time
+    L = lib(kind)
+    si = np.ascontiguousarray(screen_index, dtype=np.uint64).reshape(-1, 2)
+    L.ref_time.restype = C.c_ulonglong
+    L.ref_time.argtypes = [C.POINTER(RunDesc), C.c_uint64, C.c_uint64, C.c_void_p, C.c_int]
+    t = time.perf_counter()
+    steps = L.ref_time(C.byref(desc), size, si.shape[0], si.ctypes.data_as(C.c_void_p), nthreads)
+    return int(steps), time.perf_counter() - t

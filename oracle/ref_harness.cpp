@@ -182,6 +182,39 @@ int ref_trace(const foort_run_desc *d, uint64_t size, uint64_t first, uint64_t c
     return 0;
 }
 
+// The per-pixel work of FOORT/Main.cpp:373-400 exactly once per pixel (initial
+// conditions, Reset, Update to termination, the final value for the mesh and
+// the output strings), for timing the reference; returns the total step count.
+unsigned long long ref_time(const foort_run_desc *d, uint64_t size, uint64_t count, const uint64_t *screen_index,
+                            int nthreads)
+{
+    Run R;
+    std::vector<ScreenIndex> px(count);
+    for (uint64_t i = 0; i < count; ++i)
+        px[i] = ScreenIndex{screen_index[2 * i], screen_index[2 * i + 1]};
+    setup(R, d, std::unique_ptr<Mesh>(new ListMesh(size, std::move(px), static_cast<DiagBitflag>(d->value_diag))));
+    unsigned long long total = 0;
+#pragma omp parallel num_threads(nthreads) reduction(+ : total)
+    {
+        Geodesic geo(R.metric.get(), R.source.get(), R.diags, R.valdiag, R.terms, R.integ);
+#pragma omp for
+        for (long long ii = 0; ii < (long long)count; ++ii) {
+            Point p0;
+            OneIndex v0;
+            ScreenIndex s;
+            R.view->SetNewInitialConditions(ii, p0, v0, s);
+            geo.Reset(s, p0, v0);
+            while (geo.getTermCondition() == Term::Continue) {
+                geo.Update();
+                ++total;
+            }
+            volatile size_t sink = geo.getDiagnosticFinalValue().size() + geo.getAllOutputStr().size();
+            (void)sink;
+        }
+    }
+    return total;
+}
+
 void ref_metric(const foort_run_desc *d, const double *p, double *gdd, double *guu)
 {
     auto M = make_metric(d);

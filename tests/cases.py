@@ -5,24 +5,7 @@ import numpy as np
 
 from foort_amd import abi as A
 
-ALL_TERMS = A.TERMFLAG_HORIZON | A.TERMFLAG_BOUNDARYSPHERE | A.TERMFLAG_TIMEOUT
-ALL_DIAGS = A.DIAGFLAG_FOURCOLORSCREEN | A.DIAGFLAG_EQUATORIALPASSES | A.DIAGFLAG_CLOSESTRADIUS
-
-
-def c0(**kw):
-    """Kerr a=0.5, observer r=1000, 80 deg, 30M x 30M screen, RK4, FP64,
-    horizon / boundary 1000 / time-out 1e5, FCS + EP + CR (the plain small case)."""
-    args = dict(metric=A.METRIC_KERR, params=(0.5,), rlog=False, inclination_deg=80.0, screen=(30.0, 30.0),
-                terms=ALL_TERMS, diags=ALL_DIAGS, max_steps=100000)
-    args.update(kw)
-    return A.make_desc(**args)
-
-
-def c1(**kw):
-    """Kerr a=0.99, 85 deg, otherwise as c0 (the 1024^2 drop-in workload)."""
-    args = dict(params=(0.99,), inclination_deg=85.0)
-    args.update(kw)
-    return c0(**args)
+from foort_amd.workloads import ALL_DIAGS, ALL_TERMS, c0, c1  # noqa: F401
 
 
 def test_cfg(**kw):
