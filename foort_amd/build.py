@@ -57,6 +57,25 @@ def build_gpu(extra=(), out=LIB, verbose=False):
     return out
 
 
+def build_host_path(verbose=False):
+    """The path's templates compiled for the host (CPU tests of the analytic formulations)."""
+    out = os.path.join(PKG, "libfoort_path_host.so")
+    _run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-fPIC", "-shared",
+          "-I" + os.path.join(REPO, "include"), "-I" + CSRC, os.path.join(CSRC, "path_host.cpp"),
+          "-o", out], verbose)
+
+
+def build_opcount(verbose=False):
+    """Count the FLOPs of one step of each formulation (foort_amd/flops_per_step.json)."""
+    exe = os.path.join(BUILD_DIR, "opcount")
+    os.makedirs(BUILD_DIR, exist_ok=True)
+    _run(["g++", "-std=c++17", "-O1", "-I" + os.path.join(REPO, "include"), "-I" + CSRC,
+          os.path.join(CSRC, "opcount.cpp"), "-o", exe], verbose)
+    res = subprocess.run([exe], check=True, capture_output=True, text=True)
+    with open(os.path.join(PKG, "flops_per_step.json"), "w") as f:
+        f.write(res.stdout)
+
+
 def build_oracle(verbose=False):
     cmd = ["make", "-s", "-C", os.path.join(REPO, "oracle"), "-j4"]
     if verbose:
@@ -66,6 +85,8 @@ def build_oracle(verbose=False):
 
 def build_all(verbose=False):
     build_gpu(verbose=verbose)
+    build_host_path(verbose=verbose)
+    build_opcount(verbose=verbose)
     build_oracle(verbose=verbose)
 
 

@@ -132,20 +132,14 @@ FM_FN void fm_sincos(double x, double *s, double *c)
     double w = 1.0 - hz;
     double cs = w + ((((1.0 - w) - hz) - 0.5 * zl) + (pc - hi * lo));
 
+    /* quadrant n: (s, c) = (sn, cs), (cs, -sn), (-sn, -cs), (-cs, sn) for n = 0..3;
+       selected without branches (only exact moves and negations) */
     int n = ((int)kd) & 3;
-    if (n == 0) {
-        *s = sn;
-        *c = cs;
-    } else if (n == 1) {
-        *s = cs;
-        *c = -sn;
-    } else if (n == 2) {
-        *s = -sn;
-        *c = -cs;
-    } else {
-        *s = -cs;
-        *c = sn;
-    }
+    int odd = n & 1;
+    double a = odd ? cs : sn;
+    double b = odd ? sn : cs;
+    *s = (n & 2) ? -a : a;
+    *c = ((n + 1) & 2) ? -b : b;
 }
 
 FM_FN double fm_sin(double x)

@@ -280,6 +280,7 @@ int setup_persistent(foort_gpu_ctx *ctx)
     ctx->slots.next = ctx->d_counters + 1;
     ctx->slots.done = ctx->d_counters + 2;
     ctx->slots.L = static_cast<uint32_t>(L);
+    ctx->slots.refill = ctx->desc.refill_lanes ? std::min(64u, ctx->desc.refill_lanes) : 1u;
     return FOORT_OK;
 }
 

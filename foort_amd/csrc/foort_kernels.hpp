@@ -41,6 +41,7 @@ struct LaneSlots {
     unsigned long long *next;  // next pixel of the batch to hand out
     unsigned long long *done;  // pixels finished
     uint32_t L;
+    uint32_t refill;           // refill idle lanes once at least this many of a wavefront are idle
 };
 
 __device__ __forceinline__ void store_result(const DevResults &out, uint64_t i, const GeoState &g)
@@ -192,7 +193,8 @@ trace_persistent(const RunConst R, uint64_t side, uint64_t first,
     unsigned long long mine = 0;
     for (uint32_t it = 0; it < budget; ++it) {
         const unsigned long long need = __ballot(pix < 0 && !exhausted);
-        if (need) {
+        // refill once enough lanes are idle (or when the wavefront has nothing left to do)
+        if (need && (static_cast<uint32_t>(__popcll(need)) >= S.refill || !__any(pix >= 0))) {
             const int leader = __ffsll(static_cast<long long>(need)) - 1;
             unsigned long long base = 0;
             if (static_cast<int>(lane_id) == leader)

@@ -442,6 +442,109 @@ FM_HD_INLINE void rhs_fd(const RunConst &R, const double *p, const double *v, do
     geo_contract<double>(d1, d2, uu, v, out);
 }
 
+// Geodesic right-hand side from the first derivatives of g_dd and from g^uu,
+// in Euler-Lagrange form: g_mb a^b = 1/2 d_m g_ab v^a v^b - d_a g_mb v^a v^b,
+// with derivatives along the radial coordinate (d1) and theta (d2) only.
+// Used by the analytic formulations (not the reference's summation order).
+template <class S>
+FM_HD_INLINE void kj_contract(const S *d1, const S *d2, const S *u, const S *v, S *out)
+{
+    // D*_m = d*g_mb v^b
+    const S r_t = d1[0] * v[0] + d1[1] * v[3];
+    const S r_r = d1[2] * v[1];
+    const S r_h = d1[3] * v[2];
+    const S r_p = d1[1] * v[0] + d1[4] * v[3];
+    const S h_t = d2[0] * v[0] + d2[1] * v[3];
+    const S h_r = d2[2] * v[1];
+    const S h_h = d2[3] * v[2];
+    const S h_p = d2[1] * v[0] + d2[4] * v[3];
+    const S Kr = S(0.5) * (r_t * v[0] + r_r * v[1] + r_h * v[2] + r_p * v[3]);
+    const S Kh = S(0.5) * (h_t * v[0] + h_r * v[1] + h_h * v[2] + h_p * v[3]);
+    const S Ft = -(r_t * v[1] + h_t * v[2]);
+    const S Fr = Kr - (r_r * v[1] + h_r * v[2]);
+    const S Fh = Kh - (r_h * v[1] + h_h * v[2]);
+    const S Fp = -(r_p * v[1] + h_p * v[2]);
+    out[0] = u[0] * Ft + u[1] * Fp;
+    out[1] = u[2] * Fr;
+    out[2] = u[3] * Fh;
+    out[3] = u[1] * Ft + u[4] * Fp;
+}
+
+FM_HD_INLINE float srcp(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
+FM_HD_INLINE double srcp(double x) { return 1.0 / x; }
+
+// Kerr (M = 1) in closed form: the ten non-zero first derivatives of the
+// Boyer-Lindquist g_dd of FOORT/Metric.cpp:131-159 and g^uu of :162-191,
+// written with Sigma = r^2 + a^2 cos^2, Delta = r^2 - 2r + a^2,
+// q = Sigma - 2 r^2; then the Euler-Lagrange contraction.  With RLogScale the
+// radial coordinate is u = log r: v^r = r v^u and a^u = a^r / r - (v^u)^2.
+template <class S>
+FM_HD_INLINE void rhs_kerr_closed(const RunConst &R, const double *p, const double *v, double *out)
+{
+    S s, c;
+    ssincos(p[2], &s, &c);
+    const S a = static_cast<S>(R.M.a);
+    const S a2 = a * a;
+    S r, vr;
+    if (R.M.rlog) {
+        r = sexp(static_cast<S>(p[1]));
+        vr = r * static_cast<S>(v[1]);
+    } else {
+        r = static_cast<S>(p[1]);
+        vr = static_cast<S>(v[1]);
+    }
+    const S r2 = r * r, s2 = s * s, sc = s * c;
+    const S rpa = r2 + a2;
+    const S Sig = r2 + a2 * (c * c);
+    const S Del = r2 - S(2) * r + a2;
+    // one reciprocal for 1/Sigma, 1/Delta and 1/sin^2
+    const S SD = Sig * Del;
+    const S inv = srcp(SD * s2);
+    const S iSD = s2 * inv;
+    const S iS = Del * iSD, iD = Sig * iSD, is2 = SD * inv;
+    const S iS2 = iS * iS;
+    const S q = Sig - S(2) * r2;
+    const S qS = q * iS2;
+    const S a2sc = a2 * sc;
+    const S arS = S(4) * r * iS2;  // 4 r / Sigma^2
+    S d1[5], d2[5], uu[5];
+    d1[0] = S(2) * qS;                                       // d_r g_tt
+    d2[0] = arS * a2sc;                                      // d_th g_tt
+    d1[1] = S(-2) * a * s2 * qS;                             // d_r g_tph
+    d2[1] = -(arS * a) * sc * rpa;                           // d_th g_tph
+    d1[2] = (S(2) * r * Del - Sig * (S(2) * r - S(2))) * (iD * iD); // d_r g_rr
+    d2[2] = S(-2) * a2sc * iD;                               // d_th g_rr
+    d1[3] = S(2) * r;                                        // d_r g_thth
+    d2[3] = S(-2) * a2sc;                                    // d_th g_thth
+    d1[4] = s2 * (S(2) * r + S(2) * a2 * s2 * qS);           // d_r g_phph
+    d2[4] = S(2) * sc * rpa + arS * a2sc * s2 * (S(2) * Sig + a2 * s2); // d_th g_phph
+    const S A_ = rpa * rpa - a2 * Del * s2;
+    uu[0] = -A_ * iSD;
+    uu[1] = S(-2) * a * r * iSD;
+    uu[2] = Del * iS;
+    uu[3] = iS;
+    uu[4] = (Del - a2 * s2) * iSD * is2;
+    const S vv[4] = {static_cast<S>(v[0]), vr, static_cast<S>(v[2]), static_cast<S>(v[3])};
+    S o[4];
+    kj_contract<S>(d1, d2, uu, vv, o);
+    out[0] = static_cast<double>(o[0]);
+    out[2] = static_cast<double>(o[2]);
+    out[3] = static_cast<double>(o[3]);
+    if (R.M.rlog) {
+        const double vu = v[1];
+        out[1] = static_cast<double>(o[1] * srcp(r)) - vu * vu;
+    } else {
+        out[1] = static_cast<double>(o[1]);
+    }
+}
+
 // Analytic right-hand side: the metric and its exact derivatives in one
 // dual-number pass (scalar type S), then the same sparse contraction.
 template <int METRIC, class S>
@@ -464,7 +567,7 @@ FM_HD_INLINE void rhs_an(const RunConst &R, const double *p, const double *v, do
 #pragma unroll
     for (int k = 0; k < 4; ++k)
         vv[k] = static_cast<S>(v[k]);
-    geo_contract<S>(d1, d2, uu, vv, o);
+    kj_contract<S>(d1, d2, uu, vv, o);
 #pragma unroll
     for (int k = 0; k < 4; ++k)
         out[k] = static_cast<double>(o[k]);
@@ -476,32 +579,44 @@ FM_HD_INLINE void rhs_an(const RunConst &R, const double *p, const double *v, do
 template <int METRIC>
 struct RhsFD {
     static constexpr int metric = METRIC;
+    static constexpr bool exact = true;
     FM_HD_INLINE static void eval(const RunConst &R, const double *p, const double *v, double *out)
     {
         rhs_fd<METRIC>(R, p, v, out);
     }
 };
 
+template <int METRIC, class S>
+FM_HD_INLINE void rhs_analytic(const RunConst &R, const double *p, const double *v, double *out)
+{
+    if constexpr (METRIC == FOORT_METRIC_KERR)
+        rhs_kerr_closed<S>(R, p, v, out);
+    else
+        rhs_an<METRIC, S>(R, p, v, out);
+}
+
 template <int METRIC>
 struct RhsAn64 {
     static constexpr int metric = METRIC;
+    static constexpr bool exact = false;
     FM_HD_INLINE static void eval(const RunConst &R, const double *p, const double *v, double *out)
     {
-        rhs_an<METRIC, double>(R, p, v, out);
+        rhs_analytic<METRIC, double>(R, p, v, out);
     }
 };
 
 template <int METRIC>
 struct RhsMixed {
     static constexpr int metric = METRIC;
+    static constexpr bool exact = false;
     FM_HD_INLINE static void eval(const RunConst &R, const double *p, const double *v, double *out)
     {
         const double th = p[2];
         const bool far = p[1] > R.mixed_u && th > R.mixed_pole && (kPi - th) > R.mixed_pole;
         if (far)
-            rhs_an<METRIC, float>(R, p, v, out);
+            rhs_analytic<METRIC, float>(R, p, v, out);
         else
-            rhs_an<METRIC, double>(R, p, v, out);
+            rhs_analytic<METRIC, double>(R, p, v, out);
     }
 };
 
@@ -519,11 +634,31 @@ FM_HD_INLINE double adaptive_step(const RunConst &R, const double *pos, const do
     return ref_max(h, R.minstep);
 }
 
+// The same step size in an equivalent form with two divisions instead of six,
+// h = eps / ((|v1| + d) + (|v2| + d) / min(th, pi - th) + |v3 + d|), for the
+// formulations that are not bit-compatible with the reference anyway.
+FM_HD_INLINE double adaptive_step_fast(const RunConst &R, const double *pos, const double *vel)
+{
+    const double m = ref_min(pos[2], kPi - pos[2]);
+    const double den = (__builtin_fabs(vel[1]) + 1e-20) + (__builtin_fabs(vel[2]) + 1e-20) / m +
+                       __builtin_fabs(vel[3] + 1e-20);
+    return ref_max(R.eps / den, R.minstep);
+}
+
+template <class RHS>
+FM_HD_INLINE double step_size(const RunConst &R, const double *pos, const double *vel)
+{
+    if constexpr (RHS::exact)
+        return adaptive_step(R, pos, vel);
+    else
+        return adaptive_step_fast(R, pos, vel);
+}
+
 // IntegrateGeodesicStep_RK4, FOORT/Integrators.cpp:51-94
 template <class RHS>
 FM_HD_INLINE void rk4_step(const RunConst &R, const double *x, const double *v, double *nx, double *nv, double &hout)
 {
-    const double h = adaptive_step(R, x, v);
+    const double h = step_size<RHS>(R, x, v);
     const double hh = 0.5 * h;
     double k1[4], k2[4], k3[4], k4[4], l2[4], l3[4], l4[4], tp[4];
     RHS::eval(R, x, v, k1);
@@ -562,7 +697,7 @@ constexpr int kVerletMaxIter = 1000;
 template <class RHS>
 FM_HD_INLINE void verlet_step(const RunConst &R, const double *x, const double *v, double *nx, double *nv, double &hout)
 {
-    const double h = adaptive_step(R, x, v);
+    const double h = step_size<RHS>(R, x, v);
     double ac[4], as[4], vi[4];
     RHS::eval(R, x, v, ac);
     const double hh2 = h * h / 2.0;

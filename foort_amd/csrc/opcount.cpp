@@ -37,6 +37,7 @@ inline void ssincos(double x, Cnt *s, Cnt *c) { cnt::trans += 2; s->v = fm_sin(x
 // the reference-exact functions of the finite-difference path (fm_math.h):
 // exp is transcendental; the integer powers cost what their double-double
 // evaluation costs (an fma counted as 2)
+inline Cnt srcp(Cnt x) { ++cnt::div; return Cnt(1.0 / x.v); }
 inline Cnt fexp(Cnt x) { ++cnt::trans; return Cnt(fm_exp(x.v)); }
 inline Cnt fsqrt(Cnt x) { ++cnt::sqrt_; return Cnt(__builtin_sqrt(x.v)); }
 inline Cnt fpow2(Cnt x) { ++cnt::mul; return Cnt(x.v * x.v); }
@@ -46,12 +47,16 @@ inline Cnt fpow8(Cnt x) { cnt::mul += 9; cnt::add += 10; return Cnt(fm_pow8(x.v)
 
 static void reset() { cnt::add = cnt::mul = cnt::div = cnt::sqrt_ = cnt::trans = 0; }
 
-// Step bookkeeping of RK4 outside the right-hand sides: adaptive step
-// (FOORT/Integrators.cpp:34-47: 3 divisions, 3 reciprocals, 2 adds inside
-// fabs terms, 1 product, 1 sub, 2 adds and a final reciprocal = 13), stage
-// arguments (3 stages x 4 components x 2 vectors x 2 ops = 48), the final
-// update (2 vectors x 4 components x 6 ops = 48) and lambda += h (1).
-constexpr long long kRK4Book = 13 + 48 + 48 + 1;
+// Step bookkeeping of RK4 outside the right-hand sides: stage arguments
+// (3 stages x 4 components x 2 vectors x 2 ops = 48), the final update
+// (2 vectors x 4 components x 6 ops = 48) and lambda += h (1); plus the
+// adaptive step: the reference form (FOORT/Integrators.cpp:34-47: 3 divisions,
+// 3 reciprocals, 3 adds inside the fabs terms, 1 product, 1 sub, 2 adds and a
+// final reciprocal = 14) for the finite-difference path, the equivalent
+// two-division form (1 sub, 3 adds, 1 division, 2 adds, 1 division = 8) for
+// the analytic ones.
+constexpr long long kRK4Book = 48 + 48 + 1;
+constexpr long long kStepRef = 14, kStepFast = 8;
 
 static long long flops() { return cnt::add + cnt::mul + cnt::div + cnt::sqrt_; }
 
@@ -81,7 +86,7 @@ static void count_fd(const char *name, foort::RunConst &R, double u)
     std::printf("    {\"metric\": \"%s\", \"formulation\": \"fd\", \"rhs_flops\": %lld, \"rhs_div\": %lld, "
                 "\"rhs_sqrt\": %lld, \"rhs_transcendental\": %lld, \"rk4_step_flops\": %lld, "
                 "\"rk4_step_transcendental\": %lld}",
-                name, rhs, cnt::div, cnt::sqrt_, cnt::trans, 4 * rhs + kRK4Book, 4 * cnt::trans);
+                name, rhs, cnt::div, cnt::sqrt_, cnt::trans, 4 * rhs + kRK4Book + kStepRef, 4 * cnt::trans);
 }
 
 template <int M>
@@ -89,12 +94,12 @@ static void count_an(const char *name, foort::RunConst &R, double u)
 {
     double p[4] = {0, u, 1.4, 0}, v[4] = {-1, -0.9, 1e-4, 2e-5}, out[4];
     reset();
-    foort::rhs_an<M, Cnt>(R, p, v, out);
+    foort::rhs_analytic<M, Cnt>(R, p, v, out);
     const long long rhs = flops();
     std::printf("    {\"metric\": \"%s\", \"formulation\": \"analytic\", \"rhs_flops\": %lld, \"rhs_div\": %lld, "
                 "\"rhs_sqrt\": %lld, \"rhs_transcendental\": %lld, \"rk4_step_flops\": %lld, "
                 "\"rk4_step_transcendental\": %lld}",
-                name, rhs, cnt::div, cnt::sqrt_, cnt::trans, 4 * rhs + kRK4Book, 4 * cnt::trans);
+                name, rhs, cnt::div, cnt::sqrt_, cnt::trans, 4 * rhs + kRK4Book + kStepFast, 4 * cnt::trans);
 }
 
 template <int M>

@@ -127,6 +127,9 @@ typedef struct foort_run_desc {
     int32_t block_size;          /* threads per workgroup; 0 = tuned default */
     uint32_t steps_per_launch;   /* persistent schedule: steps per kernel launch; 0 = default */
     uint32_t lanes_per_cu;       /* persistent schedule: resident lanes per CU; 0 = default */
+    uint32_t refill_lanes;       /* persistent schedule: a wavefront refills its idle lanes once at
+                                    least this many are idle (1..64; 0 = default) */
+    uint32_t reserved0;
     double mixed_switch_radius;  /* FOORT_PRECISION_MIXED: binary32 steps only where r > this (10) */
     double mixed_pole_margin;    /* ... and where min(theta, pi - theta) > this (0.1) */
 } foort_run_desc;
