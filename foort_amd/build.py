@@ -76,6 +76,25 @@ def build_opcount(verbose=False):
         f.write(res.stdout)
 
 
+HOST_SOURCES = ["cfg.cpp", "config.cpp", "mesh.cpp", "output.cpp", "driver.cpp"]
+HOST_LIB = os.path.join(PKG, "libfoort_host.so")
+CLI = os.path.join(PKG, "foort")
+
+
+def build_host(verbose=False):
+    """The host side of the drop-in (configuration, meshes, output files, driver):
+    libfoort_host.so over libfoort_gpu.so, and the `foort` command line program."""
+    hdir = os.path.join(PKG, "host")
+    inc = ["-I" + os.path.join(REPO, "include"), "-I" + hdir]
+    flags = ["-std=c++17", "-O2", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter"]
+    _run(["g++"] + flags + inc + ["-shared"] + [os.path.join(hdir, s) for s in HOST_SOURCES]
+         + ["-L" + PKG, "-lfoort_gpu", "-Wl,-rpath,$ORIGIN", "-o", HOST_LIB + ".tmp"], verbose)
+    os.replace(HOST_LIB + ".tmp", HOST_LIB)
+    _run(["g++"] + flags + inc + [os.path.join(hdir, "main.cpp"), "-L" + PKG, "-lfoort_host",
+          "-lfoort_gpu", "-Wl,-rpath,$ORIGIN", "-o", CLI + ".tmp"], verbose)
+    os.replace(CLI + ".tmp", CLI)
+
+
 def build_oracle(verbose=False):
     cmd = ["make", "-s", "-C", os.path.join(REPO, "oracle"), "-j4"]
     if verbose:
@@ -85,6 +104,7 @@ def build_oracle(verbose=False):
 
 def build_all(verbose=False):
     build_gpu(verbose=verbose)
+    build_host(verbose=verbose)
     build_host_path(verbose=verbose)
     build_opcount(verbose=verbose)
     build_oracle(verbose=verbose)

@@ -1,0 +1,485 @@
+// driver.cpp -- the program of FOORT/Main.cpp:158-433 around the GPU batch
+// call, and the C ABI of foort_host.h.
+//
+// Per mesh iteration the reference runs, for every pixel index on an OpenMP
+// thread, SetNewInitialConditions -> Reset -> Update until terminated ->
+// GeodesicFinished + NewGeodesicOutput (FOORT/Main.cpp:320-414).  Here the
+// iteration's screen indices go to the GPU in one call
+// (foort_gpu_trace_pixels / foort_gpu_trace_grid); the returned diagnostic
+// values then feed the mesh and the output handler in index order, exactly
+// the values and texts Geodesic::getDiagnosticFinalValue and getAllOutputStr
+// form (FOORT/Geodesic.cpp:111-149).
+#include "host.hpp"
+
+#include <chrono>
+#include <cstring>
+#include <iostream>
+#include <stdexcept>
+
+namespace foort_host {
+
+namespace {
+thread_local std::string g_error;
+
+double seconds_since(std::chrono::steady_clock::time_point t0)
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// the value and text of one diagnostic for geodesic i
+struct BatchResults {
+    std::vector<int32_t> term, quadrant, passes;
+    std::vector<double> closest;
+    explicit BatchResults(uint64_t n) : term(n), quadrant(n), passes(n), closest(n) {}
+    foort_geodesic_results view()
+    {
+        return foort_geodesic_results{term.data(), quadrant.data(), passes.data(), closest.data(),
+                                      nullptr,     nullptr,         nullptr,       nullptr};
+    }
+    double value(uint32_t diag, uint64_t i) const
+    {
+        switch (diag) {
+        case FOORT_DIAGFLAG_FOURCOLORSCREEN: return static_cast<double>(quadrant[i]);
+        case FOORT_DIAGFLAG_EQUATORIALPASSES: return static_cast<double>(passes[i]);
+        default: return closest[i];
+        }
+    }
+    std::string text(uint32_t diag, uint64_t i) const
+    {
+        switch (diag) {
+        case FOORT_DIAGFLAG_FOURCOLORSCREEN: return std::to_string(quadrant[i]);
+        case FOORT_DIAGFLAG_EQUATORIALPASSES: return std::to_string(passes[i]);
+        default: return to_str(closest[i]);
+        }
+    }
+};
+
+int copy_out(const std::string &s, char *buf, size_t cap)
+{
+    if (buf && cap) {
+        size_t n = std::min(cap - 1, s.size());
+        std::memcpy(buf, s.data(), n);
+        buf[n] = '\0';
+    }
+    return static_cast<int>(s.size());
+}
+
+struct ConfigError {
+    int code;
+};
+
+// the program body; throws ConfigError for the reference's early exits
+int run(const char *cfg_path, const foort_program_options &o)
+{
+    set_output_level(4);
+    screen_output("FOORT (gfx950 GPU path) in configuration mode.", Level::Proc);
+    if (!cfg_path) {
+        screen_output("No configuration file given. Exiting...\n", Level::Warning);
+        throw ConfigError{FOORT_EINVAL};
+    }
+    std::unique_ptr<Setting> root;
+    try {
+        root = parse_config_file(cfg_path);
+    } catch (const ParseError &e) {
+        if (e.line == 0)
+            std::cerr << "Config file not found at " << cfg_path << ".\n Exiting...\n";
+        else
+            std::cerr << "Unable to parse config file correctly: parse error at " << cfg_path << ":" << e.line << " - "
+                      << e.message << ".\n"
+                      << "Remember that all numbers must be given as values, e.g. \"3.14/2.0\" is now allowed.\n"
+                         "Exiting...\n";
+        set_error(e.line == 0 ? "configuration file not found" : "configuration parse error: " + e.message);
+        throw ConfigError{FOORT_EINVAL};
+    }
+    screen_output("Initializing all object using configuration file...", Level::Proc);
+    RunConfig rc;
+    read_run_config(*root, rc, true);
+    if (o.christoffel >= 0)
+        rc.desc.christoffel = o.christoffel;
+    if (o.precision >= 0)
+        rc.desc.precision = o.precision;
+    if (o.schedule >= 0)
+        rc.desc.schedule = o.schedule;
+    if (o.screen_output_level >= 0)
+        set_output_level(o.screen_output_level);
+    if (o.output_prefix) {
+        rc.prefix = o.output_prefix;
+        rc.to_files = !rc.prefix.empty();
+    }
+
+    std::unique_ptr<Mesh> mesh = make_mesh(rc.mesh);
+    const std::string first_line = first_line_info(rc, mesh->getFullDescriptionStr());
+    const std::vector<uint32_t> order = diagnostic_order(rc.desc.diag_flags, rc.desc.value_diag);
+    std::vector<std::string> names;
+    for (uint32_t f : order)
+        names.push_back(diag_name(f));
+    OutputHandler out(rc.to_files ? rc.prefix : "", rc.timestamp, rc.extension, names,
+                      rc.to_files ? rc.to_cache : ~0UL - 1, rc.to_files ? rc.per_file : ~0UL,
+                      rc.to_files && rc.first_line_on ? first_line : "");
+    screen_output("Done loading options from configuration file.", Level::Proc);
+
+    // the objects, as FOORT/Main.cpp:262-299 lists them
+    const Level L = Level::Subproc;
+    screen_output("\n--------------------------------", L);
+    screen_output("LIST OF ALL INITIALIZED OBJECTS:", L);
+    screen_output("Metric: " + metric_description(rc.desc) + ".", L);
+    screen_output("Geodesic source: No source.", L);
+    screen_output("Diagnostics turned on: ", L);
+    screen_output("<begin list>", L);
+    for (uint32_t f : order)
+        screen_output(diag_description(f, rc) + ".", L);
+    screen_output("<end list>", L);
+    screen_output("Terminations turned on:", L);
+    screen_output("<begin list>", L);
+    std::string terms = termination_descriptions(rc);
+    for (size_t p = 0; (p = terms.find(", ")) != std::string::npos;) {
+        screen_output(terms.substr(0, p) + ".", L);
+        terms.erase(0, p + 2);
+    }
+    screen_output("<end list>", L);
+    screen_output(viewscreen_description(rc, mesh->getFullDescriptionStr()) + ".", L);
+    screen_output(integrator_description(rc), L);
+    screen_output(out.getFullDescriptionStr(), L);
+    screen_output("--------------------------------\n", L);
+
+    if (rc.desc.diag_flags & FOORT_DIAGFLAG_GEODESICPOSITION) {
+        set_error("the GeodesicPosition diagnostic is not available on the GPU path");
+        screen_output(g_error, Level::Warning);
+        return FOORT_EINVAL;
+    }
+
+    foort_gpu_ctx *gpu = nullptr;
+    if (!o.tracer) {
+        int rcode = foort_gpu_open(&rc.desc, &gpu);
+        if (rcode != FOORT_OK) {
+            set_error(std::string("GPU path: ") + foort_gpu_last_error(nullptr));
+            screen_output(g_error, Level::Warning);
+            return rcode;
+        }
+    }
+    struct Closer {
+        foort_gpu_ctx *c;
+        ~Closer() { foort_gpu_close(c); }
+    } closer{gpu};
+
+    const bool grid = rc.mesh.type == FOORT_MESH_SIMPLESQUARE;
+    foort_stats total{};
+    auto t_total = std::chrono::steady_clock::now();
+    while (!mesh->IsFinished()) {
+        screen_output("Starting new integration loop.", Level::Proc);
+        auto t_iter = std::chrono::steady_clock::now();
+        const uint64_t n = mesh->getCurNrGeodesics();
+        screen_output("Integrating " + std::to_string(static_cast<long long>(n)) + " geodesics on the GPU...",
+                      Level::Proc);
+        out.PrepareForOutput(n);
+        std::vector<uint64_t> idx(2 * n);
+        mesh->screenIndices(idx.data());
+        BatchResults res(n);
+        foort_geodesic_results view = res.view();
+        const uint64_t side = mesh->rowColumnSize();
+        int rcode = FOORT_OK;
+        if (n > 0) {
+            if (o.tracer) {
+                rcode = o.tracer(o.tracer_user, &rc.desc, side, n, idx.data(), &view);
+                if (rcode)
+                    set_error("batch tracer failed with code " + std::to_string(rcode));
+            } else {
+                foort_stats st{};
+                rcode = grid ? foort_gpu_trace_grid(gpu, side, 0, n, &view, &st)
+                             : foort_gpu_trace_pixels(gpu, side, n, idx.data(), &view, &st);
+                if (rcode)
+                    set_error(std::string("GPU path: ") + foort_gpu_last_error(gpu));
+                total.geodesics += st.geodesics;
+                total.steps += st.steps;
+                total.launches += st.launches;
+                total.device_ms += st.device_ms;
+            }
+        }
+        if (rcode != FOORT_OK) {
+            screen_output(g_error, Level::Warning);
+            return rcode;
+        }
+        const uint32_t vd = order.front();
+        std::vector<std::string> texts(order.size());
+        for (uint64_t i = 0; i < n; ++i) {
+            mesh->GeodesicFinished(i, res.value(vd, i));
+            for (size_t k = 0; k < order.size(); ++k)
+                texts[k] = res.text(order[k], i);
+            out.NewGeodesicOutput(i, idx[2 * i], idx[2 * i + 1], texts);
+        }
+        const double it = seconds_since(t_iter), tt = seconds_since(t_total);
+        screen_output("Integration loop done. Time taken for integration loop: " + std::to_string(it) + "s (" +
+                          std::to_string(it / 60) + "m); total time elapsed: " + std::to_string(tt) + "s (" +
+                          std::to_string(tt / 60) + "m).",
+                      Level::Proc);
+        mesh->EndCurrentLoop();
+    }
+    const double tt = seconds_since(t_total);
+    screen_output("All integration finished! Total time elapsed: " + std::to_string(tt) + "s (" +
+                      std::to_string(tt / 60) + "m).",
+                  Level::Proc);
+    out.OutputFinished();
+    screen_output("FOORT finished. Goodbye!", Level::Proc);
+    std::cout.flush();
+    if (o.stats)
+        *o.stats = total;
+    return FOORT_OK;
+}
+
+template <class F>
+int guarded(F &&f)
+{
+    try {
+        return f();
+    } catch (const ConfigError &e) {
+        return e.code;
+    } catch (const ParseError &e) {
+        set_error("configuration parse error at line " + std::to_string(e.line) + ": " + e.message);
+        return FOORT_EINVAL;
+    } catch (const std::bad_alloc &) {
+        set_error("out of host memory");
+        return FOORT_ENOMEM;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return FOORT_EINVAL;
+    }
+}
+} // namespace
+
+std::string set_error(const std::string &msg)
+{
+    g_error = msg;
+    return g_error;
+}
+
+} // namespace foort_host
+
+using namespace foort_host;
+
+struct foort_mesh {
+    std::unique_ptr<Mesh> m;
+};
+
+struct foort_config {
+    RunConfig rc;
+};
+
+struct foort_output {
+    std::unique_ptr<OutputHandler> h;
+};
+
+extern "C" {
+
+const char *foort_host_last_error(void) { return g_error.c_str(); }
+
+int foort_mesh_create(const foort_mesh_desc *desc, foort_mesh **mesh)
+{
+    if (!desc || !mesh) {
+        set_error("NULL argument");
+        return FOORT_EINVAL;
+    }
+    return guarded([&] {
+        if (desc->type == FOORT_MESH_SQUARESUBDIVISION || desc->type == FOORT_MESH_SQUARESUBDIVISIONV2) {
+            if (desc->max_subdivide < 1 || desc->max_subdivide > 40 || desc->initial_pixels < 4)
+                throw std::runtime_error("subdivision mesh needs max_subdivide in 1..40 and at least 2x2 pixels");
+        } else if (desc->total_pixels < 4) {
+            throw std::runtime_error("a mesh needs at least 2x2 pixels");
+        }
+        auto *m = new foort_mesh{make_mesh(*desc)};
+        *mesh = m;
+        return FOORT_OK;
+    });
+}
+
+void foort_mesh_destroy(foort_mesh *mesh) { delete mesh; }
+
+uint64_t foort_mesh_count(const foort_mesh *mesh) { return mesh ? mesh->m->getCurNrGeodesics() : 0; }
+
+uint64_t foort_mesh_row_column_size(const foort_mesh *mesh) { return mesh ? mesh->m->rowColumnSize() : 0; }
+
+int foort_mesh_screen_indices(const foort_mesh *mesh, uint64_t *screen_index)
+{
+    if (!mesh || !screen_index) {
+        set_error("NULL argument");
+        return FOORT_EINVAL;
+    }
+    mesh->m->screenIndices(screen_index);
+    return FOORT_OK;
+}
+
+int foort_mesh_finished(foort_mesh *mesh, const double *values)
+{
+    if (!mesh || (!values && mesh->m->getCurNrGeodesics())) {
+        set_error("NULL argument");
+        return FOORT_EINVAL;
+    }
+    const uint64_t n = mesh->m->getCurNrGeodesics();
+    for (uint64_t i = 0; i < n; ++i)
+        mesh->m->GeodesicFinished(i, values[i]);
+    return FOORT_OK;
+}
+
+int foort_mesh_end_loop(foort_mesh *mesh)
+{
+    if (!mesh) {
+        set_error("NULL argument");
+        return FOORT_EINVAL;
+    }
+    return guarded([&] {
+        mesh->m->EndCurrentLoop();
+        return FOORT_OK;
+    });
+}
+
+int foort_mesh_is_finished(const foort_mesh *mesh) { return mesh ? mesh->m->IsFinished() : 1; }
+
+int foort_mesh_description(const foort_mesh *mesh, char *buf, size_t cap)
+{
+    if (!mesh) {
+        set_error("NULL argument");
+        return FOORT_EINVAL;
+    }
+    return copy_out(mesh->m->getFullDescriptionStr(), buf, cap);
+}
+
+int foort_config_read(const char *path, const char *text, foort_config **cfg)
+{
+    if (!cfg || (!path && !text)) {
+        set_error("NULL argument");
+        return FOORT_EINVAL;
+    }
+    return guarded([&] {
+        std::unique_ptr<Setting> root = path ? parse_config_file(path) : parse_config(text);
+        auto *c = new foort_config{};
+        read_run_config(*root, c->rc, false);
+        *cfg = c;
+        return FOORT_OK;
+    });
+}
+
+void foort_config_free(foort_config *cfg) { delete cfg; }
+
+int foort_config_run_desc(const foort_config *cfg, foort_run_desc *desc)
+{
+    if (!cfg || !desc) {
+        set_error("NULL argument");
+        return FOORT_EINVAL;
+    }
+    *desc = cfg->rc.desc;
+    return FOORT_OK;
+}
+
+int foort_config_mesh_desc(const foort_config *cfg, foort_mesh_desc *desc)
+{
+    if (!cfg || !desc) {
+        set_error("NULL argument");
+        return FOORT_EINVAL;
+    }
+    *desc = cfg->rc.mesh;
+    return FOORT_OK;
+}
+
+int foort_config_first_line(const foort_config *cfg, char *buf, size_t cap)
+{
+    if (!cfg) {
+        set_error("NULL argument");
+        return FOORT_EINVAL;
+    }
+    return guarded([&] {
+        std::unique_ptr<Mesh> m = make_mesh(cfg->rc.mesh);
+        return copy_out(first_line_info(cfg->rc, m->getFullDescriptionStr()), buf, cap);
+    });
+}
+
+int foort_config_diag_names(const foort_config *cfg, char *buf, size_t cap)
+{
+    if (!cfg) {
+        set_error("NULL argument");
+        return FOORT_EINVAL;
+    }
+    std::string s;
+    for (uint32_t f : diagnostic_order(cfg->rc.desc.diag_flags, cfg->rc.desc.value_diag))
+        s += (s.empty() ? "" : " ") + diag_name(f);
+    return copy_out(s, buf, cap);
+}
+
+int foort_output_create(const char *prefix, const char *timestamp, const char *extension,
+                        const char *const *diag_names, int n_diags, uint64_t to_cache, uint64_t per_file,
+                        const char *first_line, foort_output **out)
+{
+    if (!out || n_diags < 0 || (n_diags && !diag_names)) {
+        set_error("NULL argument");
+        return FOORT_EINVAL;
+    }
+    return guarded([&] {
+        std::vector<std::string> names(diag_names, diag_names + n_diags);
+        *out = new foort_output{std::make_unique<OutputHandler>(prefix ? prefix : "", timestamp ? timestamp : "",
+                                                                extension ? extension : "", names, to_cache,
+                                                                per_file == 0 ? ~0UL : per_file,
+                                                                first_line ? first_line : "")};
+        return FOORT_OK;
+    });
+}
+
+void foort_output_destroy(foort_output *out) { delete out; }
+
+int foort_output_prepare(foort_output *out, uint64_t count)
+{
+    if (!out) {
+        set_error("NULL argument");
+        return FOORT_EINVAL;
+    }
+    return guarded([&] {
+        out->h->PrepareForOutput(count);
+        return FOORT_OK;
+    });
+}
+
+int foort_output_geodesic(foort_output *out, uint64_t index, uint64_t row, uint64_t column, const char *const *texts)
+{
+    if (!out) {
+        set_error("NULL argument");
+        return FOORT_EINVAL;
+    }
+    return guarded([&] {
+        std::vector<std::string> t;
+        for (int k = 0; texts && texts[k]; ++k)
+            t.emplace_back(texts[k]);
+        out->h->NewGeodesicOutput(index, row, column, std::move(t));
+        return FOORT_OK;
+    });
+}
+
+int foort_output_finished(foort_output *out)
+{
+    if (!out) {
+        set_error("NULL argument");
+        return FOORT_EINVAL;
+    }
+    return guarded([&] {
+        out->h->OutputFinished();
+        return FOORT_OK;
+    });
+}
+
+void foort_program_options_init(foort_program_options *opts)
+{
+    if (!opts)
+        return;
+    std::memset(opts, 0, sizeof *opts);
+    opts->christoffel = -1;
+    opts->precision = -1;
+    opts->schedule = -1;
+    opts->screen_output_level = -1;
+}
+
+int foort_program_run(const char *cfg_path, const foort_program_options *opts)
+{
+    foort_program_options o;
+    foort_program_options_init(&o);
+    if (opts)
+        o = *opts;
+    return guarded([&] { return run(cfg_path, o); });
+}
+
+} // extern "C"

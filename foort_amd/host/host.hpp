@@ -1,0 +1,135 @@
+// host.hpp -- host side of the drop-in: screen messages, the configuration
+// semantics of FOORT/Config.cpp, the description strings that form the first
+// line of every output file, the meshes of FOORT/Mesh.cpp and the output
+// handler of FOORT/InputOutput.cpp.  The GPU does the integration
+// (foort_gpu.h); everything here runs once per pixel at most.
+#pragma once
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "cfg.hpp"
+#include "foort_gpu.h"
+#include "foort_host.h"
+
+namespace foort_host {
+
+// ---- screen output: FOORT/InputOutput.cpp:12-56 ----
+enum class Level { Warning = 0, Proc = 1, Subproc = 2, AllDetail = 3, Debug = 4, Max = 5 };
+void set_output_level(int level);
+void screen_output(const std::string &text, Level lvl = Level::AllDetail, bool newline = true);
+void set_loop_message_frequency(unsigned long f);
+unsigned long loop_message_frequency();
+// levels of the reference's "default used" messages (FOORT/Config.h:43-46)
+constexpr Level kOtherDefault = Level::Proc;
+constexpr Level kImportantDefault = Level::Warning;
+
+// ---- number texts as std::to_string and std::scientific write them ----
+std::string to_str(double v);       // "%f"
+std::string to_str_sci(double v);   // "%e"
+std::string point_str(const double *p, int n); // toString(std::array<real, n>), FOORT/Geometry.h:98-112
+
+// ---- the objects Config::* builds, as plain values ----
+struct RunConfig {
+    foort_run_desc desc{};
+    bool horizon_metric = true;   // the metric is a SphericalHorizonMetric (all but FlatSpace)
+    double direction[4] = {0, -1, 0, 0};
+    foort_mesh_desc mesh{};
+    std::vector<uint64_t> input_pixels; // InputCertainPixelsMesh list (read from stdin)
+    std::string integrator_name = "RK4"; // Integrators::IntegratorDescription
+    // GeodesicPosition options (read for the description; the GPU path does not run it)
+    unsigned long gp_output_steps = 0;
+    foort_update_frequency gp_update{1, 0, 1};
+    // output handler (Config::GetOutputHandler)
+    bool to_files = false;
+    std::string prefix, timestamp, extension;
+    unsigned long to_cache = ~0UL - 1;
+    unsigned long per_file = ~0UL;
+    bool first_line_on = true;
+    int screen_output_level = 4;
+};
+
+// Config::InitializeScreenOutput ... Config::GetOutputHandler in the order of
+// FOORT/Main.cpp:202-235.  Reads the console for InputCertainPixelsMesh when
+// read_pixels_from_stdin is true.
+void read_run_config(const Setting &root, RunConfig &rc, bool read_pixels_from_stdin);
+
+// CreateDiagnosticVector order (FOORT/Diagnostics.cpp:14-99): the diagnostics in
+// flag order, each followed by std::rotate(begin, begin + 1, end) when it is the
+// value diagnostic.  The first entry supplies the mesh value
+// (Geodesic::getDiagnosticFinalValue, FOORT/Geodesic.cpp:141-149).
+std::vector<uint32_t> diagnostic_order(uint32_t all, uint32_t val);
+std::string diag_name(uint32_t flag);
+std::string diag_description(uint32_t flag, const RunConfig &rc);
+std::string metric_description(const foort_run_desc &d);
+std::string termination_descriptions(const RunConfig &rc);
+std::string integrator_description(const RunConfig &rc);
+std::string viewscreen_description(const RunConfig &rc, const std::string &mesh_description);
+std::string first_line_info(const RunConfig &rc, const std::string &mesh_description);
+
+// FinalDataValDistance of the value diagnostic (FOORT/Diagnostics.cpp:214-221, :395-401, :460-464)
+double value_distance(uint32_t valdiag, double a, double b);
+
+// ---- meshes: FOORT/Mesh.h:24-61 ----
+class Mesh {
+public:
+    explicit Mesh(uint32_t valdiag) : m_valdiag(valdiag) {}
+    virtual ~Mesh() = default;
+    virtual uint64_t getCurNrGeodesics() const = 0;
+    virtual void getNewInitConds(uint64_t index, uint64_t &row, uint64_t &col) const = 0;
+    virtual void GeodesicFinished(uint64_t index, double value) = 0;
+    virtual void EndCurrentLoop() = 0;
+    virtual bool IsFinished() const = 0;
+    virtual std::string getFullDescriptionStr() const = 0;
+    virtual uint64_t rowColumnSize() const = 0;
+    // all screen indices of the current iteration as (row, column) pairs
+    void screenIndices(uint64_t *out) const;
+
+protected:
+    double distance(double a, double b) const { return value_distance(m_valdiag, a, b); }
+    uint32_t m_valdiag;
+};
+
+std::unique_ptr<Mesh> make_mesh(const foort_mesh_desc &d);
+
+// ---- GeodesicOutputHandler: FOORT/InputOutput.h:62-133 ----
+class OutputHandler {
+public:
+    OutputHandler(std::string prefix, std::string timestamp, std::string extension,
+                  std::vector<std::string> diag_names, unsigned long to_cache = ~0UL - 1,
+                  unsigned long per_file = ~0UL, std::string first_line = "");
+    void PrepareForOutput(uint64_t count);
+    // screen index text (row, column) and one text per diagnostic
+    void NewGeodesicOutput(uint64_t index, uint64_t row, uint64_t col, std::vector<std::string> texts);
+    void OutputFinished();
+    std::string getFullDescriptionStr() const;
+    bool writesToConsole() const { return m_console; }
+
+private:
+    struct Entry {
+        uint64_t row = 0, col = 0;
+        std::vector<std::string> texts;
+    };
+    void WriteCached();
+    void OpenForFirstTime(const std::string &name);
+    std::string FileName(int diag, unsigned short filenr) const;
+
+    const std::string m_prefix, m_timestamp, m_extension;
+    const std::vector<std::string> m_names;
+    const unsigned long m_to_cache;
+    const unsigned long m_per_file;
+    const bool m_first_line_on;
+    const std::string m_first_line;
+    bool m_console = false;
+    unsigned long m_prev_cached = 0;
+    unsigned long m_in_file = 0;
+    unsigned short m_full_files = 0;
+    std::vector<Entry> m_cache;
+};
+
+std::string set_error(const std::string &msg); // records the thread's last host error
+
+} // namespace foort_host

@@ -1,0 +1,580 @@
+// mesh.cpp -- the four meshes of FOORT/Mesh.cpp: which pixels each
+// integration iteration traces, and, for the subdivision meshes, how the
+// values of one iteration choose the squares the next iteration subdivides.
+//
+// The selection has to reproduce the reference's pixel queues element for
+// element (the output files list pixels in queue order), so the steps of
+// SquareSubdivisionMesh::EndCurrentLoop / SubdivideAndQueue and their V2
+// counterparts are kept, including std::sort (not stable) with the same
+// comparator over the same candidate sequence.  The data layout is this
+// file's own: pixels live in one vector and refer to each other by index;
+// V1's linear searches of the pixel lists (FOORT/Mesh.cpp:290-326, :393-406)
+// become hash lookups on (row, column), which find the same pixel because a
+// screen index occurs at most once in each list.
+#include "host.hpp"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <stdexcept>
+
+namespace foort_host {
+
+void Mesh::screenIndices(uint64_t *out) const
+{
+    const uint64_t n = getCurNrGeodesics();
+    for (uint64_t i = 0; i < n; ++i)
+        getNewInitConds(i, out[2 * i], out[2 * i + 1]);
+}
+
+namespace {
+
+// ExpInt, FOORT/Mesh.cpp:240-251 (a non-positive exponent gives 1)
+uint64_t exp2i(int e)
+{
+    uint64_t r = 1;
+    while (e > 0) {
+        r *= 2;
+        --e;
+    }
+    return r;
+}
+
+uint64_t isqrt_trunc(uint64_t n) { return static_cast<uint64_t>(std::sqrt(static_cast<double>(n))); }
+
+inline uint64_t key(uint64_t row, uint64_t col) { return (row << 32) ^ col; }
+
+// ---------------------------------------------------------------------------
+// SimpleSquareMesh, FOORT/Mesh.cpp:25-73 and FOORT/Mesh.h:68-103
+// ---------------------------------------------------------------------------
+class SimpleSquareMesh final : public Mesh {
+public:
+    SimpleSquareMesh(uint64_t total, uint32_t valdiag)
+        : Mesh(valdiag), m_side(isqrt_trunc(total)), m_total(m_side * m_side) {}
+    uint64_t getCurNrGeodesics() const override { return m_total; }
+    void getNewInitConds(uint64_t index, uint64_t &row, uint64_t &col) const override
+    {
+        if (index >= m_total)
+            screen_output("Trying to initialize a pixel after all pixels are done!", Level::Warning);
+        row = index / m_side;
+        col = index - row * m_side;
+    }
+    void GeodesicFinished(uint64_t, double) override {}
+    void EndCurrentLoop() override { m_finished = true; }
+    bool IsFinished() const override { return m_finished; }
+    std::string getFullDescriptionStr() const override
+    {
+        return "Mesh: simple square grid (" + std::to_string(static_cast<unsigned long>(m_side)) + "^2 pixels)";
+    }
+    uint64_t rowColumnSize() const override { return m_side; }
+
+private:
+    uint64_t m_side, m_total;
+    bool m_finished = false;
+};
+
+// ---------------------------------------------------------------------------
+// InputCertainPixelsMesh, FOORT/Mesh.cpp:81-185: the pixel list is read from
+// the console while the configuration is read; here it arrives as an array.
+// ---------------------------------------------------------------------------
+class InputCertainPixelsMesh final : public Mesh {
+public:
+    InputCertainPixelsMesh(uint64_t total, const uint64_t *px, uint64_t n, uint32_t valdiag)
+        : Mesh(valdiag), m_side(isqrt_trunc(total)), m_px(px, px + 2 * n) {}
+    uint64_t getCurNrGeodesics() const override { return m_px.size() / 2; }
+    void getNewInitConds(uint64_t index, uint64_t &row, uint64_t &col) const override
+    {
+        row = m_px[2 * index];
+        col = m_px[2 * index + 1];
+    }
+    void GeodesicFinished(uint64_t, double) override {}
+    void EndCurrentLoop() override { m_finished = true; }
+    bool IsFinished() const override { return m_finished; }
+    std::string getFullDescriptionStr() const override { return "Mesh: User-input pixels"; }
+    uint64_t rowColumnSize() const override { return m_side; }
+
+private:
+    uint64_t m_side;
+    std::vector<uint64_t> m_px;
+    bool m_finished = false;
+};
+
+// Common constructor arithmetic of both subdivision meshes (FOORT/Mesh.h:166-183, :321-337)
+struct SubdivisionShape {
+    uint64_t initial, side, max_pixels, iteration;
+    int max_sub;
+    bool to_final, infinite;
+    SubdivisionShape(const foort_mesh_desc &d)
+    {
+        const uint64_t root = isqrt_trunc(d.initial_pixels);
+        initial = root * root;
+        max_sub = d.max_subdivide;
+        side = static_cast<uint64_t>((std::sqrt(static_cast<double>(d.initial_pixels)) - 1) *
+                                         static_cast<double>(exp2i(max_sub - 1)) + 1);
+        max_pixels = d.max_pixels;
+        infinite = d.max_pixels == 0;
+        iteration = d.iteration_pixels;
+        to_final = d.initial_subdivision_to_final != 0;
+    }
+    std::string describe(const char *title) const
+    {
+        return std::string("Mesh: ") + title + " (initial pixels: " +
+               std::to_string(static_cast<unsigned long>(isqrt_trunc(initial))) +
+               "^2; max subdivision: " + std::to_string(max_sub) + "; pixels subdivided per iteration: " +
+               std::to_string(static_cast<unsigned long>(iteration)) + "; max total pixels: " +
+               (infinite ? std::string("infinite") : std::to_string(static_cast<unsigned long>(max_pixels))) +
+               "; if pixel is initially subdivided, will continue to max: " + std::to_string(to_final ? 1 : 0);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// SquareSubdivisionMesh, FOORT/Mesh.cpp:193-652
+// ---------------------------------------------------------------------------
+class SquareSubdivisionMesh final : public Mesh {
+public:
+    explicit SquareSubdivisionMesh(const foort_mesh_desc &d)
+        : Mesh(d.value_diag), S(d), m_left(d.max_pixels)
+    {
+        screen_output("SquareSubdivisionMesh constructed: maxPixels: " +
+                          (S.infinite ? std::string("infinite") : std::to_string(static_cast<unsigned long>(S.max_pixels))) +
+                          "; m_InitialPixels: " + std::to_string(static_cast<unsigned long>(S.initial)) +
+                          "; m_RowColumnSize: " + std::to_string(static_cast<unsigned long>(S.side)),
+                      Level::Debug);
+        // InitializeFirstGrid, FOORT/Mesh.cpp:258-289
+        const uint64_t n = isqrt_trunc(S.initial);
+        const uint64_t step = exp2i(S.max_sub - 1);
+        m_queue.reserve(S.initial);
+        for (uint64_t i = 0; i < S.initial; ++i) {
+            uint64_t row = i / n, col = i - (i / n) * n;
+            int sub = (row == n - 1 || col == n - 1) ? 0 : 1;
+            enqueue(step * row, step * col, sub);
+        }
+        if (!S.infinite)
+            m_left -= m_queue.size();
+        m_done.assign(m_queue.size(), 0);
+    }
+
+    uint64_t getCurNrGeodesics() const override { return m_queue.size(); }
+    void getNewInitConds(uint64_t index, uint64_t &row, uint64_t &col) const override
+    {
+        row = m_queue[index].row;
+        col = m_queue[index].col;
+    }
+    void GeodesicFinished(uint64_t index, double value) override
+    {
+        m_queue[index].value = value;
+        m_done[index] = 1;
+    }
+    bool IsFinished() const override { return m_queue.empty(); }
+    std::string getFullDescriptionStr() const override { return S.describe("square subdivision") + ")"; }
+    uint64_t rowColumnSize() const override { return S.side; }
+
+    void EndCurrentLoop() override
+    {
+        if (std::find(m_done.begin(), m_done.end(), 0) != m_done.end())
+            screen_output("Not all pixels have been integrated!", Level::Warning);
+        // the integrated queue joins the list of all pixels (FOORT/Mesh.cpp:546-560)
+        for (const Pixel &p : m_queue) {
+            m_all_pos.emplace(key(p.row, p.col), m_all.size());
+            m_all.push_back(p);
+        }
+        m_queue.clear();
+        m_queue_pos.clear();
+        m_done.clear();
+        screen_output("Total integrated geodesic so far: " + std::to_string(static_cast<unsigned long>(m_all.size())) +
+                          ".",
+                      Level::Subproc);
+        screen_output("Calculating pixels to subdivide next...", Level::Subproc);
+        if (S.infinite || m_left > 0) {
+            update_neighbors();
+            update_weights();
+            // candidates (FOORT/Mesh.cpp:579-598)
+            std::vector<uint64_t> cand;
+            cand.reserve(m_all.size());
+            for (uint64_t i = 0; i < m_all.size(); ++i) {
+                const Pixel &p = m_all[i];
+                if (p.sub > 0 && p.sub < S.max_sub && (p.weight > 0 || (S.to_final && p.sub > 1)))
+                    cand.push_back(i);
+            }
+            // most important first; equal weights prefer the less subdivided pixel (FOORT/Mesh.cpp:603-614)
+            std::sort(cand.begin(), cand.end(), [this](uint64_t a, uint64_t b) {
+                if (m_all[a].weight > m_all[b].weight)
+                    return true;
+                return m_all[a].weight == m_all[b].weight && m_all[a].sub < m_all[b].sub;
+            });
+            if (cand.size() > S.iteration)
+                cand.resize(S.iteration);
+            m_queue.reserve(5 * cand.size());
+            for (uint64_t i : cand)
+                subdivide(i);
+            if (!S.infinite && m_queue.size() > m_left) {
+                for (uint64_t j = m_left; j < m_queue.size(); ++j)
+                    m_queue_pos.erase(key(m_queue[j].row, m_queue[j].col));
+                m_queue.resize(m_left);
+            }
+            if (!S.infinite) {
+                m_left -= m_queue.size();
+                screen_output("Mesh says: Still max. " + std::to_string(static_cast<unsigned long>(m_left)) +
+                                  " left to integrate after the currently queued " +
+                                  std::to_string(static_cast<unsigned long>(m_queue.size())) + ".",
+                              Level::Subproc);
+            }
+            m_done.assign(m_queue.size(), 0);
+        }
+        screen_output("Done calculating next iteration of pixels.", Level::Subproc);
+    }
+
+private:
+    struct Pixel {
+        uint64_t row, col;
+        int sub;
+        double weight = -1;
+        double value = 0;
+        uint64_t lower = 0, right = 0; // indices into m_all; 0 with both 0 = not yet known
+    };
+
+    SubdivisionShape S;
+    uint64_t m_left;
+    std::vector<Pixel> m_queue, m_all;
+    std::vector<char> m_done;
+    std::unordered_map<uint64_t, uint64_t> m_queue_pos, m_all_pos;
+
+    void enqueue(uint64_t row, uint64_t col, int sub)
+    {
+        m_queue_pos.emplace(key(row, col), m_queue.size());
+        m_queue.push_back(Pixel{row, col, sub});
+    }
+
+    uint64_t find_all(uint64_t row, uint64_t col) const
+    {
+        auto it = m_all_pos.find(key(row, col));
+        return it == m_all_pos.end() ? UINT64_MAX : it->second;
+    }
+
+    // UpdateAllNeighbors, FOORT/Mesh.cpp:292-331
+    void update_neighbors()
+    {
+        screen_output("Updating all pixel neighbor information...", Level::AllDetail);
+        for (Pixel &p : m_all) {
+            if (p.sub > 0 && p.lower == 0 && p.right == 0) {
+                const uint64_t d = exp2i(S.max_sub - p.sub);
+                const uint64_t r = find_all(p.row, p.col + d), l = find_all(p.row + d, p.col);
+                if (r == UINT64_MAX || l == UINT64_MAX)
+                    throw std::runtime_error("SquareSubdivisionMesh: pixel (" + std::to_string(p.row) + ", " +
+                                             std::to_string(p.col) + ") lacks a neighbour");
+                p.right = r;
+                p.lower = l;
+            }
+        }
+        screen_output("Done updating pixel neighbor information.", Level::AllDetail);
+    }
+
+    // UpdateAllWeights, FOORT/Mesh.cpp:336-383
+    void update_weights()
+    {
+        screen_output("Updating all pixel weights...", Level::AllDetail);
+        for (Pixel &p : m_all) {
+            if (p.weight < 0 && p.sub > 0 && p.sub < S.max_sub) {
+                std::array<double, 3> dist{};
+                dist[0] = distance(p.value, m_all[p.right].value);
+                dist[1] = distance(p.value, m_all[p.lower].value);
+                if (p.right > 0 && m_all[p.right].lower > 0)
+                    dist[2] = distance(p.value, m_all[m_all[p.right].lower].value);
+                else if (p.lower > 0 && m_all[p.lower].right > 0)
+                    dist[2] = distance(p.value, m_all[m_all[p.lower].right].value);
+                else
+                    dist[2] = dist[1];
+                p.weight = *std::max_element(dist.begin(), dist.end());
+            }
+        }
+        screen_output("Done updating pixel weights.", Level::AllDetail);
+    }
+
+    // one of the pixels (row, col) of a subdivided square that will itself have
+    // neighbours: queue it, or raise the level of the pixel already there
+    void place_corner(uint64_t row, uint64_t col, int sub)
+    {
+        const uint64_t at = find_all(row, col);
+        if (at == UINT64_MAX) {
+            auto it = m_queue_pos.find(key(row, col));
+            if (it == m_queue_pos.end())
+                enqueue(row, col, sub);
+            else
+                m_queue[it->second].sub = std::max(m_queue[it->second].sub, sub);
+        } else {
+            Pixel &p = m_all[at];
+            p.sub = sub;
+            p.weight = -1;
+            p.lower = p.right = 0;
+        }
+    }
+
+    // a pixel on the far edge of the subdivided square: queued at level 0 if new
+    void place_edge(uint64_t row, uint64_t col)
+    {
+        if (find_all(row, col) == UINT64_MAX && m_queue_pos.find(key(row, col)) == m_queue_pos.end())
+            enqueue(row, col, 0);
+    }
+
+    // SubdivideAndQueue, FOORT/Mesh.cpp:387-523
+    void subdivide(uint64_t ind)
+    {
+        const int sub = m_all[ind].sub + 1;
+        m_all[ind].sub = sub;
+        m_all[ind].lower = m_all[ind].right = 0;
+        m_all[ind].weight = -1;
+        const uint64_t d = exp2i(S.max_sub - sub);
+        const uint64_t row = m_all[ind].row, col = m_all[ind].col;
+        place_corner(row, col + d, sub);
+        place_corner(row + d, col, sub);
+        place_corner(row + d, col + d, sub);
+        place_edge(row + 2 * d, col + d);
+        place_edge(row + d, col + 2 * d);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// SquareSubdivisionMeshV2, FOORT/Mesh.cpp:660-1187
+// ---------------------------------------------------------------------------
+class SquareSubdivisionMeshV2 final : public Mesh {
+public:
+    explicit SquareSubdivisionMeshV2(const foort_mesh_desc &d) : Mesh(d.value_diag), S(d), m_left(d.max_pixels)
+    {
+        // InitializeFirstGrid, FOORT/Mesh.cpp:726-790
+        const uint64_t n = isqrt_trunc(S.initial);
+        const uint64_t step = exp2i(S.max_sub - 1);
+        m_px.reserve(S.initial);
+        for (uint64_t row = 0; row < n; ++row)
+            for (uint64_t col = 0; col < n; ++col)
+                m_px.push_back(Pixel{step * row, step * col, (row == n - 1 || col == n - 1) ? S.max_sub : 1});
+        for (uint64_t row = 0; row + 1 < n; ++row)
+            for (uint64_t col = 0; col + 1 < n; ++col) {
+                const int64_t p = static_cast<int64_t>(row * n + col);
+                link_right(p, p + 1);
+                link_down(p, p + static_cast<int64_t>(n));
+                m_px[p].se = p + static_cast<int64_t>(n) + 1;
+            }
+        m_queue.reserve(S.initial);
+        for (uint64_t i = 0; i < n * n; ++i) {
+            m_queue.push_back(static_cast<int64_t>(i));
+            if (m_px[i].sub < S.max_sub)
+                m_updating.push_back(static_cast<int64_t>(i));
+        }
+        if (!S.infinite)
+            m_left -= m_queue.size();
+        m_done.assign(m_queue.size(), 0);
+    }
+
+    uint64_t getCurNrGeodesics() const override { return m_queue.size(); }
+    void getNewInitConds(uint64_t index, uint64_t &row, uint64_t &col) const override
+    {
+        row = m_px[m_queue[index]].row;
+        col = m_px[m_queue[index]].col;
+    }
+    void GeodesicFinished(uint64_t index, double value) override
+    {
+        m_px[m_queue[index]].value = value;
+        m_done[index] = 1;
+    }
+    bool IsFinished() const override { return m_queue.empty(); }
+    std::string getFullDescriptionStr() const override
+    {
+        return S.describe("square subdivision v2") +
+               "; row/column size: " + std::to_string(static_cast<unsigned long>(S.side)) + ")";
+    }
+    uint64_t rowColumnSize() const override { return S.side; }
+
+    // FOORT/Mesh.cpp:1092-1187
+    void EndCurrentLoop() override
+    {
+        if (std::find(m_done.begin(), m_done.end(), 0) != m_done.end())
+            screen_output("Not all pixels have been integrated!", Level::Warning);
+        m_integrated += m_queue.size();
+        m_queue.clear();
+        m_done.clear();
+        screen_output("Total integrated geodesic so far: " + std::to_string(static_cast<unsigned long>(m_integrated)) +
+                          ".",
+                      Level::Subproc);
+        screen_output("Calculating pixels to subdivide next...", Level::Subproc);
+        if (S.infinite || m_left > 0) {
+            update_weights();
+            std::sort(m_active.begin(), m_active.end(), [this](int64_t a, int64_t b) {
+                if (m_px[a].weight > m_px[b].weight)
+                    return true;
+                return m_px[a].weight == m_px[b].weight && m_px[a].sub < m_px[b].sub;
+            });
+            m_queue.reserve(5 * S.iteration);
+            const uint64_t lim = std::min<uint64_t>(m_active.size(), S.iteration);
+            for (uint64_t i = 0; i < lim; ++i)
+                subdivide(m_active[i]);
+            m_active.erase(m_active.begin(), m_active.begin() + static_cast<std::ptrdiff_t>(lim));
+            if (!S.infinite && m_queue.size() > m_left)
+                m_queue.resize(m_left);
+            if (!S.infinite) {
+                m_left -= m_queue.size();
+                screen_output("Still max. " + std::to_string(static_cast<unsigned long>(m_left)) +
+                                  " left to integrate after the currently queued " +
+                                  std::to_string(static_cast<unsigned long>(m_queue.size())) + ".",
+                              Level::Subproc);
+            }
+            m_done.assign(m_queue.size(), 0);
+        }
+        screen_output("Done calculating next iteration of pixels.", Level::Subproc);
+    }
+
+private:
+    struct Pixel {
+        uint64_t row, col;
+        int sub;
+        double weight = -1;
+        double value = 0;
+        int64_t left = -1, right = -1, up = -1, down = -1, se = -1;
+    };
+
+    SubdivisionShape S;
+    uint64_t m_left;
+    uint64_t m_integrated = 0;
+    std::vector<Pixel> m_px;                       // every pixel ever created
+    std::vector<int64_t> m_active, m_queue, m_updating;
+    std::vector<char> m_done;
+
+    void link_right(int64_t a, int64_t b) { m_px[a].right = b, m_px[b].left = a; }
+    void link_down(int64_t a, int64_t b) { m_px[a].down = b, m_px[b].up = a; }
+
+    int64_t add(uint64_t row, uint64_t col, int sub)
+    {
+        m_px.push_back(Pixel{row, col, sub});
+        const int64_t p = static_cast<int64_t>(m_px.size()) - 1;
+        m_queue.push_back(p);
+        return p;
+    }
+
+    // UpdateAllWeights, FOORT/Mesh.cpp:797-828
+    void update_weights()
+    {
+        screen_output("Updating pixel weights for " + std::to_string(static_cast<unsigned long>(m_updating.size())) +
+                          " pixels...",
+                      Level::AllDetail);
+        for (int64_t i : m_updating) {
+            Pixel &p = m_px[i];
+            std::array<double, 3> dist{distance(p.value, m_px[p.right].value), distance(p.value, m_px[p.down].value),
+                                       distance(p.value, m_px[p.se].value)};
+            p.weight = *std::max_element(dist.begin(), dist.end());
+            if (p.weight > 0.0)
+                m_active.push_back(i);
+        }
+        m_updating.clear();
+        screen_output("Done updating pixel weights.", Level::AllDetail);
+    }
+
+    // GetUp / GetDown / GetLeft / GetRight, FOORT/Mesh.cpp:831-935: the neighbour at
+    // subdivision level `sub`, walking a finer grid where the pixels are subdivided further.
+    // Up and left test the neighbour's level, down and right the pixel's own.
+    int64_t get_up(int64_t p, int sub) const
+    {
+        if (p < 0 || m_px[p].up < 0 || m_px[m_px[p].up].sub < sub)
+            return -1;
+        if (m_px[m_px[p].up].sub == sub)
+            return m_px[p].up;
+        return get_up(get_up(p, sub + 1), sub + 1);
+    }
+    int64_t get_down(int64_t p, int sub) const
+    {
+        if (p < 0 || m_px[p].down < 0 || m_px[p].sub < sub)
+            return -1;
+        if (m_px[p].sub == sub)
+            return m_px[p].down;
+        return get_down(get_down(p, sub + 1), sub + 1);
+    }
+    int64_t get_left(int64_t p, int sub) const
+    {
+        if (p < 0 || m_px[p].left < 0 || m_px[m_px[p].left].sub < sub)
+            return -1;
+        if (m_px[m_px[p].left].sub == sub)
+            return m_px[p].left;
+        return get_left(get_left(p, sub + 1), sub + 1);
+    }
+    int64_t get_right(int64_t p, int sub) const
+    {
+        if (p < 0 || m_px[p].right < 0 || m_px[p].sub < sub)
+            return -1;
+        if (m_px[p].sub == sub)
+            return m_px[p].right;
+        return get_right(get_right(p, sub + 1), sub + 1);
+    }
+
+    // SubdivideAndQueue, FOORT/Mesh.cpp:941-1086.  The square's pixels are numbered
+    //   1 2 3
+    //   4 5 6
+    //   7 8 9
+    // with 1 the subdivided pixel and 3, 7, 9 its right, down and diagonal neighbours.
+    void subdivide(int64_t p1)
+    {
+        const int64_t p3 = m_px[p1].right, p7 = m_px[p1].down, p9 = m_px[p1].se;
+        const int sub = m_px[p1].sub + 1;
+        const uint64_t d = exp2i(S.max_sub - sub);
+        const uint64_t row = m_px[p1].row, col = m_px[p1].col;
+        const bool more = sub < S.max_sub;
+
+        m_px[p1].sub = sub;
+        if (more)
+            m_updating.push_back(p1);
+
+        int64_t p2 = get_down(get_right(get_up(p1, sub), sub), sub);
+        if (p2 < 0)
+            p2 = add(row, col + d, sub);
+        else
+            m_px[p2].sub = sub;
+        if (more)
+            m_updating.push_back(p2);
+
+        int64_t p4 = get_right(get_down(get_left(p1, sub), sub), sub);
+        if (p4 < 0)
+            p4 = add(row + d, col, sub);
+        else
+            m_px[p4].sub = sub;
+        if (more)
+            m_updating.push_back(p4);
+
+        const int64_t p5 = add(row + d, col + d, sub);
+        if (more)
+            m_updating.push_back(p5);
+
+        int64_t p6 = get_down(p3, sub);
+        if (p6 < 0)
+            p6 = add(row + d, col + 2 * d, 0);
+        int64_t p8 = get_right(p7, sub);
+        if (p8 < 0)
+            p8 = add(row + 2 * d, col + d, 0);
+
+        link_right(p1, p2);
+        link_down(p1, p4);
+        m_px[p1].se = p5;
+        link_right(p2, p3);
+        link_down(p2, p5);
+        m_px[p2].se = p6;
+        link_right(p4, p5);
+        link_down(p4, p7);
+        m_px[p4].se = p8;
+        link_right(p5, p6);
+        link_down(p5, p8);
+        m_px[p5].se = p9;
+    }
+};
+
+} // namespace
+
+std::unique_ptr<Mesh> make_mesh(const foort_mesh_desc &d)
+{
+    switch (d.type) {
+    case FOORT_MESH_SIMPLESQUARE: return std::make_unique<SimpleSquareMesh>(d.total_pixels, d.value_diag);
+    case FOORT_MESH_INPUTCERTAINPIXELS:
+        return std::make_unique<InputCertainPixelsMesh>(d.total_pixels, d.pixels, d.pixels ? d.pixel_count : 0,
+                                                        d.value_diag);
+    case FOORT_MESH_SQUARESUBDIVISION: return std::make_unique<SquareSubdivisionMesh>(d);
+    case FOORT_MESH_SQUARESUBDIVISIONV2: return std::make_unique<SquareSubdivisionMeshV2>(d);
+    default: throw std::runtime_error("unknown mesh type " + std::to_string(d.type));
+    }
+}
+
+} // namespace foort_host

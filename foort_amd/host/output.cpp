@@ -1,0 +1,181 @@
+// output.cpp -- output files in the reference's format
+// (GeodesicOutputHandler, FOORT/InputOutput.cpp:62-331): one file per
+// diagnostic named prefix_[timestamp_]Name[_n][.ext], an optional first line
+// describing the run, then one line "row col  value" per geodesic in the order
+// the mesh handed the geodesics out.  Geodesics are cached and written when the
+// cache would overflow and at the end; a file holds at most `per_file`
+// geodesics, after which numbered files follow.
+#include "host.hpp"
+
+#include <algorithm>
+#include <cstdio>
+#include <filesystem>
+#include <iostream>
+#include <limits>
+
+namespace foort_host {
+
+namespace {
+// the cache limit the reference can hold: min(requested, vector::max_size() - 1, ULONG_MAX - 1)
+unsigned long cache_limit(unsigned long n)
+{
+    const unsigned long vmax =
+        static_cast<unsigned long>(std::numeric_limits<std::ptrdiff_t>::max()) / sizeof(std::vector<std::string>);
+    return std::min({n, vmax - 1, std::numeric_limits<unsigned long>::max() - 1});
+}
+} // namespace
+
+OutputHandler::OutputHandler(std::string prefix, std::string timestamp, std::string extension,
+                             std::vector<std::string> diag_names, unsigned long to_cache, unsigned long per_file,
+                             std::string first_line)
+    : m_prefix(std::move(prefix)), m_timestamp(std::move(timestamp)), m_extension(std::move(extension)),
+      m_names(std::move(diag_names)), m_to_cache(cache_limit(to_cache)), m_per_file(per_file),
+      m_first_line_on(!first_line.empty()), m_first_line(std::move(first_line))
+{
+    if (m_prefix.empty() || m_per_file == 0)
+        m_console = true;
+}
+
+std::string OutputHandler::getFullDescriptionStr() const
+{
+    if (!m_console)
+        return "Output Handler: Basic (value diagnostic) file name: " + FileName(0, 1) +
+               ", caching outputs: " + std::to_string(m_to_cache) +
+               ", geodesics per file: " + std::to_string(m_per_file) +
+               ", printing first line info: " + std::to_string(m_first_line_on ? 1 : 0);
+    return "Output Handler: Writing to console, caching outputs: " + std::to_string(m_to_cache);
+}
+
+void OutputHandler::PrepareForOutput(uint64_t count)
+{
+    if (m_cache.size() + count > m_to_cache)
+        WriteCached();
+    m_prev_cached = m_cache.size();
+    m_cache.resize(m_cache.size() + count);
+}
+
+void OutputHandler::NewGeodesicOutput(uint64_t index, uint64_t row, uint64_t col, std::vector<std::string> texts)
+{
+    Entry &e = m_cache[m_prev_cached + index];
+    e.row = row;
+    e.col = col;
+    e.texts = std::move(texts);
+}
+
+void OutputHandler::OutputFinished() { WriteCached(); }
+
+std::string OutputHandler::FileName(int diag, unsigned short filenr) const
+{
+    std::string name = m_prefix + "_";
+    if (!m_timestamp.empty())
+        name += m_timestamp + "_";
+    name += m_names[diag];
+    if (filenr > 1)
+        name += "_" + std::to_string(filenr);
+    if (!m_extension.empty())
+        name += "." + m_extension;
+    return name;
+}
+
+void OutputHandler::OpenForFirstTime(const std::string &name)
+{
+    auto slash = name.find_last_of('/');
+    if (slash != std::string::npos) {
+        std::error_code ec;
+        std::filesystem::create_directories(name.substr(0, slash), ec);
+    }
+    std::FILE *f = std::fopen(name.c_str(), "wb");
+    if (!f) {
+        screen_output("Output file error! Could not open " + name + ". Will write rest of output to console.",
+                      Level::Warning);
+        m_console = true;
+        return;
+    }
+    if (m_first_line_on) {
+        std::fputs(m_first_line.c_str(), f);
+        std::fputc('\n', f);
+    }
+    std::fclose(f);
+}
+
+// WriteCachedOutputToFile, FOORT/InputOutput.cpp:129-267
+void OutputHandler::WriteCached()
+{
+    if (m_cache.empty())
+        return;
+    if (!m_console) {
+        screen_output("Writing cached geodesic output to file(s)...", Level::Subproc);
+        const unsigned long total = m_cache.size();
+        unsigned short nfiles = 1;
+        while (m_in_file + total > static_cast<unsigned long>(nfiles) * m_per_file)
+            ++nfiles;
+        unsigned short cur = 1;
+        unsigned long next = 0, last_count = 0;
+        const int ndiag = static_cast<int>(m_cache[0].texts.size());
+        if (m_in_file == 0)
+            for (int i = 0; i < ndiag && !m_console; ++i)
+                OpenForFirstTime(FileName(i, static_cast<unsigned short>(m_full_files + cur)));
+        std::string buf;
+        while (cur <= nfiles && !m_console) {
+            unsigned long n = cur == 1 ? std::min(total, m_per_file - m_in_file) : std::min(total - next, m_per_file);
+            for (int d = 0; d < ndiag; ++d) {
+                const std::string name = FileName(d, static_cast<unsigned short>(m_full_files + cur));
+                std::FILE *f = std::fopen(name.c_str(), "ab");
+                if (!f) {
+                    screen_output("Output file error! Could not open " + name +
+                                      ". Will write rest of output to console.",
+                                  Level::Warning);
+                    m_console = true;
+                    continue;
+                }
+                buf.clear();
+                for (unsigned long j = next; j < next + n; ++j) {
+                    const Entry &e = m_cache[j];
+                    buf += std::to_string(static_cast<unsigned long>(e.row));
+                    buf += ' ';
+                    buf += std::to_string(static_cast<unsigned long>(e.col));
+                    buf += "  ";
+                    buf += e.texts[d];
+                    buf += '\n';
+                    if (buf.size() > (1u << 22)) {
+                        std::fwrite(buf.data(), 1, buf.size(), f);
+                        buf.clear();
+                    }
+                }
+                std::fwrite(buf.data(), 1, buf.size(), f);
+                std::fclose(f);
+            }
+            ++cur;
+            if (cur <= nfiles) {
+                next += n;
+                for (int i = 0; i < ndiag && !m_console; ++i)
+                    OpenForFirstTime(FileName(i, static_cast<unsigned short>(m_full_files + cur)));
+            } else {
+                last_count = n;
+            }
+        }
+        m_full_files = static_cast<unsigned short>(m_full_files + nfiles - 1);
+        if (last_count == m_per_file) {
+            ++m_full_files;
+            last_count = 0;
+        }
+        if (nfiles == 1)
+            m_in_file += last_count;
+        else
+            m_in_file = last_count;
+        screen_output("Done writing cached geodesic output to file(s).", Level::Subproc);
+    }
+    if (m_console) {
+        for (const Entry &e : m_cache) {
+            screen_output(std::to_string(static_cast<unsigned long>(e.row)) + " " +
+                              std::to_string(static_cast<unsigned long>(e.col)) + " ",
+                          Level::Proc);
+            for (const std::string &t : e.texts)
+                screen_output(t, Level::Proc);
+        }
+        std::cout.flush();
+    }
+    m_cache.clear();
+}
+
+} // namespace foort_host

@@ -1,0 +1,250 @@
+"""ctypes binding of include/foort_host.h (libfoort_host.so): the host side of
+the drop-in -- configuration files, meshes, output files and the program loop
+of FOORT/Main.cpp around the GPU batch call.
+
+    from foort_amd import program
+    program.run("my.cfg")                       # what `foort my.cfg` does
+    cfg = program.Config(text=open("my.cfg").read())
+    cfg.run_desc(), cfg.mesh_desc(), cfg.first_line(), cfg.diag_names()
+    mesh = program.Mesh(cfg.mesh_desc())        # Mesh::getNewInitConds / GeodesicFinished / EndCurrentLoop
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from .abi import GeodesicResults, RunDesc, Stats
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfoort_host.so")
+CLI_PATH = os.path.join(_HERE, "foort")
+_lib = None
+
+MESH_SIMPLESQUARE = 0
+MESH_INPUTCERTAINPIXELS = 1
+MESH_SQUARESUBDIVISION = 2
+MESH_SQUARESUBDIVISIONV2 = 3
+
+EXPORTS = [
+    "foort_host_last_error", "foort_mesh_create", "foort_mesh_destroy", "foort_mesh_count",
+    "foort_mesh_row_column_size", "foort_mesh_screen_indices", "foort_mesh_finished", "foort_mesh_end_loop",
+    "foort_mesh_is_finished", "foort_mesh_description", "foort_config_read", "foort_config_free",
+    "foort_config_run_desc", "foort_config_mesh_desc", "foort_config_first_line", "foort_config_diag_names",
+    "foort_output_create", "foort_output_destroy", "foort_output_prepare", "foort_output_geodesic",
+    "foort_output_finished", "foort_program_options_init", "foort_program_run",
+]
+
+
+class MeshDesc(C.Structure):
+    """foort_mesh_desc (include/foort_host.h)"""
+    _fields_ = [
+        ("type", C.c_int32), ("value_diag", C.c_uint32), ("total_pixels", C.c_uint64),
+        ("initial_pixels", C.c_uint64), ("max_pixels", C.c_uint64), ("iteration_pixels", C.c_uint64),
+        ("max_subdivide", C.c_int32), ("initial_subdivision_to_final", C.c_int32),
+        ("pixels", C.c_void_p), ("pixel_count", C.c_uint64),
+    ]
+
+
+TRACE_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(RunDesc), C.c_uint64, C.c_uint64,
+                       C.POINTER(C.c_uint64), C.POINTER(GeodesicResults))
+
+
+class ProgramOptions(C.Structure):
+    """foort_program_options (include/foort_host.h)"""
+    _fields_ = [
+        ("output_prefix", C.c_char_p), ("christoffel", C.c_int32), ("precision", C.c_int32),
+        ("schedule", C.c_int32), ("screen_output_level", C.c_int32), ("tracer", TRACE_FN),
+        ("tracer_user", C.c_void_p), ("stats", C.POINTER(Stats)),
+    ]
+
+
+class HostError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise HostError(f"{LIB_PATH} is not built: run python -c 'import __graft_entry__ as g; g.build()'")
+        L = C.CDLL(LIB_PATH)
+        L.foort_host_last_error.restype = C.c_char_p
+        L.foort_mesh_create.argtypes = [C.POINTER(MeshDesc), C.POINTER(C.c_void_p)]
+        L.foort_mesh_destroy.argtypes = [C.c_void_p]
+        for f in ("foort_mesh_count", "foort_mesh_row_column_size"):
+            getattr(L, f).restype = C.c_uint64
+            getattr(L, f).argtypes = [C.c_void_p]
+        L.foort_mesh_screen_indices.argtypes = [C.c_void_p, C.c_void_p]
+        L.foort_mesh_finished.argtypes = [C.c_void_p, C.c_void_p]
+        L.foort_mesh_end_loop.argtypes = [C.c_void_p]
+        L.foort_mesh_is_finished.argtypes = [C.c_void_p]
+        L.foort_mesh_description.argtypes = [C.c_void_p, C.c_char_p, C.c_size_t]
+        L.foort_config_read.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p)]
+        L.foort_config_free.argtypes = [C.c_void_p]
+        L.foort_config_run_desc.argtypes = [C.c_void_p, C.POINTER(RunDesc)]
+        L.foort_config_mesh_desc.argtypes = [C.c_void_p, C.POINTER(MeshDesc)]
+        L.foort_config_first_line.argtypes = [C.c_void_p, C.c_char_p, C.c_size_t]
+        L.foort_config_diag_names.argtypes = [C.c_void_p, C.c_char_p, C.c_size_t]
+        L.foort_output_create.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(C.c_char_p), C.c_int,
+                                          C.c_uint64, C.c_uint64, C.c_char_p, C.POINTER(C.c_void_p)]
+        L.foort_output_destroy.argtypes = [C.c_void_p]
+        L.foort_output_prepare.argtypes = [C.c_void_p, C.c_uint64]
+        L.foort_output_geodesic.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.POINTER(C.c_char_p)]
+        L.foort_output_finished.argtypes = [C.c_void_p]
+        L.foort_program_options_init.argtypes = [C.POINTER(ProgramOptions)]
+        L.foort_program_run.argtypes = [C.c_char_p, C.POINTER(ProgramOptions)]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise HostError(lib().foort_host_last_error().decode())
+
+
+def _text(fn, *args):
+    n = fn(*args, None, 0)
+    if n < 0:
+        _check(n)
+    buf = C.create_string_buffer(n + 1)
+    fn(*args, buf, n + 1)
+    return buf.value.decode()
+
+
+class Mesh:
+    """A mesh (foort_mesh_create): one of the reference's four Mesh classes."""
+
+    def __init__(self, desc):
+        self.desc = desc
+        self._h = C.c_void_p()
+        _check(lib().foort_mesh_create(C.byref(desc), C.byref(self._h)))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().foort_mesh_destroy(self._h)
+            self._h = None
+
+    @property
+    def count(self):
+        return lib().foort_mesh_count(self._h)
+
+    @property
+    def side(self):
+        return lib().foort_mesh_row_column_size(self._h)
+
+    def screen_indices(self):
+        out = np.zeros((self.count, 2), np.uint64)
+        _check(lib().foort_mesh_screen_indices(self._h, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def finished(self, values):
+        v = np.ascontiguousarray(values, np.float64)
+        assert v.shape == (self.count,)
+        _check(lib().foort_mesh_finished(self._h, v.ctypes.data_as(C.c_void_p)))
+
+    def end_loop(self):
+        _check(lib().foort_mesh_end_loop(self._h))
+
+    @property
+    def is_finished(self):
+        return bool(lib().foort_mesh_is_finished(self._h))
+
+    def description(self):
+        return _text(lib().foort_mesh_description, self._h)
+
+
+def mesh_desc(type, value_diag, total_pixels=0, initial_pixels=0, max_pixels=0, iteration_pixels=0,
+              max_subdivide=1, initial_subdivision_to_final=False):
+    return MeshDesc(type=type, value_diag=value_diag, total_pixels=total_pixels, initial_pixels=initial_pixels,
+                    max_pixels=max_pixels, iteration_pixels=iteration_pixels, max_subdivide=max_subdivide,
+                    initial_subdivision_to_final=int(initial_subdivision_to_final))
+
+
+class Config:
+    """A configuration file read the way FOORT/Config.cpp reads it (foort_config_read)."""
+
+    def __init__(self, path=None, text=None):
+        self._h = C.c_void_p()
+        _check(lib().foort_config_read(path.encode() if path else None, text.encode() if text is not None else None,
+                                       C.byref(self._h)))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().foort_config_free(self._h)
+            self._h = None
+
+    def run_desc(self):
+        d = RunDesc()
+        _check(lib().foort_config_run_desc(self._h, C.byref(d)))
+        return d
+
+    def mesh_desc(self):
+        m = MeshDesc()
+        _check(lib().foort_config_mesh_desc(self._h, C.byref(m)))
+        return m
+
+    def first_line(self):
+        return _text(lib().foort_config_first_line, self._h)
+
+    def diag_names(self):
+        return _text(lib().foort_config_diag_names, self._h).split()
+
+
+class Output:
+    """GeodesicOutputHandler (foort_output_create)."""
+
+    def __init__(self, prefix, names, timestamp="", extension="", to_cache=2 ** 64 - 2, per_file=0, first_line=""):
+        self._names = (C.c_char_p * len(names))(*[n.encode() for n in names])
+        self._h = C.c_void_p()
+        _check(lib().foort_output_create(prefix.encode(), timestamp.encode(), extension.encode(), self._names,
+                                         len(names), to_cache, per_file, first_line.encode(), C.byref(self._h)))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().foort_output_destroy(self._h)
+            self._h = None
+
+    def prepare(self, n):
+        _check(lib().foort_output_prepare(self._h, n))
+
+    def geodesic(self, index, row, col, texts):
+        arr = (C.c_char_p * (len(texts) + 1))(*[t.encode() for t in texts], None)
+        _check(lib().foort_output_geodesic(self._h, index, row, col, arr))
+
+    def finished(self):
+        _check(lib().foort_output_finished(self._h))
+
+
+def run(cfg_path, output_prefix=None, christoffel=-1, precision=-1, schedule=-1, screen_output_level=-1,
+        tracer=None):
+    """Run the program (foort_program_run) on a configuration file; returns the GPU totals
+    (foort_stats).  `tracer(desc, side, screen_index[n, 2]) -> dict of numpy arrays` replaces the
+    GPU batch call when given (the tests drive the driver with the CPU oracle this way)."""
+    o = ProgramOptions()
+    lib().foort_program_options_init(C.byref(o))
+    o.output_prefix = output_prefix.encode() if output_prefix is not None else None
+    o.christoffel, o.precision, o.schedule = christoffel, precision, schedule
+    o.screen_output_level = screen_output_level
+    st = Stats()
+    o.stats = C.pointer(st)
+    if tracer is not None:
+        def _cb(user, desc_p, side, count, si_p, out_p):
+            try:
+                si = np.ctypeslib.as_array(si_p, shape=(count * 2,)).reshape(count, 2).copy()
+                res = tracer(desc_p.contents, side, si)
+                out = out_p.contents
+                for name, ctype in (("term", C.c_int32), ("quadrant", C.c_int32), ("equat_passes", C.c_int32),
+                                    ("closest_radius", C.c_double)):
+                    ptr = getattr(out, name)
+                    if ptr:
+                        dst = np.ctypeslib.as_array(C.cast(ptr, C.POINTER(ctype)), shape=(count,))
+                        dst[:] = res[name]
+                return 0
+            except Exception:  # noqa: BLE001 -- reported through the return code
+                import traceback
+                traceback.print_exc()
+                return -1
+        o.tracer = TRACE_FN(_cb)
+    rc = lib().foort_program_run(cfg_path.encode(), C.byref(o))
+    _check(rc)
+    return st

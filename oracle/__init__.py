@@ -138,3 +138,60 @@ def time_reference(kind, desc, size, screen_index, nthreads):
     t = time.perf_counter()
     steps = L.ref_time(C.byref(desc), size, si.shape[0], si.ctypes.data_as(C.c_void_p), nthreads)
     return int(steps), time.perf_counter() - t
+
+
+def ref_mesh_run(mesh_type, pixels, max_pixels, iteration, max_sub, to_final, valdiag, value_grid):
+    """Drive one of the reference's meshes (oracle/_ref) with values read from
+    value_grid[row, col]; returns (screen indices [n, 2] in queue order, pixels per
+    iteration, the mesh's description string)."""
+    L = lib("ref")
+    grid = np.ascontiguousarray(value_grid, dtype=np.float64)
+    side = grid.shape[0]
+    cap = side * side
+    out = np.zeros((cap, 2), np.uint64)
+    iters = np.zeros(4096, np.uint64)
+    nit = C.c_int()
+    desc = C.create_string_buffer(1024)
+    L.ref_mesh_run.restype = C.c_longlong
+    L.ref_mesh_run.argtypes = [C.c_int, C.c_uint64, C.c_uint64, C.c_uint64, C.c_int, C.c_int, C.c_uint32,
+                               C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64, C.c_void_p, C.c_int,
+                               C.POINTER(C.c_int), C.c_char_p, C.c_size_t]
+    n = L.ref_mesh_run(mesh_type, pixels, max_pixels, iteration, max_sub, int(to_final), valdiag,
+                       grid.ctypes.data_as(C.c_void_p), side, out.ctypes.data_as(C.c_void_p), cap,
+                       iters.ctypes.data_as(C.c_void_p), len(iters), C.byref(nit), desc, len(desc))
+    if n < 0:
+        raise RuntimeError(f"ref_mesh_run failed ({n})")
+    return out[:n], iters[:nit.value], desc.value.decode()
+
+
+def ref_first_line(desc, mesh_type, pixels, max_pixels=0, iteration=0, max_sub=1, to_final=False):
+    """Utilities::GetFirstLineInfoString of the reference for a run description and mesh."""
+    L = lib("ref")
+    buf = C.create_string_buffer(4096)
+    L.ref_first_line.argtypes = [C.POINTER(RunDesc), C.c_int, C.c_uint64, C.c_uint64, C.c_uint64, C.c_int,
+                                 C.c_int, C.c_char_p, C.c_size_t]
+    L.ref_first_line(C.byref(desc), mesh_type, pixels, max_pixels, iteration, max_sub, int(to_final), buf, len(buf))
+    return buf.value.decode()
+
+
+def ref_output_run(prefix, names, timestamp, ext, to_cache, per_file, first_line, batches):
+    """Feed the reference GeodesicOutputHandler batches of synthetic geodesic outputs."""
+    L = lib("ref")
+    arr = (C.c_char_p * len(names))(*[n.encode() for n in names])
+    b = np.ascontiguousarray(batches, dtype=np.uint64)
+    L.ref_output_run.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(C.c_char_p), C.c_int, C.c_uint64,
+                                 C.c_uint64, C.c_char_p, C.c_void_p, C.c_int]
+    L.ref_output_run(prefix.encode(), timestamp.encode(), ext.encode(), arr, len(names), to_cache, per_file,
+                     first_line.encode(), b.ctypes.data_as(C.c_void_p), len(b))
+
+
+def ref_program_run(desc, mesh_type, pixels, prefix, ext="dat", max_pixels=0, iteration=0, max_sub=1,
+                    to_final=False, to_cache=2 ** 64 - 2, per_file=2 ** 64 - 1, first_line=True, nthreads=None):
+    """The reference program loop (oracle/_ref) with one of its meshes, writing output files."""
+    L = lib("ref")
+    L.ref_program_run.restype = C.c_longlong
+    L.ref_program_run.argtypes = [C.POINTER(RunDesc), C.c_int, C.c_uint64, C.c_uint64, C.c_uint64, C.c_int, C.c_int,
+                                  C.c_char_p, C.c_char_p, C.c_uint64, C.c_uint64, C.c_int, C.c_int]
+    return L.ref_program_run(C.byref(desc), mesh_type, pixels, max_pixels, iteration, max_sub, int(to_final),
+                             prefix.encode(), ext.encode(), to_cache, per_file, int(first_line),
+                             nthreads or min(8, os.cpu_count() or 1))

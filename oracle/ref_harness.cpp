@@ -10,10 +10,12 @@
 // itself needs libconfig++, which is not installed, so the static option
 // members it defines (FOORT/Config.cpp:11-28) are defined here.
 #include "Geodesic.h"
+#include "InputOutput.h"
 #include "Integrators.h"
 #include "Metric.h"
-#include "ViewScreen.h"
 #include "Mesh.h"
+#include "Utilities.h"
+#include "ViewScreen.h"
 
 #include <cstring>
 #include <memory>
@@ -110,9 +112,158 @@ void setup(Run &R, const foort_run_desc *d, std::unique_ptr<Mesh> mesh)
         std::move(mesh), R.metric.get()));
 }
 
+// The reference's own meshes, by FOORT_MESH_* code (InputCertainPixelsMesh, which
+// reads the console, is left out).
+std::unique_ptr<Mesh> make_ref_mesh(int type, uint64_t pixels, uint64_t max_pixels, uint64_t iteration, int max_sub,
+                                    int to_final, DiagBitflag val)
+{
+    switch (type) {
+    case 0: return std::unique_ptr<Mesh>(new SimpleSquareMesh(pixels, val));
+    case 2:
+        return std::unique_ptr<Mesh>(new SquareSubdivisionMesh(max_pixels, pixels, max_sub, iteration, to_final != 0, val));
+    case 3:
+        return std::unique_ptr<Mesh>(new SquareSubdivisionMeshV2(max_pixels, pixels, max_sub, iteration, to_final != 0, val));
+    default: return nullptr;
+    }
+}
+
+void copy_text(const std::string &s, char *buf, size_t cap)
+{
+    if (!buf || !cap)
+        return;
+    size_t n = std::min(cap - 1, s.size());
+    std::memcpy(buf, s.data(), n);
+    buf[n] = '\0';
+}
+
 } // namespace
 
 extern "C" {
+
+// Drive a reference mesh with values read from a grid: every iteration's queue of
+// screen indices is appended to out_idx as (row, col) pairs and each geodesic
+// reports value_grid[row * grid_side + col] to Mesh::GeodesicFinished.  Returns
+// the number of pixels handed out, -1 if out_idx or iter_counts is too small,
+// -2 if a screen index falls outside the grid.
+long long ref_mesh_run(int type, uint64_t pixels, uint64_t max_pixels, uint64_t iteration, int max_sub, int to_final,
+                       uint32_t valdiag, const double *value_grid, uint64_t grid_side, uint64_t *out_idx, uint64_t cap,
+                       uint64_t *iter_counts, int iter_cap, int *n_iters, char *desc, size_t desc_cap)
+{
+    SetOutputLevel(OutputLevel::Level_0_WARNING);
+    EquatorialPassesDiagnostic::DiagOptions = std::make_unique<EquatorialPassesOptions>(0.01, UpdateFrequency{1, true, true});
+    ClosestRadiusDiagnostic::DiagOptions = std::make_unique<ClosestRadiusOptions>(false, UpdateFrequency{1, true, true});
+    std::unique_ptr<Mesh> mesh =
+        make_ref_mesh(type, pixels, max_pixels, iteration, max_sub, to_final, static_cast<DiagBitflag>(valdiag));
+    if (!mesh)
+        return -3;
+    copy_text(mesh->getFullDescriptionStr(), desc, desc_cap);
+    long long total = 0;
+    int it = 0;
+    while (!mesh->IsFinished()) {
+        const uint64_t n = mesh->getCurNrGeodesics();
+        if (it >= iter_cap || total + (long long)n > (long long)cap)
+            return -1;
+        iter_counts[it++] = n;
+        for (uint64_t i = 0; i < n; ++i) {
+            ScreenPoint u;
+            ScreenIndex s;
+            mesh->getNewInitConds(i, u, s);
+            if (s[0] >= grid_side || s[1] >= grid_side)
+                return -2;
+            out_idx[2 * (total + i)] = s[0];
+            out_idx[2 * (total + i) + 1] = s[1];
+            mesh->GeodesicFinished(i, std::vector<real>{value_grid[s[0] * grid_side + s[1]]});
+        }
+        total += n;
+        mesh->EndCurrentLoop();
+    }
+    *n_iters = it;
+    return total;
+}
+
+// Utilities::GetFirstLineInfoString for a run description and one of the reference meshes.
+void ref_first_line(const foort_run_desc *d, int mesh_type, uint64_t pixels, uint64_t max_pixels, uint64_t iteration,
+                    int max_sub, int to_final, char *buf, size_t cap)
+{
+    Run R;
+    setup(R, d, make_ref_mesh(mesh_type, pixels, max_pixels, iteration, max_sub, to_final,
+                              static_cast<DiagBitflag>(d->value_diag)));
+    Integrators::IntegratorDescription = d->integrator == FOORT_INTEGRATOR_VERLET ? "Verlet" : "RK4";
+    copy_text(Utilities::GetFirstLineInfoString(R.metric.get(), R.source.get(), R.diags, R.valdiag, R.terms, R.view.get()),
+              buf, cap);
+}
+
+// Drive the reference GeodesicOutputHandler: batch b holds batch_sizes[b] geodesics; geodesic i of
+// batch b has the screen index text "b i " and, for diagnostic k, the text "v<b>_<i>_<k>".
+void ref_output_run(const char *prefix, const char *timestamp, const char *ext, const char *const *names, int ndiag,
+                    uint64_t to_cache, uint64_t per_file, const char *first_line, const uint64_t *batch_sizes, int nbatch)
+{
+    SetOutputLevel(OutputLevel::Level_0_WARNING);
+    GeodesicOutputHandler h(prefix, timestamp, ext, std::vector<std::string>(names, names + ndiag), to_cache, per_file,
+                            first_line);
+    for (int b = 0; b < nbatch; ++b) {
+        h.PrepareForOutput(batch_sizes[b]);
+        for (uint64_t i = 0; i < batch_sizes[b]; ++i) {
+            std::vector<std::string> v{std::to_string(b) + " " + std::to_string(i) + " "};
+            for (int k = 0; k < ndiag; ++k)
+                v.push_back("v" + std::to_string(b) + "_" + std::to_string(i) + "_" + std::to_string(k));
+            h.NewGeodesicOutput(i, std::move(v));
+        }
+    }
+    h.OutputFinished();
+}
+
+// The reference program (FOORT/Main.cpp:205-427) on a run description, one of the
+// reference meshes and a file output handler: every mesh iteration integrates its
+// pixels (in parallel), then hands the values to the mesh and the texts to the
+// output handler in index order.  Returns the number of geodesics integrated.
+long long ref_program_run(const foort_run_desc *d, int mesh_type, uint64_t pixels, uint64_t max_pixels,
+                          uint64_t iteration, int max_sub, int to_final, const char *prefix, const char *ext,
+                          uint64_t to_cache, uint64_t per_file, int first_line_on, int nthreads)
+{
+    Run R;
+    setup(R, d, make_ref_mesh(mesh_type, pixels, max_pixels, iteration, max_sub, to_final,
+                              static_cast<DiagBitflag>(d->value_diag)));
+    Integrators::IntegratorDescription = d->integrator == FOORT_INTEGRATOR_VERLET ? "Verlet" : "RK4";
+    std::string first = first_line_on ? Utilities::GetFirstLineInfoString(R.metric.get(), R.source.get(), R.diags,
+                                                                           R.valdiag, R.terms, R.view.get())
+                                      : std::string();
+    GeodesicOutputHandler out(prefix, "", ext, Utilities::GetDiagNameStrings(R.diags, R.valdiag), to_cache, per_file,
+                              first);
+    long long total = 0;
+    while (!R.view->IsFinished()) {
+        const long long n = static_cast<long long>(R.view->getCurNrGeodesics());
+        out.PrepareForOutput(static_cast<largecounter>(n));
+        std::vector<std::vector<real>> values(n);
+        std::vector<std::vector<std::string>> texts(n);
+#pragma omp parallel num_threads(nthreads)
+        {
+            Geodesic geo(R.metric.get(), R.source.get(), R.diags, R.valdiag, R.terms, R.integ);
+#pragma omp for schedule(dynamic, 1)
+            for (long long i = 0; i < n; ++i) {
+                Point p0;
+                OneIndex v0;
+                ScreenIndex s;
+                R.view->SetNewInitialConditions(static_cast<largecounter>(i), p0, v0, s);
+                geo.Reset(s, p0, v0);
+                while (geo.getTermCondition() == Term::Continue)
+                    geo.Update();
+                values[i] = geo.getDiagnosticFinalValue();
+                texts[i] = geo.getAllOutputStr();
+            }
+        }
+        // the mesh's done-flags are a std::vector<bool>, so its callbacks run on one thread here
+        for (long long i = 0; i < n; ++i) {
+            R.view->GeodesicFinished(static_cast<largecounter>(i), values[i]);
+            out.NewGeodesicOutput(static_cast<largecounter>(i), texts[i]);
+        }
+        total += n;
+        R.view->EndCurrentLoop();
+    }
+    out.OutputFinished();
+    return total;
+}
+
 
 // Integrate pixels with the reference code: screen_index (row, col) pairs, or
 // grid indices first..first+count-1 of a SimpleSquareMesh(size*size).

@@ -8,7 +8,7 @@ from foort_amd import abi as A
 from foort_amd.workloads import ALL_DIAGS, ALL_TERMS, c0, c1  # noqa: F401
 
 
-def test_cfg(**kw):
+def johannsen_cfg(**kw):
     """FOORT/Test.cfg, which produced FOORT/Output/Test_*.dat: Johannsen a=0.7 alpha13=2,
     RLogScale, Verlet with tolerance -1, 5 deg, 15x15 screen, 250^2 pixels."""
     args = dict(metric=A.METRIC_JOHANNSEN, params=(0.7, 2.0, 0.0, 0.0, 0.0), rlog=True,
@@ -46,7 +46,7 @@ PARITY_CASES = {
     "kerr099_rlog_85deg_grid16": (lambda: c1(rlog=True, max_steps=10000), 16, None),
     "c0_kerr05_80deg_sample": (lambda: c0(), 128, sample_pixels(128, 24, 1)),
     "c1_kerr099_85deg_sample": (lambda: c1(), 1024, sample_pixels(1024, 24, 2)),
-    "test_cfg_johannsen_verlet": (lambda: test_cfg(), 250, sample_pixels(250, 192, 3)),
+    "test_cfg_johannsen_verlet": (lambda: johannsen_cfg(), 250, sample_pixels(250, 192, 3)),
     "debug_cfg_johannsen_rk4": (lambda: debug_cfg(), 250, sample_pixels(250, 64, 4)),
     "rasheed_larsen_rlog": (lambda: c0(metric=A.METRIC_RASHEEDLARSEN, params=(1.0, 0.6, 2.5, 3.0), rlog=True,
                                        inclination_deg=60.0, max_steps=20000), 12, None),

@@ -11,7 +11,7 @@ import pytest
 
 import oracle as O
 from foort_amd import abi as A
-from tests.cases import c0, c1, test_cfg
+from tests.cases import c0, c1, johannsen_cfg
 
 pytestmark = pytest.mark.gpu
 
@@ -29,7 +29,7 @@ def _compare(g, o):
 
 CASES = {
     "kerr099_rlog": (lambda **k: c1(rlog=True, max_steps=10000, **k), 32),
-    "test_cfg": (lambda **k: test_cfg(**k), 40),
+    "test_cfg": (lambda **k: johannsen_cfg(**k), 40),
     "rasheed_larsen": (lambda **k: c0(metric=A.METRIC_RASHEEDLARSEN, params=(1.0, 0.6, 2.5, 3.0), rlog=True,
                                       inclination_deg=60.0, max_steps=20000, **k), 24),
     "manko_novikov": (lambda **k: c0(metric=A.METRIC_MANKONOVIKOV, params=(0.5, 1.0), rlog=True,

@@ -133,8 +133,8 @@ def test_persistent_schedule_bitexact(budget):
 
 
 def test_persistent_pixel_list_verlet():
-    from tests.cases import test_cfg
-    desc = test_cfg(
+    from tests.cases import johannsen_cfg
+    desc = johannsen_cfg(
         schedule=A.SCHEDULE_PERSISTENT, lanes_per_cu=1, steps_per_launch=100)
     from tests.cases import sample_pixels
     px = sample_pixels(250, 700, 9)

@@ -1,0 +1,101 @@
+"""The drop-in program on the GPU (`foort <file.cfg>`, libfoort_host.so over libfoort_gpu.so).
+
+1. The command line program on the settings of the reference's two shipped
+   runs writes FOORT/Output/{Test,debug}_FourColorScreen.dat and
+   _EquatPasses.dat byte for byte (expected bytes rebuilt from
+   tests/golden/reference_outputs.npz: the recorded first line, then
+   "row col  value" for the 250 x 250 grid).
+2. With an adaptive mesh the program writes the same files on the GPU as with
+   the CPU oracle (portable-math build) as its batch tracer: the GPU path is bit
+   for bit the oracle, so the mesh sees the same values and queues the same
+   pixels in every iteration.
+3. The persistent schedule gives the same files as the one-shot schedule.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+from foort_amd import program as P
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden")
+
+
+def _tree(root):
+    out = {}
+    for dp, _, fs in os.walk(root):
+        for f in fs:
+            p = os.path.join(dp, f)
+            out[os.path.relpath(p, root)] = open(p, "rb").read()
+    return out
+
+
+def _expected(header, arr):
+    side = arr.shape[0]
+    lines = [header] + [f"{r} {c}  {int(arr[r, c])}" for r in range(side) for c in range(side)]
+    return ("\n".join(lines) + "\n").encode()
+
+
+@pytest.mark.parametrize("run", ["Test", "debug"])
+def test_cli_reproduces_the_shipped_output_files(run, tmp_path):
+    print(f"[program] foort johannsen_{run.lower()}.cfg (250x250 Johannsen)", flush=True)
+    cfg = os.path.join(GOLD, f"johannsen_{run.lower()}.cfg")
+    r = subprocess.run([P.CLI_PATH, cfg, "--output-prefix", str(tmp_path / run), "--screen-output-level", "1"],
+                       capture_output=True, text=True, timeout=300)
+    print(r.stdout[-800:], flush=True)
+    assert r.returncode == 0, r.stderr
+    g = np.load(os.path.join(GOLD, "reference_outputs.npz"))
+    assert sorted(os.listdir(tmp_path)) == [f"{run}_EquatPasses.dat", f"{run}_FourColorScreen.dat"]
+    for diag, key in (("FourColorScreen", "fcs"), ("EquatPasses", "ep")):
+        got = (tmp_path / f"{run}_{diag}.dat").read_bytes()
+        want = _expected(str(g[f"{run}_{key}_header"]), g[f"{run}_{key}"])
+        assert got == want, f"{diag}: {sum(a != b for a, b in zip(got.splitlines(), want.splitlines()))} lines differ"
+
+
+ADAPTIVE = """
+Metric = {{ Name = "Kerr"; a = 0.94; RLogScale = true; }};
+Diagnostics = {{ FourColorScreen = {{ On = true; UseForMesh = true; }}; EquatorialPasses = {{ On = true; }};
+                ClosestRadius = {{ On = true; }}; }};
+Terminations = {{ Horizon = {{ On = true; }}; BoundarySphere = {{ On = true; SphereRadius = 1000.0; }};
+                 TimeOut = {{ On = true; MaxSteps = 100000; }}; }};
+ViewScreen = {{ Position = {{ r = 1000.0; theta = 1.0471975511965976; }}; ScreenSize = {{ x = 24.0; y = 24.0; }};
+               Mesh = {{ Type = "{mesh}"; InitialPixels = 1024; MaxPixels = 6000; IterationPixels = 150;
+                        MaxSubdivide = 4; }}; }};
+Integrator = {{ Type = "RK4"; }};
+Output = {{ FilePrefix = "unused"; FileExtension = "dat"; TimeStamp = false; ScreenOutputLevel = 0; }};
+"""
+
+
+def _fm_tracer(desc, side, si):
+    return O.trace("fm", desc, side, screen_index=si)
+
+
+@pytest.mark.parametrize("mesh", ["SquareSubdivisionMesh", "SquareSubdivisionMeshV2"])
+def test_adaptive_program_matches_oracle(mesh, tmp_path):
+    print(f"[program] adaptive {mesh} GPU vs oracle", flush=True)
+    cfg = tmp_path / "adaptive.cfg"
+    cfg.write_text(ADAPTIVE.format(mesh=mesh))
+    st = P.run(str(cfg), output_prefix=str(tmp_path / "gpu" / "run"))
+    assert st.geodesics > 1024
+    P.run(str(cfg), output_prefix=str(tmp_path / "cpu" / "run"), tracer=_fm_tracer)
+    gpu, cpu = _tree(tmp_path / "gpu"), _tree(tmp_path / "cpu")
+    assert sorted(gpu) == sorted(cpu) == ["run_ClosestRadius.dat", "run_EquatPasses.dat", "run_FourColorScreen.dat"]
+    for k in gpu:
+        assert gpu[k] == cpu[k], k
+    n = len(gpu["run_FourColorScreen.dat"].splitlines()) - 1
+    assert n == st.geodesics
+
+
+def test_persistent_schedule_writes_the_same_files(tmp_path):
+    print("[program] persistent vs one-shot schedule", flush=True)
+    cfg = tmp_path / "adaptive.cfg"
+    cfg.write_text(ADAPTIVE.format(mesh="SquareSubdivisionMeshV2"))
+    P.run(str(cfg), output_prefix=str(tmp_path / "one" / "run"))
+    P.run(str(cfg), output_prefix=str(tmp_path / "per" / "run"), schedule=1)
+    one, per = _tree(tmp_path / "one"), _tree(tmp_path / "per")
+    assert one == per

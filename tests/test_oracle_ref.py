@@ -14,7 +14,7 @@ import pytest
 
 import oracle as O
 from foort_amd import abi as A
-from tests.cases import FIELDS, NO_REFERENCE_COUNTERPART, PARITY_CASES, bitwise_equal, debug_cfg, test_cfg
+from tests.cases import FIELDS, NO_REFERENCE_COUNTERPART, PARITY_CASES, bitwise_equal, debug_cfg, johannsen_cfg
 
 def _gold(name):
     import os
@@ -24,7 +24,7 @@ def _gold(name):
 @pytest.mark.parametrize("run,kind", [("Test", "libm"), ("Test", "fm"), ("debug", "libm")])
 def test_reference_output_files_reproduced(run, kind):
     g = _gold("reference_outputs.npz")
-    desc = test_cfg() if run == "Test" else debug_cfg()
+    desc = johannsen_cfg() if run == "Test" else debug_cfg()
     o = O.trace(kind, desc, 250)
     for key, field in (("fcs", "quadrant"), ("ep", "equat_passes")):
         ref = g[f"{run}_{key}"].astype(np.int32)
