@@ -114,6 +114,8 @@ def main():
     ap.add_argument("--steps-per-launch", type=int, default=0, help="persistent schedule budget (0 = default)")
     ap.add_argument("--block", type=int, default=0, help="one-shot block size (0 = default)")
     ap.add_argument("--refill", type=int, default=0, help="persistent schedule refill threshold (0 = default)")
+    ap.add_argument("--interval", type=int, default=0, help="persistent schedule steps between refills (0 = default)")
+    ap.add_argument("--launch-us", type=int, default=0, help="persistent schedule time per launch (0 = unbounded)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
@@ -129,7 +131,8 @@ def main():
     path = dict(christoffel=A.CHRISTOFFEL_FD if args.christoffel == "fd" else A.CHRISTOFFEL_ANALYTIC,
                 precision=A.PRECISION_FP64 if args.precision == "fp64" else A.PRECISION_MIXED,
                 schedule=A.SCHEDULE_ONESHOT if args.schedule == "oneshot" else A.SCHEDULE_PERSISTENT,
-                steps_per_launch=args.steps_per_launch, block_size=args.block, refill_lanes=args.refill)
+                steps_per_launch=args.steps_per_launch, block_size=args.block, refill_lanes=args.refill,
+                refill_interval=args.interval, launch_us=args.launch_us)
     desc = W.c1(**path)
     ref_desc = W.c1()
     res = F.device_results(n)

@@ -42,8 +42,9 @@ class RunDesc(C.Structure):
         ("camera_position", C.c_double * 4), ("screen_size", C.c_double * 2), ("screen_center", C.c_double * 2),
         ("step_cap", C.c_uint64), ("schedule", C.c_int32), ("block_size", C.c_int32),
         ("steps_per_launch", C.c_uint32), ("lanes_per_cu", C.c_uint32),
-        ("refill_lanes", C.c_uint32), ("reserved0", C.c_uint32),
+        ("refill_lanes", C.c_uint32), ("refill_interval", C.c_uint32),
         ("mixed_switch_radius", C.c_double), ("mixed_pole_margin", C.c_double),
+        ("launch_us", C.c_uint32), ("reserved1", C.c_uint32),
     ]
 
 

@@ -211,7 +211,8 @@ struct foort_gpu_ctx {
     // persistent schedule
     LaneSlots slots{};
     void *d_slots = nullptr;
-    uint32_t budget = 0;
+    uint32_t budget = 0;          // steps per launch
+    int clock_khz = 0;            // wall-clock rate (for the launch time limit)
     uint64_t last_launches = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
@@ -281,6 +282,10 @@ int setup_persistent(foort_gpu_ctx *ctx)
     ctx->slots.done = ctx->d_counters + 2;
     ctx->slots.L = static_cast<uint32_t>(L);
     ctx->slots.refill = ctx->desc.refill_lanes ? std::min(64u, ctx->desc.refill_lanes) : 1u;
+    ctx->slots.interval = std::min(ctx->budget, ctx->desc.refill_interval ? ctx->desc.refill_interval : 32u);
+    ctx->slots.ticks = static_cast<uint64_t>(ctx->desc.launch_us) * static_cast<uint64_t>(ctx->clock_khz) / 1000u;
+    if (ctx->desc.launch_us && ctx->slots.ticks == 0)
+        ctx->slots.ticks = 1;
     return FOORT_OK;
 }
 
@@ -313,10 +318,11 @@ int launch(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, uint64_t count, co
     HIP_TRY(ctx, hipGetLastError());
     if (count == 0)
         return FOORT_OK;
-    // every launch advances each busy lane by `budget` steps; no geodesic is longer than
-    // step_cap, so the batch needs at most (count / L + 1) * (step_cap / budget + 1) launches
+    // every launch advances each busy lane by at least min(budget, interval) steps (or to the end
+    // of its geodesic); no geodesic is longer than step_cap, so the launches are bounded
+    const uint64_t per_launch = S.ticks ? S.interval : ctx->budget;
     const uint64_t max_launches =
-        (count / S.L + 2) * (static_cast<uint64_t>(ctx->rc.step_cap) / ctx->budget + 2) + 16;
+        (count / S.L + 2) * (static_cast<uint64_t>(ctx->rc.step_cap) / per_launch + 2) + 16;
     unsigned long long done = 0;
     uint64_t n = 0;
     while (done < count) {
@@ -537,14 +543,23 @@ int foort_gpu_open(const foort_run_desc *desc, foort_gpu_ctx **out)
     R.mixed_u = R.M.rlog ? std::log(d.mixed_switch_radius) : d.mixed_switch_radius;
     R.mixed_pole = d.mixed_pole_margin;
     build_camera(d.metric, R, d.camera_position);
+    // kernels without update counters when every termination and diagnostic updates every step
+    // (UpdateFrequency 1; for a termination 0 means the same, FOORT/Terminations.cpp:71-90)
+    auto term_every = [&R](uint32_t flag, uint32_t n) { return !(R.terms & flag) || n <= 1; };
+    const bool every = term_every(FOORT_TERMFLAG_HORIZON, R.h_n) && term_every(FOORT_TERMFLAG_BOUNDARYSPHERE, R.s_n) &&
+                       term_every(FOORT_TERMFLAG_TIMEOUT, R.t_n) && term_every(FOORT_TERMFLAG_THETASINGULARITY, R.th_n) &&
+                       (!(R.diags & FOORT_DIAGFLAG_EQUATORIALPASSES) || R.ep_n == 1) &&
+                       (!(R.diags & FOORT_DIAGFLAG_CLOSESTRADIUS) || R.cr_n == 1);
     ctx->kernels = d.christoffel == FOORT_CHRISTOFFEL_FD
-                       ? foort::pick_policy<foort::RhsFD>(d.metric, d.integrator)
-                       : foort::pick_analytic(d.metric, d.integrator, static_cast<int>(d.precision));
-    ctx->budget = d.steps_per_launch ? d.steps_per_launch : 8192u;
+                       ? foort::pick_policy<foort::RhsFD>(d.metric, d.integrator, every)
+                       : foort::pick_analytic(d.metric, d.integrator, static_cast<int>(d.precision), every);
+    ctx->budget = d.steps_per_launch ? d.steps_per_launch : 0xffffffffu;
 
     hipError_t e = hipGetDevice(&ctx->device);
     if (e == hipSuccess)
         e = hipDeviceGetAttribute(&ctx->num_cu, hipDeviceAttributeMultiprocessorCount, ctx->device);
+    if (e == hipSuccess)
+        e = hipDeviceGetAttribute(&ctx->clock_khz, hipDeviceAttributeWallClockRate, ctx->device);
     if (e == hipSuccess)
         e = hipMalloc(reinterpret_cast<void **>(&ctx->d_counters), 4 * sizeof(unsigned long long));
     if (e == hipSuccess)

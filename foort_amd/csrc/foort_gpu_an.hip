@@ -7,11 +7,11 @@
 
 namespace foort {
 
-KernelPair pick_analytic(int metric, int integ, int precision)
+KernelPair pick_analytic(int metric, int integ, int precision, bool every)
 {
     if (precision == FOORT_PRECISION_MIXED)
-        return pick_policy<RhsMixed>(metric, integ);
-    return pick_policy<RhsAn64>(metric, integ);
+        return pick_policy<RhsMixed>(metric, integ, every);
+    return pick_policy<RhsAn64>(metric, integ, every);
 }
 
 } // namespace foort

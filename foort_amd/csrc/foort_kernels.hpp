@@ -42,6 +42,8 @@ struct LaneSlots {
     unsigned long long *done;  // pixels finished
     uint32_t L;
     uint32_t refill;           // refill idle lanes once at least this many of a wavefront are idle
+    uint32_t interval;         // steps between refills (the length of the inner step loop)
+    uint64_t ticks;            // wall-clock ticks after which a launch ends (0 = no time limit)
 };
 
 __device__ __forceinline__ void store_result(const DevResults &out, uint64_t i, const GeoState &g)
@@ -99,7 +101,7 @@ __device__ __forceinline__ void add_steps(unsigned long long *total, unsigned lo
 
 // One lane per pixel, integrated to termination: the drop-in for the
 // per-thread loop of FOORT/Main.cpp:373-400.
-template <class RHS, int INTEG>
+template <class RHS, int INTEG, bool EVERY>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FOORT_WPE_ONESHOT, FOORT_WPE_ONESHOT)))
 trace_oneshot(const RunConst R, uint64_t side, uint64_t first, uint64_t count,
                                                      const uint64_t *__restrict__ si, DevResults out,
@@ -112,9 +114,9 @@ trace_oneshot(const RunConst R, uint64_t side, uint64_t first, uint64_t count,
         double ux, uy;
         pixel_unit(side, first, si, i, ux, uy);
         GeoState g;
-        geo_init(R, ux, uy, g);
+        geo_init<EVERY>(R, ux, uy, g);
         while (g.term == FOORT_TERM_CONTINUE)
-            geo_update<RHS, INTEG>(R, g);
+            geo_update<RHS, INTEG, EVERY>(R, g);
         store_result(out, i, g);
         mine = g.steps;
     }
@@ -169,14 +171,17 @@ __device__ __forceinline__ void slot_save(const LaneSlots &S, uint32_t l, const 
     S.u[10 * L + l] = static_cast<uint32_t>(g.passes);
 }
 
-// Persistent work-stealing schedule.  L resident lanes; a lane whose geodesic
-// terminates writes its result and takes the next pixel of the batch, the
-// lanes of a wavefront claiming their pixels with one atomic (ballot + prefix
-// count).  A launch advances every busy lane by at most `budget` steps and
-// parks the lane state in HBM, so every launch is bounded; a wavefront leaves
-// as soon as none of its lanes has work.  The host relaunches until `done`
-// reaches the batch size.
-template <class RHS, int INTEG>
+// Persistent work-stealing schedule.  L resident lanes run the one-shot
+// kernel's step loop for `interval` steps at a time; between those runs, lanes
+// whose geodesic terminated write their result and, once at least `refill`
+// lanes of the wavefront are idle, take the next pixels of the batch, the
+// lanes of a wavefront claiming them with one atomic (ballot + prefix count).
+// A launch advances every busy lane by at most `budget` steps, or stops once
+// `ticks` of the wall clock have passed (checked between step runs, so all
+// wavefronts stop together), and parks the lane state in HBM; a wavefront
+// leaves as soon as none of its lanes has work.  The host relaunches until `done` reaches the
+// batch size.  The steps counted are those of the geodesics that finish.
+template <class RHS, int INTEG, bool EVERY>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FOORT_WPE_PERSIST, FOORT_WPE_PERSIST)))
 trace_persistent(const RunConst R, uint64_t side, uint64_t first,
                                                         uint64_t count, const uint64_t *__restrict__ si,
@@ -189,11 +194,15 @@ trace_persistent(const RunConst R, uint64_t side, uint64_t first,
     long long pix = S.pix[l];
     if (pix >= 0)
         slot_load(S, l, g);
+    else
+        g.term = -1; // no geodesic: the step loop below skips the lane
     bool exhausted = false;
     unsigned long long mine = 0;
-    for (uint32_t it = 0; it < budget; ++it) {
+    uint64_t it = 0;
+    const uint64_t t0 = S.ticks ? wall_clock64() : 0;
+    for (;;) {
+        // refill: the idle lanes of the wavefront claim consecutive pixels with one atomic
         const unsigned long long need = __ballot(pix < 0 && !exhausted);
-        // refill once enough lanes are idle (or when the wavefront has nothing left to do)
         if (need && (static_cast<uint32_t>(__popcll(need)) >= S.refill || !__any(pix >= 0))) {
             const int leader = __ffsll(static_cast<long long>(need)) - 1;
             unsigned long long base = 0;
@@ -206,23 +215,26 @@ trace_persistent(const RunConst R, uint64_t side, uint64_t first,
                     pix = static_cast<long long>(k);
                     double ux, uy;
                     pixel_unit(side, first, si, k, ux, uy);
-                    geo_init(R, ux, uy, g);
+                    geo_init<EVERY>(R, ux, uy, g);
                 } else {
                     exhausted = true;
                 }
             }
         }
-        if (!__any(pix >= 0))
+        if (!__any(pix >= 0) || it >= budget ||
+            (S.ticks && static_cast<uint64_t>(wall_clock64()) - t0 >= S.ticks))
             break;
-        bool finished = false;
-        if (pix >= 0) {
-            geo_update<RHS, INTEG>(R, g);
-            ++mine;
-            if (g.term != FOORT_TERM_CONTINUE) {
-                store_result(out, static_cast<uint64_t>(pix), g);
-                pix = -1;
-                finished = true;
-            }
+        // up to `interval` steps: the step loop of the one-shot kernel
+        const uint32_t n = budget - it < S.interval ? static_cast<uint32_t>(budget - it) : S.interval;
+        for (uint32_t k = 0; k < n && g.term == FOORT_TERM_CONTINUE; ++k)
+            geo_update<RHS, INTEG, EVERY>(R, g);
+        it += n;
+        // lanes whose geodesic ended write it out and become idle
+        const bool finished = pix >= 0 && g.term != FOORT_TERM_CONTINUE;
+        if (finished) {
+            store_result(out, static_cast<uint64_t>(pix), g);
+            mine += g.steps;
+            pix = -1;
         }
         const unsigned long long fin = __ballot(finished);
         if (fin && static_cast<int>(lane_id) == __ffsll(static_cast<long long>(fin)) - 1)
@@ -242,32 +254,35 @@ struct KernelPair {
     KernelFn persistent;
 };
 
+// `every`: every termination and diagnostic updates on every step (the kernels
+// without update counters)
 template <class RHS, int INTEG>
-KernelPair kernel_pair()
+KernelPair kernel_pair(bool every)
 {
-    return KernelPair{trace_oneshot<RHS, INTEG>, trace_persistent<RHS, INTEG>};
+    return every ? KernelPair{trace_oneshot<RHS, INTEG, true>, trace_persistent<RHS, INTEG, true>}
+                 : KernelPair{trace_oneshot<RHS, INTEG, false>, trace_persistent<RHS, INTEG, false>};
 }
 
 template <template <int> class POLICY, int M>
-KernelPair pick_integ(int integ)
+KernelPair pick_integ(int integ, bool every)
 {
-    return integ == FOORT_INTEGRATOR_VERLET ? kernel_pair<POLICY<M>, FOORT_INTEGRATOR_VERLET>()
-                                            : kernel_pair<POLICY<M>, FOORT_INTEGRATOR_RK4>();
+    return integ == FOORT_INTEGRATOR_VERLET ? kernel_pair<POLICY<M>, FOORT_INTEGRATOR_VERLET>(every)
+                                            : kernel_pair<POLICY<M>, FOORT_INTEGRATOR_RK4>(every);
 }
 
 template <template <int> class POLICY>
-KernelPair pick_policy(int metric, int integ)
+KernelPair pick_policy(int metric, int integ, bool every)
 {
     switch (metric) {
-    case FOORT_METRIC_KERR: return pick_integ<POLICY, FOORT_METRIC_KERR>(integ);
-    case FOORT_METRIC_FLATSPACE: return pick_integ<POLICY, FOORT_METRIC_FLATSPACE>(integ);
-    case FOORT_METRIC_RASHEEDLARSEN: return pick_integ<POLICY, FOORT_METRIC_RASHEEDLARSEN>(integ);
-    case FOORT_METRIC_JOHANNSEN: return pick_integ<POLICY, FOORT_METRIC_JOHANNSEN>(integ);
-    default: return pick_integ<POLICY, FOORT_METRIC_MANKONOVIKOV>(integ);
+    case FOORT_METRIC_KERR: return pick_integ<POLICY, FOORT_METRIC_KERR>(integ, every);
+    case FOORT_METRIC_FLATSPACE: return pick_integ<POLICY, FOORT_METRIC_FLATSPACE>(integ, every);
+    case FOORT_METRIC_RASHEEDLARSEN: return pick_integ<POLICY, FOORT_METRIC_RASHEEDLARSEN>(integ, every);
+    case FOORT_METRIC_JOHANNSEN: return pick_integ<POLICY, FOORT_METRIC_JOHANNSEN>(integ, every);
+    default: return pick_integ<POLICY, FOORT_METRIC_MANKONOVIKOV>(integ, every);
     }
 }
 
 // defined in foort_gpu_an.hip (analytic binary64 and mixed right-hand sides)
-KernelPair pick_analytic(int metric, int integ, int precision);
+KernelPair pick_analytic(int metric, int integ, int precision, bool every);
 
 } // namespace foort

@@ -58,15 +58,16 @@ FM_HD_INLINE double sexp(double x) { return fm_exp(x); }
 FM_HD_INLINE float ssqrt(float x) { return __builtin_sqrtf(x); }
 FM_HD_INLINE double ssqrt(double x) { return __builtin_sqrt(x); }
 FM_HD_INLINE void ssincos(double x, double *s, double *c) { fm_sincos(x, s, c); }
-FM_HD_INLINE void ssincos(double x, float *s, float *c)
+FM_HD_INLINE void ssincos(float x, float *s, float *c)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
-    __sincosf(static_cast<float>(x), s, c);
+    __sincosf(x, s, c);
 #else
-    *s = __builtin_sinf(static_cast<float>(x));
-    *c = __builtin_cosf(static_cast<float>(x));
+    *s = __builtin_sinf(x);
+    *c = __builtin_cosf(x);
 #endif
 }
+FM_HD_INLINE void ssincos(double x, float *s, float *c) { ssincos(static_cast<float>(x), s, c); }
 
 template <class S>
 FM_HD_INLINE D2<S> fexp(const D2<S> &a)
@@ -485,8 +486,8 @@ FM_HD_INLINE double srcp(double x) { return 1.0 / x; }
 // written with Sigma = r^2 + a^2 cos^2, Delta = r^2 - 2r + a^2,
 // q = Sigma - 2 r^2; then the Euler-Lagrange contraction.  With RLogScale the
 // radial coordinate is u = log r: v^r = r v^u and a^u = a^r / r - (v^u)^2.
-template <class S>
-FM_HD_INLINE void rhs_kerr_closed(const RunConst &R, const double *p, const double *v, double *out)
+template <class S, class T>
+FM_HD_INLINE void rhs_kerr_closed(const RunConst &R, const T *p, const T *v, T *out)
 {
     S s, c;
     ssincos(p[2], &s, &c);
@@ -534,21 +535,21 @@ FM_HD_INLINE void rhs_kerr_closed(const RunConst &R, const double *p, const doub
     const S vv[4] = {static_cast<S>(v[0]), vr, static_cast<S>(v[2]), static_cast<S>(v[3])};
     S o[4];
     kj_contract<S>(d1, d2, uu, vv, o);
-    out[0] = static_cast<double>(o[0]);
-    out[2] = static_cast<double>(o[2]);
-    out[3] = static_cast<double>(o[3]);
+    out[0] = static_cast<T>(o[0]);
+    out[2] = static_cast<T>(o[2]);
+    out[3] = static_cast<T>(o[3]);
     if (R.M.rlog) {
-        const double vu = v[1];
-        out[1] = static_cast<double>(o[1] * srcp(r)) - vu * vu;
+        const T vu = v[1];
+        out[1] = static_cast<T>(o[1] * srcp(r)) - vu * vu;
     } else {
-        out[1] = static_cast<double>(o[1]);
+        out[1] = static_cast<T>(o[1]);
     }
 }
 
 // Analytic right-hand side: the metric and its exact derivatives in one
 // dual-number pass (scalar type S), then the same sparse contraction.
-template <int METRIC, class S>
-FM_HD_INLINE void rhs_an(const RunConst &R, const double *p, const double *v, double *out)
+template <int METRIC, class S, class T>
+FM_HD_INLINE void rhs_an(const RunConst &R, const T *p, const T *v, T *out)
 {
     S s, c;
     ssincos(p[2], &s, &c);
@@ -570,7 +571,7 @@ FM_HD_INLINE void rhs_an(const RunConst &R, const double *p, const double *v, do
     kj_contract<S>(d1, d2, uu, vv, o);
 #pragma unroll
     for (int k = 0; k < 4; ++k)
-        out[k] = static_cast<double>(o[k]);
+        out[k] = static_cast<T>(o[k]);
 }
 
 // Right-hand-side policies: FD = the reference's finite differences (exact
@@ -580,25 +581,27 @@ template <int METRIC>
 struct RhsFD {
     static constexpr int metric = METRIC;
     static constexpr bool exact = true;
+    static constexpr bool split_rk4 = false;
     FM_HD_INLINE static void eval(const RunConst &R, const double *p, const double *v, double *out)
     {
         rhs_fd<METRIC>(R, p, v, out);
     }
 };
 
-template <int METRIC, class S>
-FM_HD_INLINE void rhs_analytic(const RunConst &R, const double *p, const double *v, double *out)
+template <int METRIC, class S, class T = double>
+FM_HD_INLINE void rhs_analytic(const RunConst &R, const T *p, const T *v, T *out)
 {
     if constexpr (METRIC == FOORT_METRIC_KERR)
-        rhs_kerr_closed<S>(R, p, v, out);
+        rhs_kerr_closed<S, T>(R, p, v, out);
     else
-        rhs_an<METRIC, S>(R, p, v, out);
+        rhs_an<METRIC, S, T>(R, p, v, out);
 }
 
 template <int METRIC>
 struct RhsAn64 {
     static constexpr int metric = METRIC;
     static constexpr bool exact = false;
+    static constexpr bool split_rk4 = false;
     FM_HD_INLINE static void eval(const RunConst &R, const double *p, const double *v, double *out)
     {
         rhs_analytic<METRIC, double>(R, p, v, out);
@@ -609,11 +612,15 @@ template <int METRIC>
 struct RhsMixed {
     static constexpr int metric = METRIC;
     static constexpr bool exact = false;
-    FM_HD_INLINE static void eval(const RunConst &R, const double *p, const double *v, double *out)
+    static constexpr bool split_rk4 = true; // RK4 steps go through rk4_step_mixed
+    FM_HD_INLINE static bool far(const RunConst &R, const double *p)
     {
         const double th = p[2];
-        const bool far = p[1] > R.mixed_u && th > R.mixed_pole && (kPi - th) > R.mixed_pole;
-        if (far)
+        return p[1] > R.mixed_u && th > R.mixed_pole && (kPi - th) > R.mixed_pole;
+    }
+    FM_HD_INLINE static void eval(const RunConst &R, const double *p, const double *v, double *out)
+    {
+        if (far(R, p))
             rhs_analytic<METRIC, float>(R, p, v, out);
         else
             rhs_analytic<METRIC, double>(R, p, v, out);
@@ -689,6 +696,57 @@ FM_HD_INLINE void rk4_step(const RunConst &R, const double *x, const double *v, 
     hout = h;
 }
 
+// RK4 step of the mixed policy.  The precision is chosen once per step, from
+// the step's starting point.  In the far field all four stages run in binary32
+// and only their increments reach the binary64 state, combined as
+//   v' = v + (h/6)(k1 + 2 k2 + 2 k3 + k4),   x' = x + h v + (h^2/6)(k1 + k2 + k3),
+// which is the RK4 combination above (l2 - v = (h/2) k1, l3 - v = (h/2) k2,
+// l4 - v = h k3).  The step size stays binary64.  Near the hole and the poles
+// the step is the binary64 analytic one.
+template <int METRIC>
+FM_HD_INLINE void rk4_step_mixed(const RunConst &R, const double *x, const double *v, double *nx, double *nv,
+                                 double &hout)
+{
+    if (!RhsMixed<METRIC>::far(R, x)) {
+        rk4_step<RhsAn64<METRIC>>(R, x, v, nx, nv, hout);
+        return;
+    }
+    const double h = adaptive_step_fast(R, x, v);
+    const float hf = static_cast<float>(h), hh = 0.5f * hf;
+    float xf[4], vf[4], k1[4], k2[4], k3[4], k4[4], tp[4], l2[4], l3[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        xf[i] = static_cast<float>(x[i]);
+        vf[i] = static_cast<float>(v[i]);
+    }
+    rhs_analytic<METRIC, float, float>(R, xf, vf, k1);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        tp[i] = xf[i] + vf[i] * hh;
+        l2[i] = vf[i] + k1[i] * hh;
+    }
+    rhs_analytic<METRIC, float, float>(R, tp, l2, k2);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        tp[i] = xf[i] + l2[i] * hh;
+        l3[i] = vf[i] + k2[i] * hh;
+    }
+    rhs_analytic<METRIC, float, float>(R, tp, l3, k3);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        tp[i] = xf[i] + l3[i] * hf;
+        l2[i] = vf[i] + k3[i] * hf; // l4
+    }
+    rhs_analytic<METRIC, float, float>(R, tp, l2, k4);
+    const float h6 = static_cast<float>(h / 6.0), hsq6 = static_cast<float>(h * h / 6.0);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        nv[i] = v[i] + static_cast<double>((((k1[i] + k2[i] * 2.0f) + k3[i] * 2.0f) + k4[i]) * h6);
+        nx[i] = (x[i] + v[i] * h) + static_cast<double>(((k1[i] + k2[i]) + k3[i]) * hsq6);
+    }
+    hout = h;
+}
+
 // Verlet iterations are capped (the reference loops without bound); the CPU
 // oracle uses the same cap.
 constexpr int kVerletMaxIter = 1000;
@@ -749,9 +807,12 @@ struct GeoState {
 };
 
 // Termination::DecideUpdate, FOORT/Terminations.cpp:71-90
+// EVERY: every termination and diagnostic updates on every step (UpdateFrequency 1,
+// or 0 for a termination), known at compile time, so no counters are kept.
+template <bool EVERY>
 FM_HD_INLINE bool term_decide(uint32_t n, uint32_t &cnt)
 {
-    if (n == 0)
+    if (EVERY || n == 0)
         return true;
     ++cnt;
     if (cnt >= n) {
@@ -762,8 +823,11 @@ FM_HD_INLINE bool term_decide(uint32_t n, uint32_t &cnt)
 }
 
 // Diagnostic::DecideUpdate, FOORT/Diagnostics.cpp:113-140
+template <bool EVERY>
 FM_HD_INLINE bool diag_decide(uint32_t n, int32_t start, int32_t finish, uint32_t &cnt, const GeoState &g)
 {
+    if (EVERY)
+        return true;
     if (n > 0) {
         ++cnt;
         if (cnt >= n) {
@@ -794,6 +858,7 @@ FM_HD_INLINE int32_t four_color_quadrant(const double *pos)
     return (p3 < kPi) ? 3 : 4;
 }
 
+template <bool EVERY>
 FM_HD_INLINE void diag_update(const RunConst &R, GeoState &g)
 {
     if (R.diags & FOORT_DIAGFLAG_FOURCOLORSCREEN) {
@@ -801,7 +866,7 @@ FM_HD_INLINE void diag_update(const RunConst &R, GeoState &g)
             g.quadrant = four_color_quadrant(g.pos);
     }
     if (R.diags & FOORT_DIAGFLAG_EQUATORIALPASSES) { // FOORT/Diagnostics.cpp:351-380
-        if (diag_decide(R.ep_n, R.ep_start, R.ep_finish, g.epc, g)) {
+        if (diag_decide<EVERY>(R.ep_n, R.ep_start, R.ep_finish, g.epc, g)) {
             double cur = g.pos[2];
             if (__builtin_fabs(cur - kPi / 2.0) > kPi / 2.0 * R.ep_thr) {
                 if (g.prevtheta > 0 && ((g.prevtheta - kPi / 2.0) * (cur - kPi / 2.0)) < 0.0)
@@ -813,7 +878,7 @@ FM_HD_INLINE void diag_update(const RunConst &R, GeoState &g)
             g.passes = -g.passes;
     }
     if (R.diags & FOORT_DIAGFLAG_CLOSESTRADIUS) { // FOORT/Diagnostics.cpp:427-446
-        if (diag_decide(R.cr_n, R.cr_start, R.cr_finish, g.crc, g)) {
+        if (diag_decide<EVERY>(R.cr_n, R.cr_start, R.cr_finish, g.crc, g)) {
             double cur = true_r(R.M, g.pos[1]);
             if (g.closest < 0 || cur < g.closest)
                 g.closest = cur;
@@ -824,31 +889,32 @@ FM_HD_INLINE void diag_update(const RunConst &R, GeoState &g)
 }
 
 // Terminations in the order of CreateTerminationVector, FOORT/Terminations.cpp:22-40
+template <bool EVERY>
 FM_HD_INLINE int32_t check_terms(const RunConst &R, GeoState &g)
 {
     if (R.terms & FOORT_TERMFLAG_HORIZON) { // :96-117
-        if (term_decide(R.h_n, g.hc)) {
+        if (term_decide<EVERY>(R.h_n, g.hc)) {
             double r = true_r(R.M, g.pos[1]);
             if ((r < R.M.horizon * (1 + R.h_eps)) || (R.M.horizon == 0.0 && r < R.h_eps))
                 return FOORT_TERM_HORIZON;
         }
     }
     if (R.terms & FOORT_TERMFLAG_BOUNDARYSPHERE) { // :130-146
-        if (term_decide(R.s_n, g.bc)) {
+        if (term_decide<EVERY>(R.s_n, g.bc)) {
             double r = true_r(R.M, g.pos[1]);
             if (r > R.sphere_R)
                 return FOORT_TERM_BOUNDARYSPHERE;
         }
     }
     if (R.terms & FOORT_TERMFLAG_TIMEOUT) { // :168-186
-        if (term_decide(R.t_n, g.tc)) {
+        if (term_decide<EVERY>(R.t_n, g.tc)) {
             if (g.tsteps >= R.max_steps)
                 return FOORT_TERM_TIMEOUT;
             ++g.tsteps;
         }
     }
     if (R.terms & FOORT_TERMFLAG_THETASINGULARITY) { // :200-214
-        if (term_decide(R.th_n, g.thc)) {
+        if (term_decide<EVERY>(R.th_n, g.thc)) {
             double th = g.pos[2];
             if (__builtin_fabs(th) < R.theta_eps || __builtin_fabs(kPi - th) < R.theta_eps)
                 return FOORT_TERM_THETASINGULARITY;
@@ -858,6 +924,7 @@ FM_HD_INLINE int32_t check_terms(const RunConst &R, GeoState &g)
 }
 
 // ViewScreen::SetNewInitialConditions + Geodesic::Reset for the unit screen point (ux, uy)
+template <bool EVERY>
 FM_HD_INLINE void geo_init(const RunConst &R, double ux, double uy, GeoState &g)
 {
     g.pos[0] = R.cam_t;
@@ -894,16 +961,18 @@ FM_HD_INLINE void geo_init(const RunConst &R, double ux, double uy, GeoState &g)
     g.passes = 0;
     g.prevtheta = -1;
     g.closest = -1;
-    diag_update(R, g);
+    diag_update<EVERY>(R, g);
 }
 
 // Geodesic::Update, FOORT/Geodesic.cpp:61-88
-template <class RHS, int INTEG>
+template <class RHS, int INTEG, bool EVERY>
 FM_HD_INLINE void geo_update(const RunConst &R, GeoState &g)
 {
     double np[4], nv[4], h;
     if constexpr (INTEG == FOORT_INTEGRATOR_VERLET)
         verlet_step<RHS>(R, g.pos, g.vel, np, nv, h);
+    else if constexpr (RHS::split_rk4)
+        rk4_step_mixed<RHS::metric>(R, g.pos, g.vel, np, nv, h);
     else
         rk4_step<RHS>(R, g.pos, g.vel, np, nv, h);
     g.lambda += h;
@@ -913,11 +982,11 @@ FM_HD_INLINE void geo_update(const RunConst &R, GeoState &g)
         g.vel[i] = nv[i];
     }
     ++g.steps;
-    int32_t t = check_terms(R, g);
+    int32_t t = check_terms<EVERY>(R, g);
     if (t == FOORT_TERM_CONTINUE && g.steps >= R.step_cap)
         t = FOORT_TERM_STEPCAP;
     g.term = t;
-    diag_update(R, g);
+    diag_update<EVERY>(R, g);
 }
 
 } // namespace foort

@@ -125,13 +125,17 @@ typedef struct foort_run_desc {
     uint64_t step_cap;           /* per-geodesic ceiling on integration steps; 0 = 2^31-1 */
     int32_t schedule;            /* FOORT_SCHEDULE_* */
     int32_t block_size;          /* threads per workgroup; 0 = tuned default */
-    uint32_t steps_per_launch;   /* persistent schedule: steps per kernel launch; 0 = default */
+    uint32_t steps_per_launch;   /* persistent schedule: at most this many steps per kernel launch;
+                                    0 = no step limit */
     uint32_t lanes_per_cu;       /* persistent schedule: resident lanes per CU; 0 = default */
     uint32_t refill_lanes;       /* persistent schedule: a wavefront refills its idle lanes once at
                                     least this many are idle (1..64; 0 = default) */
-    uint32_t reserved0;
+    uint32_t refill_interval;    /* persistent schedule: steps a lane runs between refills (0 = default) */
     double mixed_switch_radius;  /* FOORT_PRECISION_MIXED: binary32 steps only where r > this (10) */
     double mixed_pole_margin;    /* ... and where min(theta, pi - theta) > this (0.1) */
+    uint32_t launch_us;          /* persistent schedule: end each kernel launch after about this many
+                                    microseconds (0 = no time limit; with both limits 0 a batch is one launch) */
+    uint32_t reserved1;
 } foort_run_desc;
 
 /*

@@ -114,21 +114,27 @@ def test_device_entry_point_matches_host():
     assert st.steps == h["stats"].steps and st.geodesics == n
 
 
-@pytest.mark.parametrize("budget", [37, 4096])
-def test_persistent_schedule_bitexact(budget):
+@pytest.mark.parametrize("budget,interval,launch_us", [(37, 0, 0), (4096, 0, 0), (0, 1, 0), (0, 300, 0),
+                                                       (0, 0, 100), (500, 7, 50)])
+def test_persistent_schedule_bitexact(budget, interval, launch_us):
     """Work-stealing lanes (256 lanes for 2304 pixels, so every lane refills
-    several times) with lane state parked in HBM between launches: results
-    must equal the oracle bit for bit."""
-    desc = c1(rlog=True, max_steps=10000, schedule=A.SCHEDULE_PERSISTENT, lanes_per_cu=1, steps_per_launch=budget)
+    several times) with lane state parked in HBM between launches, launches
+    bounded by steps or by time, refills every `interval` steps: results must
+    equal the oracle bit for bit."""
+    desc = c1(rlog=True, max_steps=10000, schedule=A.SCHEDULE_PERSISTENT, lanes_per_cu=1, steps_per_launch=budget,
+              refill_interval=interval, launch_us=launch_us)
     side = 48
-    print(f"[persistent] budget={budget}", flush=True)
+    print(f"[persistent] budget={budget} interval={interval} launch_us={launch_us}", flush=True)
     F = _gpu()
     with F.Context(desc) as ctx:
         g = ctx.trace_grid(side)
     o = O.trace("fm", desc, side)
     for k in FIELDS:
         assert bitwise_equal(g[k], o[k]), k
-    assert g["stats"].launches > (1 if budget == 37 else 0)
+    bounded = budget == 37 or launch_us > 0
+    assert g["stats"].launches > (1 if bounded else 0)
+    if budget == 0 and launch_us == 0:
+        assert g["stats"].launches == 1
     assert g["stats"].steps == o["steps"].sum()
 
 
