@@ -98,8 +98,82 @@ def agreement(ref, out):
     }
 
 
+def quiet(fn):
+    """Run fn with the process's stdout (file descriptor 1, which the C++ program
+    writes its messages to) sent to stderr, so stdout carries only the JSON line."""
+    sys.stdout.flush()
+    saved = os.dup(1)
+    os.dup2(2, 1)
+    try:
+        return fn()
+    finally:
+        os.dup2(saved, 1)
+        os.close(saved)
+
+
+def run_c2(args):
+    """C2: the whole adaptive run through the drop-in program (configuration, mesh
+    refinement on the host, one GPU batch per mesh iteration, output files in a
+    temporary directory).  One step = one complete run."""
+    import shutil
+    import tempfile
+    from foort_amd import program as P
+    from foort_amd import workloads as W
+    # (name, mesh, Christoffel, precision, schedule, autotune, graph); the first is the reported path:
+    # the reference's finite differences (bit-exact FP64), schedule and options picked by the autotuner
+    # on the first iteration, every mesh iteration one captured HIP graph
+    V2, V1 = "SquareSubdivisionMeshV2", "SquareSubdivisionMesh"
+    variants = [("fd_tuned_graph", V2, "FD", "FP64", "Persistent", "Schedule", True),
+                ("fd_oneshot", V2, "FD", "FP64", "OneShot", "None", False),
+                ("fd_persistent", V2, "FD", "FP64", "Persistent", "None", False),
+                ("fd_persistent_graph", V2, "FD", "FP64", "Persistent", "None", True),
+                ("analytic_fp64_tuned_graph", V2, "Analytic", "FP64", "Persistent", "Schedule", True),
+                ("mixed_tuned_graph", V2, "Analytic", "Mixed", "Persistent", "Schedule", True),
+                ("fd_tuned_graph_mesh_v1", V1, "FD", "FP64", "Persistent", "Schedule", True)]
+    if args.no_variants:
+        variants = variants[:1]
+    results = {}
+    tmp = tempfile.mkdtemp(prefix="foort_c2_")
+    # the autotuner's results are kept between runs of a configuration: the warm-up run tunes,
+    # the timed runs reuse its choice
+    os.environ["FOORT_TUNE_CACHE"] = os.path.join(tmp, "tune.txt")
+    try:
+        if args.warmup < 1:
+            log("c2: with --warmup 0 the tuned variants include the autotuner's own trials")
+        for name, mesh, chris, prec, sched, tune, graph in variants:
+            cfg = os.path.join(tmp, f"{name}.cfg")
+            with open(cfg, "w") as f:
+                f.write(W.c2_config(mesh=mesh, christoffel=chris, precision=prec, schedule=sched,
+                                    prefix=os.path.join(tmp, name), autotune=tune, use_graph=graph))
+            walls, gpu_ms = [], []
+            for i in range(args.warmup + args.steps):
+                t = time.perf_counter()
+                st = quiet(lambda: P.run(cfg, screen_output_level=2))
+                dt = time.perf_counter() - t
+                if i >= args.warmup:
+                    walls.append(dt)
+                    gpu_ms.append(st.device_ms)
+                log(f"c2 {name} run {i}: {st.geodesics} pixels, {st.steps} steps, {dt:.2f} s wall, "
+                    f"{st.device_ms / 1e3:.2f} s kernels")
+            wall = sum(walls) / len(walls)
+            results[name] = {"pixels": st.geodesics, "geodesic_steps": st.steps, "wall_s": wall,
+                             "kernel_s": sum(gpu_ms) / len(gpu_ms) / 1e3, "pixels_per_s": st.geodesics / wall,
+                             "geodesic_steps_per_s": st.steps / wall}
+        first = results[variants[0][0]]
+        line = {"metric": "pixels/s of a complete adaptive-mesh run", "value": first["pixels_per_s"],
+                "unit": "pixels/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                "ms_per_step": first["wall_s"] * 1e3, "higher_is_better": True, "scaling": "weak",
+                "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+                "config": {"workload": W.C2_TEXT, "path": variants[0][0]}, "variants": results}
+        print(json.dumps(line), flush=True)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", choices=["c1", "c2"], default="c1",
+                    help="c1: the 1024^2 drop-in image (default); c2: the adaptive-mesh run")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
@@ -121,6 +195,10 @@ def main():
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
               f"the sharding design is described in DESIGN.md", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "c2":
+        run_c2(args)
+        return
 
     import foort_amd as F
     from foort_amd import workloads as W

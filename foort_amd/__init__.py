@@ -22,7 +22,7 @@ EXPORTS = [
     "foort_desc_init", "foort_desc_size", "foort_gpu_open", "foort_gpu_close", "foort_gpu_last_error",
     "foort_gpu_trace_grid", "foort_gpu_trace_pixels", "foort_gpu_trace_device", "foort_gpu_camera",
     "foort_gpu_device_alloc", "foort_gpu_device_free", "foort_gpu_copy_to_device", "foort_gpu_copy_to_host",
-    "foort_gpu_synchronize",
+    "foort_gpu_synchronize", "foort_gpu_autotune",
 ]
 
 
@@ -49,6 +49,8 @@ def lib():
         L.foort_gpu_trace_device.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_void_p,
                                              C.POINTER(GeodesicResults), C.c_void_p, C.POINTER(Stats)]
         L.foort_gpu_camera.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.foort_gpu_autotune.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p, C.c_uint32,
+                                         C.POINTER(RunDesc), C.POINTER(Stats)]
         L.foort_gpu_device_alloc.argtypes = [C.c_size_t, C.POINTER(C.c_void_p)]
         L.foort_gpu_device_free.argtypes = [C.c_void_p]
         L.foort_gpu_copy_to_device.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
@@ -145,6 +147,19 @@ class Context:
                                                    C.byref(st) if st is not None else None),
                     "foort_gpu_trace_device")
         return st
+
+    def autotune(self, side, screen_index=None, count=None, tune_schedule=False):
+        """foort_gpu_autotune on a sample batch (a (row, column) list, or grid indices
+        0 .. count-1); returns (the context's descriptor afterwards, the sample's Stats)."""
+        chosen, st = RunDesc(), Stats()
+        if screen_index is not None:
+            si = np.ascontiguousarray(screen_index, dtype=np.uint64).reshape(-1, 2)
+            n, ptr = si.shape[0], si.ctypes.data_as(C.c_void_p)
+        else:
+            n, ptr = count, None
+        self._check(self._L.foort_gpu_autotune(self._h, side, n, ptr, 1 if tune_schedule else 0, C.byref(chosen),
+                                               C.byref(st)), "foort_gpu_autotune")
+        return chosen, st
 
     def camera(self):
         V, g0 = np.zeros(16), np.zeros(4)

@@ -44,7 +44,7 @@ class RunDesc(C.Structure):
         ("steps_per_launch", C.c_uint32), ("lanes_per_cu", C.c_uint32),
         ("refill_lanes", C.c_uint32), ("refill_interval", C.c_uint32),
         ("mixed_switch_radius", C.c_double), ("mixed_pole_margin", C.c_double),
-        ("launch_us", C.c_uint32), ("reserved1", C.c_uint32),
+        ("launch_us", C.c_uint32), ("use_graph", C.c_uint32),
     ]
 
 

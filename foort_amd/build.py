@@ -86,7 +86,7 @@ def build_host(verbose=False):
     libfoort_host.so over libfoort_gpu.so, and the `foort` command line program."""
     hdir = os.path.join(PKG, "host")
     inc = ["-I" + os.path.join(REPO, "include"), "-I" + hdir]
-    flags = ["-std=c++17", "-O2", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter"]
+    flags = ["-std=c++17", "-O2", "-fPIC", "-fopenmp", "-Wall", "-Wextra", "-Wno-unused-parameter"]
     _run(["g++"] + flags + inc + ["-shared"] + [os.path.join(hdir, s) for s in HOST_SOURCES]
          + ["-L" + PKG, "-lfoort_gpu", "-Wl,-rpath,$ORIGIN", "-o", HOST_LIB + ".tmp"], verbose)
     os.replace(HOST_LIB + ".tmp", HOST_LIB)

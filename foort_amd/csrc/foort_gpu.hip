@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "foort_gpu.h"
 #include "foort_kernels.hpp"
@@ -211,10 +212,17 @@ struct foort_gpu_ctx {
     // persistent schedule
     LaneSlots slots{};
     void *d_slots = nullptr;
-    uint32_t budget = 0;          // steps per launch
+    uint64_t budget = 0;          // steps per launch (UINT64_MAX: no limit)
     int clock_khz = 0;            // wall-clock rate (for the launch time limit)
     uint64_t last_launches = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // captured batches (use_graph): the context's own stream, pinned host staging with the
+    // layout of d_buf, and the executable graph of the last batch shape
+    hipStream_t stream = nullptr;
+    void *h_buf = nullptr;
+    size_t h_cap = 0;
+    hipGraphExec_t graph = nullptr;
+    std::string graph_key;
 };
 
 namespace {
@@ -244,6 +252,7 @@ int ensure_buffer(foort_gpu_ctx *ctx, size_t bytes)
 {
     if (bytes <= ctx->d_cap)
         return FOORT_OK;
+    bytes = std::max(bytes, ctx->d_cap + ctx->d_cap / 2); // grow geometrically
     if (ctx->d_buf)
         (void)hipFree(ctx->d_buf);
     ctx->d_buf = nullptr;
@@ -255,8 +264,45 @@ int ensure_buffer(foort_gpu_ctx *ctx, size_t bytes)
     return FOORT_OK;
 }
 
+int ensure_pinned(foort_gpu_ctx *ctx, size_t bytes)
+{
+    if (bytes <= ctx->h_cap)
+        return FOORT_OK;
+    bytes = std::max(bytes, ctx->h_cap + ctx->h_cap / 2);
+    if (ctx->h_buf)
+        (void)hipHostFree(ctx->h_buf);
+    ctx->h_buf = nullptr;
+    ctx->h_cap = 0;
+    hipError_t e = hipHostMalloc(&ctx->h_buf, bytes, hipHostMallocDefault);
+    if (e != hipSuccess)
+        return fail(ctx, FOORT_ENOMEM, std::string("hipHostMalloc of staging buffer: ") + hipGetErrorString(e));
+    ctx->h_cap = bytes;
+    return FOORT_OK;
+}
+
+void drop_graph(foort_gpu_ctx *ctx)
+{
+    if (ctx->graph)
+        (void)hipGraphExecDestroy(ctx->graph);
+    ctx->graph = nullptr;
+    ctx->graph_key.clear();
+}
+
+// the persistent schedule's options, from the descriptor (the autotuner changes them)
+void apply_lane_options(foort_gpu_ctx *ctx)
+{
+    ctx->budget = ctx->desc.steps_per_launch ? ctx->desc.steps_per_launch : UINT64_MAX;
+    ctx->slots.refill = ctx->desc.refill_lanes ? std::min(64u, ctx->desc.refill_lanes) : 1u;
+    ctx->slots.interval = static_cast<uint32_t>(
+        std::min<uint64_t>(ctx->budget, ctx->desc.refill_interval ? ctx->desc.refill_interval : 32u));
+    ctx->slots.ticks = static_cast<uint64_t>(ctx->desc.launch_us) * static_cast<uint64_t>(ctx->clock_khz) / 1000u;
+    if (ctx->desc.launch_us && ctx->slots.ticks == 0)
+        ctx->slots.ticks = 1;
+}
+
 int setup_persistent(foort_gpu_ctx *ctx)
 {
+    apply_lane_options(ctx);
     if (ctx->d_slots)
         return FOORT_OK;
     uint32_t lanes_per_cu = ctx->desc.lanes_per_cu;
@@ -281,14 +327,18 @@ int setup_persistent(foort_gpu_ctx *ctx)
     ctx->slots.next = ctx->d_counters + 1;
     ctx->slots.done = ctx->d_counters + 2;
     ctx->slots.L = static_cast<uint32_t>(L);
-    ctx->slots.refill = ctx->desc.refill_lanes ? std::min(64u, ctx->desc.refill_lanes) : 1u;
-    ctx->slots.interval = std::min(ctx->budget, ctx->desc.refill_interval ? ctx->desc.refill_interval : 32u);
-    ctx->slots.ticks = static_cast<uint64_t>(ctx->desc.launch_us) * static_cast<uint64_t>(ctx->clock_khz) / 1000u;
-    if (ctx->desc.launch_us && ctx->slots.ticks == 0)
-        ctx->slots.ticks = 1;
     return FOORT_OK;
 }
 
+// a batch is one kernel launch (and needs no host decisions between launches)
+bool single_launch(const foort_gpu_ctx *ctx)
+{
+    return ctx->desc.schedule == FOORT_SCHEDULE_ONESHOT ||
+           (ctx->desc.steps_per_launch == 0 && ctx->desc.launch_us == 0);
+}
+
+// Queue the kernels of one batch on `stream`.  A batch that is one launch is only queued;
+// otherwise the persistent schedule relaunches until the device counts every pixel done.
 int launch(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, uint64_t count, const uint64_t *si,
            const DevResults &out, hipStream_t stream)
 {
@@ -305,7 +355,7 @@ int launch(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, uint64_t count, co
         if (grid > 0x7fffffffull)
             return fail(ctx, FOORT_EINVAL, "batch too large");
         hipLaunchKernelGGL(ctx->kernels.oneshot, dim3(static_cast<unsigned>(grid)), dim3(block), 0, stream, ctx->rc,
-                           side, first, count, si, out, ctx->d_steps, ctx->slots, 0u);
+                           side, first, count, si, out, ctx->d_steps, ctx->slots, uint64_t(0));
         HIP_TRY(ctx, hipGetLastError());
         ctx->last_launches = 1;
         return FOORT_OK;
@@ -318,9 +368,17 @@ int launch(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, uint64_t count, co
     HIP_TRY(ctx, hipGetLastError());
     if (count == 0)
         return FOORT_OK;
+    if (single_launch(ctx)) {
+        // no step or time limit: the launch runs until every pixel is done
+        hipLaunchKernelGGL(ctx->kernels.persistent, dim3(S.L / 256), dim3(256), 0, stream, ctx->rc, side, first,
+                           count, si, out, ctx->d_steps, S, ctx->budget);
+        HIP_TRY(ctx, hipGetLastError());
+        ctx->last_launches = 1;
+        return FOORT_OK;
+    }
     // every launch advances each busy lane by at least min(budget, interval) steps (or to the end
     // of its geodesic); no geodesic is longer than step_cap, so the launches are bounded
-    const uint64_t per_launch = S.ticks ? S.interval : ctx->budget;
+    const uint64_t per_launch = S.ticks ? S.interval : std::min<uint64_t>(ctx->budget, UINT32_MAX);
     const uint64_t max_launches =
         (count / S.L + 2) * (static_cast<uint64_t>(ctx->rc.step_cap) / per_launch + 2) + 16;
     unsigned long long done = 0;
@@ -348,70 +406,126 @@ int check_batch(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, uint64_t coun
     return FOORT_OK;
 }
 
-// Host-pointer entry points: stage through the context buffer, one launch, copy back.
+// Host-pointer entry points: stage through the context buffer, launch, copy back.  With
+// use_graph, a batch that is one launch goes through pinned host staging and runs as one HIP
+// graph (index upload, counter reset, kernel, result copies), replayed while the batch shape
+// (sizes, result fields, schedule, buffers) stays the same and captured again otherwise.
 int trace_host(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, uint64_t count, const uint64_t *si,
                const foort_geodesic_results *out, foort_stats *stats)
 {
     if (!out)
         return fail(ctx, FOORT_EINVAL, "results pointer is NULL");
-    // layout of the staging buffer: [screen index | term | quadrant | passes | steps | closest | lambda | pos | vel]
+    // layout of the staging buffers: [screen index | term | quadrant | passes | steps | closest | lambda | pos | vel]
     auto align = [](size_t v) { return (v + 255) & ~size_t(255); };
+    struct Field {
+        void *host;
+        size_t off, bytes;
+    };
     size_t off = 0;
     const size_t o_si = off;
     off = align(off + (si ? count * 16 : 0));
-    const size_t o_term = off;
-    off = align(off + count * 4);
-    const size_t o_quad = off;
-    off = align(off + count * 4);
-    const size_t o_pass = off;
-    off = align(off + count * 4);
-    const size_t o_steps = off;
-    off = align(off + count * 4);
-    const size_t o_close = off;
-    off = align(off + count * 8);
-    const size_t o_lam = off;
-    off = align(off + count * 8);
-    const size_t o_pos = off;
-    off = align(off + count * 32);
-    const size_t o_vel = off;
-    off = align(off + count * 32);
+    Field f[8] = {{out->term, 0, count * 4},         {out->quadrant, 0, count * 4},
+                  {out->equat_passes, 0, count * 4}, {out->steps, 0, count * 4},
+                  {out->closest_radius, 0, count * 8}, {out->lambda, 0, count * 8},
+                  {out->final_pos, 0, count * 32},   {out->final_vel, 0, count * 32}};
+    for (Field &x : f) {
+        x.off = off;
+        off = align(off + x.bytes);
+    }
     int rc = ensure_buffer(ctx, off);
     if (rc)
         return rc;
     char *base = static_cast<char *>(ctx->d_buf);
-    const uint64_t *d_si = nullptr;
-    if (si) {
-        HIP_TRY(ctx, hipMemcpy(base + o_si, si, count * 16, hipMemcpyHostToDevice));
-        d_si = reinterpret_cast<const uint64_t *>(base + o_si);
-    }
+    auto dptr = [&](const Field &x) -> void * { return x.host ? base + x.off : nullptr; };
     DevResults d{};
-    d.term = out->term ? reinterpret_cast<int32_t *>(base + o_term) : nullptr;
-    d.quadrant = out->quadrant ? reinterpret_cast<int32_t *>(base + o_quad) : nullptr;
-    d.passes = out->equat_passes ? reinterpret_cast<int32_t *>(base + o_pass) : nullptr;
-    d.steps = out->steps ? reinterpret_cast<uint32_t *>(base + o_steps) : nullptr;
-    d.closest = out->closest_radius ? reinterpret_cast<double *>(base + o_close) : nullptr;
-    d.lambda = out->lambda ? reinterpret_cast<double *>(base + o_lam) : nullptr;
-    d.pos = out->final_pos ? reinterpret_cast<double *>(base + o_pos) : nullptr;
-    d.vel = out->final_vel ? reinterpret_cast<double *>(base + o_vel) : nullptr;
-    HIP_TRY(ctx, hipEventRecord(ctx->ev0, nullptr));
-    rc = launch(ctx, side, first, count, d_si, d, nullptr);
-    if (rc)
-        return rc;
-    HIP_TRY(ctx, hipEventRecord(ctx->ev1, nullptr));
-    HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
-    auto back = [&](void *dst, const void *src, size_t bytes) -> hipError_t {
-        if (!dst || !src)
-            return hipSuccess;
-        return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
-    };
-    HIP_TRY(ctx, back(out->term, d.term, count * 4));
-    HIP_TRY(ctx, back(out->quadrant, d.quadrant, count * 4));
-    HIP_TRY(ctx, back(out->equat_passes, d.passes, count * 4));
-    HIP_TRY(ctx, back(out->steps, d.steps, count * 4));
-    HIP_TRY(ctx, back(out->closest_radius, d.closest, count * 8));
-    HIP_TRY(ctx, back(out->lambda, d.lambda, count * 8));
-    HIP_TRY(ctx, back(out->final_pos, d.pos, count * 32));
-    HIP_TRY(ctx, back(out->final_vel, d.vel, count * 32));
+    d.term = static_cast<int32_t *>(dptr(f[0]));
+    d.quadrant = static_cast<int32_t *>(dptr(f[1]));
+    d.passes = static_cast<int32_t *>(dptr(f[2]));
+    d.steps = static_cast<uint32_t *>(dptr(f[3]));
+    d.closest = static_cast<double *>(dptr(f[4]));
+    d.lambda = static_cast<double *>(dptr(f[5]));
+    d.pos = static_cast<double *>(dptr(f[6]));
+    d.vel = static_cast<double *>(dptr(f[7]));
+    const uint64_t *d_si = si ? reinterpret_cast<const uint64_t *>(base + o_si) : nullptr;
+
+    const bool graph = ctx->desc.use_graph && single_launch(ctx) && count > 0;
+    if (!graph) {
+        if (si)
+            HIP_TRY(ctx, hipMemcpy(base + o_si, si, count * 16, hipMemcpyHostToDevice));
+        HIP_TRY(ctx, hipEventRecord(ctx->ev0, nullptr));
+        rc = launch(ctx, side, first, count, d_si, d, nullptr);
+        if (rc)
+            return rc;
+        HIP_TRY(ctx, hipEventRecord(ctx->ev1, nullptr));
+        HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+        for (const Field &x : f)
+            if (x.host)
+                HIP_TRY(ctx, hipMemcpy(x.host, base + x.off, x.bytes, hipMemcpyDeviceToHost));
+    } else {
+        rc = ensure_pinned(ctx, off);
+        if (rc)
+            return rc;
+        if (ctx->desc.schedule == FOORT_SCHEDULE_PERSISTENT) {
+            rc = setup_persistent(ctx); // allocates: not inside the capture
+            if (rc)
+                return rc;
+        }
+        if (!ctx->stream)
+            HIP_TRY(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        char *hb = static_cast<char *>(ctx->h_buf);
+        unsigned mask = 0;
+        for (int k = 0; k < 8; ++k)
+            mask |= f[k].host ? 1u << k : 0u;
+        const std::string key = std::to_string(side) + "/" + std::to_string(first) + "/" + std::to_string(count) +
+                                "/" + (si ? "s" : "g") + "/" + std::to_string(mask) + "/" +
+                                std::to_string(ctx->desc.schedule) + "/" + std::to_string(ctx->desc.block_size) + "/" +
+                                std::to_string(reinterpret_cast<uintptr_t>(ctx->d_buf)) + "/" +
+                                std::to_string(reinterpret_cast<uintptr_t>(ctx->h_buf)) + "/" +
+                                std::to_string(ctx->slots.refill) + "/" + std::to_string(ctx->slots.interval);
+        if (key != ctx->graph_key) {
+            drop_graph(ctx);
+            HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            auto capture = [&]() -> int {
+                if (si)
+                    HIP_TRY(ctx, hipMemcpyAsync(base + o_si, hb + o_si, count * 16, hipMemcpyHostToDevice, ctx->stream));
+                int r = launch(ctx, side, first, count, d_si, d, ctx->stream);
+                if (r)
+                    return r;
+                for (const Field &x : f)
+                    if (x.host)
+                        HIP_TRY(ctx, hipMemcpyAsync(hb + x.off, base + x.off, x.bytes, hipMemcpyDeviceToHost,
+                                                    ctx->stream));
+                return FOORT_OK;
+            };
+            rc = capture();
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+            if (rc) {
+                if (g)
+                    (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (e != hipSuccess)
+                return hip_fail(ctx, e, "hipStreamEndCapture");
+            e = hipGraphInstantiate(&ctx->graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                ctx->graph = nullptr;
+                return hip_fail(ctx, e, "hipGraphInstantiate");
+            }
+            ctx->graph_key = key;
+        }
+        if (si)
+            std::memcpy(hb + o_si, si, count * 16);
+        HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+        HIP_TRY(ctx, hipGraphLaunch(ctx->graph, ctx->stream));
+        HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+        HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+        for (const Field &x : f)
+            if (x.host)
+                std::memcpy(x.host, hb + x.off, x.bytes);
+        ctx->last_launches = 1;
+    }
     if (stats) {
         unsigned long long steps = 0;
         HIP_TRY(ctx, hipMemcpy(&steps, ctx->d_steps, sizeof(steps), hipMemcpyDeviceToHost));
@@ -553,7 +667,7 @@ int foort_gpu_open(const foort_run_desc *desc, foort_gpu_ctx **out)
     ctx->kernels = d.christoffel == FOORT_CHRISTOFFEL_FD
                        ? foort::pick_policy<foort::RhsFD>(d.metric, d.integrator, every)
                        : foort::pick_analytic(d.metric, d.integrator, static_cast<int>(d.precision), every);
-    ctx->budget = d.steps_per_launch ? d.steps_per_launch : 0xffffffffu;
+    ctx->budget = d.steps_per_launch ? d.steps_per_launch : UINT64_MAX;
 
     hipError_t e = hipGetDevice(&ctx->device);
     if (e == hipSuccess)
@@ -581,6 +695,11 @@ void foort_gpu_close(foort_gpu_ctx *ctx)
 {
     if (!ctx)
         return;
+    drop_graph(ctx);
+    if (ctx->stream)
+        (void)hipStreamDestroy(ctx->stream);
+    if (ctx->h_buf)
+        (void)hipHostFree(ctx->h_buf);
     if (ctx->d_buf)
         (void)hipFree(ctx->d_buf);
     if (ctx->d_counters)
@@ -657,6 +776,89 @@ int foort_gpu_trace_device(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, ui
         stats->launches = ctx->last_launches;
         stats->device_ms = ms;
     }
+    return FOORT_OK;
+}
+
+int foort_gpu_autotune(foort_gpu_ctx *ctx, uint64_t side, uint64_t count, const uint64_t *screen_index,
+                       uint32_t flags, foort_run_desc *chosen, foort_stats *best)
+{
+    if (!ctx)
+        return fail(nullptr, FOORT_EINVAL, "NULL context");
+    if (count == 0)
+        return fail(ctx, FOORT_EINVAL, "the autotuner needs a non-empty sample");
+    int rc = check_batch(ctx, side, 0, count, screen_index == nullptr);
+    if (rc)
+        return rc;
+    if (screen_index)
+        for (uint64_t i = 0; i < 2 * count; ++i)
+            if (screen_index[i] >= side)
+                return fail(ctx, FOORT_EINVAL, "screen index outside the row_column_size grid");
+    // candidates: block sizes of the one-shot schedule; steps between refills and the refill
+    // threshold of the persistent one (each batch one launch, the fastest for throughput)
+    struct Cand {
+        int32_t schedule, block;
+        uint32_t interval, refill;
+    };
+    std::vector<Cand> cands;
+    const bool both = (flags & FOORT_TUNE_SCHEDULE) != 0;
+    if (both || ctx->desc.schedule == FOORT_SCHEDULE_ONESHOT)
+        for (int32_t b : {64, 128, 256})
+            cands.push_back({FOORT_SCHEDULE_ONESHOT, b, 0, 0});
+    if (both || ctx->desc.schedule == FOORT_SCHEDULE_PERSISTENT)
+        for (uint32_t iv : {8u, 32u, 128u})
+            for (uint32_t rf : {1u, 16u, 64u})
+                cands.push_back({FOORT_SCHEDULE_PERSISTENT, 0, iv, rf});
+    const foort_run_desc saved = ctx->desc;
+    const foort_geodesic_results none{};
+    auto run = [&](const Cand &c, foort_stats &st) -> int {
+        ctx->desc.schedule = c.schedule;
+        ctx->desc.block_size = c.block;
+        ctx->desc.refill_interval = c.interval;
+        ctx->desc.refill_lanes = c.refill;
+        ctx->desc.steps_per_launch = 0;
+        ctx->desc.launch_us = 0;
+        ctx->desc.use_graph = 0;
+        apply_lane_options(ctx);
+        return trace_host(ctx, side, 0, count, screen_index, &none, &st);
+    };
+    foort_stats st{}, top{};
+    size_t pick = 0;
+    double top_ms = 0;
+    rc = run(cands[0], st); // warm-up (code loading, first allocations)
+    for (size_t k = 0; rc == FOORT_OK && k < cands.size(); ++k) {
+        double ms = 1e300;
+        // short samples are timed twice, keeping the faster run
+        for (int rep = 0; rep < 2 && rc == FOORT_OK; ++rep) {
+            rc = run(cands[k], st);
+            if (rc == FOORT_OK && st.device_ms < ms)
+                ms = st.device_ms;
+            if (ms > 50.0)
+                break;
+        }
+        if (rc == FOORT_OK && (k == 0 || ms < top_ms)) {
+            top_ms = ms;
+            top = st;
+            top.device_ms = ms;
+            pick = k;
+        }
+    }
+    ctx->desc = saved;
+    if (rc != FOORT_OK) {
+        apply_lane_options(ctx);
+        return rc;
+    }
+    ctx->desc.schedule = cands[pick].schedule;
+    ctx->desc.block_size = cands[pick].block;
+    if (cands[pick].schedule == FOORT_SCHEDULE_PERSISTENT) {
+        ctx->desc.refill_interval = cands[pick].interval;
+        ctx->desc.refill_lanes = cands[pick].refill;
+    }
+    apply_lane_options(ctx);
+    drop_graph(ctx);
+    if (chosen)
+        *chosen = ctx->desc;
+    if (best)
+        *best = top;
     return FOORT_OK;
 }
 

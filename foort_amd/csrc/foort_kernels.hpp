@@ -106,7 +106,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FOORT_
 trace_oneshot(const RunConst R, uint64_t side, uint64_t first, uint64_t count,
                                                      const uint64_t *__restrict__ si, DevResults out,
                                                      unsigned long long *__restrict__ total_steps, LaneSlots,
-                                                     uint32_t)
+                                                     uint64_t)
 {
     const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     unsigned long long mine = 0;
@@ -186,7 +186,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FOORT_
 trace_persistent(const RunConst R, uint64_t side, uint64_t first,
                                                         uint64_t count, const uint64_t *__restrict__ si,
                                                         DevResults out, unsigned long long *__restrict__ total_steps,
-                                                        LaneSlots S, uint32_t budget)
+                                                        LaneSlots S, uint64_t budget)
 {
     const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x; // grid covers exactly S.L lanes
     const uint32_t lane_id = threadIdx.x & 63;
@@ -247,7 +247,7 @@ trace_persistent(const RunConst R, uint64_t side, uint64_t first,
 }
 
 using KernelFn = void (*)(const RunConst, uint64_t, uint64_t, uint64_t, const uint64_t *, DevResults,
-                          unsigned long long *, LaneSlots, uint32_t);
+                          unsigned long long *, LaneSlots, uint64_t);
 
 struct KernelPair {
     KernelFn oneshot;

@@ -665,6 +665,23 @@ void get_gpu_options(const Setting &root, RunConfig &rc)
         d.refill_lanes = static_cast<uint32_t>(n);
     g->lookup("MixedSwitchRadius", d.mixed_switch_radius);
     g->lookup("MixedPoleMargin", d.mixed_pole_margin);
+    n = d.launch_us;
+    if (g->lookup_counter("LaunchMicroseconds", n))
+        d.launch_us = static_cast<uint32_t>(n);
+    bool b = d.use_graph != 0;
+    if (g->lookup("UseGraph", b))
+        d.use_graph = b;
+    if (g->lookup("Autotune", v)) {
+        v = lower(v);
+        if (v == "none" || v == "off")
+            rc.autotune = 0;
+        else if (v == "options")
+            rc.autotune = 1;
+        else if (v == "schedule")
+            rc.autotune = 2;
+        else
+            screen_output("GPU.Autotune must be \"None\", \"Options\" or \"Schedule\"; not tuning.", Level::Warning);
+    }
 }
 
 std::string timestamp_now()

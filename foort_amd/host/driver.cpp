@@ -12,8 +12,12 @@
 #include "host.hpp"
 
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
+#include <filesystem>
+#include <fstream>
 #include <iostream>
+#include <sstream>
 #include <stdexcept>
 
 namespace foort_host {
@@ -44,14 +48,6 @@ struct BatchResults {
         default: return closest[i];
         }
     }
-    std::string text(uint32_t diag, uint64_t i) const
-    {
-        switch (diag) {
-        case FOORT_DIAGFLAG_FOURCOLORSCREEN: return std::to_string(quadrant[i]);
-        case FOORT_DIAGFLAG_EQUATORIALPASSES: return std::to_string(passes[i]);
-        default: return to_str(closest[i]);
-        }
-    }
 };
 
 int copy_out(const std::string &s, char *buf, size_t cap)
@@ -67,6 +63,63 @@ int copy_out(const std::string &s, char *buf, size_t cap)
 struct ConfigError {
     int code;
 };
+
+// Autotuner results are kept in a small text file (FOORT_TUNE_CACHE, else
+// ~/.cache/foort_tune.txt), one line per configuration: the key (what the kernels
+// depend on, and the tuning mode) and the chosen schedule, block size, refill
+// interval and refill threshold.  A later run of the same configuration reuses them.
+std::string tune_cache_path()
+{
+    if (const char *p = std::getenv("FOORT_TUNE_CACHE"))
+        return p;
+    if (const char *h = std::getenv("HOME"))
+        return std::string(h) + "/.cache/foort_tune.txt";
+    return "";
+}
+
+std::string tune_key(const foort_run_desc &d, int mode)
+{
+    std::ostringstream k;
+    k << "metric=" << d.metric << " rlog=" << d.rlog_scale << " integ=" << d.integrator << " chris=" << d.christoffel
+      << " prec=" << d.precision << " terms=" << d.term_flags << " diags=" << d.diag_flags << " mode=" << mode
+      << " sched=" << (mode == 2 ? -1 : d.schedule);
+    return k.str();
+}
+
+bool tune_lookup(const std::string &key, foort_run_desc &d)
+{
+    std::ifstream f(tune_cache_path());
+    std::string line;
+    while (std::getline(f, line)) {
+        const size_t bar = line.find(" | ");
+        if (bar == std::string::npos || line.compare(0, bar, key) != 0 || bar != key.size())
+            continue;
+        std::istringstream v(line.substr(bar + 3));
+        int sched, block;
+        unsigned interval, refill;
+        if (v >> sched >> block >> interval >> refill) {
+            d.schedule = sched;
+            d.block_size = block;
+            d.refill_interval = interval;
+            d.refill_lanes = refill;
+            return true;
+        }
+    }
+    return false;
+}
+
+void tune_store(const std::string &key, const foort_run_desc &d)
+{
+    const std::string path = tune_cache_path();
+    if (path.empty())
+        return;
+    std::error_code ec;
+    std::filesystem::create_directories(std::filesystem::path(path).parent_path(), ec);
+    std::ofstream f(path, std::ios::app);
+    if (f)
+        f << key << " | " << d.schedule << " " << d.block_size << " " << d.refill_interval << " " << d.refill_lanes
+          << "\n";
+}
 
 // the program body; throws ConfigError for the reference's early exits
 int run(const char *cfg_path, const foort_program_options &o)
@@ -116,6 +169,14 @@ int run(const char *cfg_path, const foort_program_options &o)
     OutputHandler out(rc.to_files ? rc.prefix : "", rc.timestamp, rc.extension, names,
                       rc.to_files ? rc.to_cache : ~0UL - 1, rc.to_files ? rc.per_file : ~0UL,
                       rc.to_files && rc.first_line_on ? first_line : "");
+    {
+        // the diagnostics' texts: std::to_string of the quadrant and of the passes (ints) and of the
+        // closest radius (a double), FOORT/Diagnostics.cpp:204-208, :383-387, :449-453
+        std::vector<char> kinds;
+        for (uint32_t f : order)
+            kinds.push_back(f == FOORT_DIAGFLAG_CLOSESTRADIUS ? OutputHandler::kFixed : OutputHandler::kInt);
+        out.setValueKinds(kinds);
+    }
     screen_output("Done loading options from configuration file.", Level::Proc);
 
     // the objects, as FOORT/Main.cpp:262-299 lists them
@@ -150,6 +211,11 @@ int run(const char *cfg_path, const foort_program_options &o)
 
     foort_gpu_ctx *gpu = nullptr;
     if (!o.tracer) {
+        if (rc.autotune && tune_lookup(tune_key(rc.desc, rc.autotune), rc.desc)) {
+            screen_output("Autotuner: options of an earlier run of this configuration (" + tune_cache_path() + ").",
+                          Level::Subproc);
+            rc.autotune = 0;
+        }
         int rcode = foort_gpu_open(&rc.desc, &gpu);
         if (rcode != FOORT_OK) {
             set_error(std::string("GPU path: ") + foort_gpu_last_error(nullptr));
@@ -163,18 +229,32 @@ int run(const char *cfg_path, const foort_program_options &o)
     } closer{gpu};
 
     const bool grid = rc.mesh.type == FOORT_MESH_SIMPLESQUARE;
+    bool tuned = false;
+    uint64_t iteration = 0;
     foort_stats total{};
+    // wall time of the phases of the loop: screen indices, integration, results into the mesh and
+    // the output cache, mesh refinement (EndCurrentLoop), writing the files
+    double t_index = 0, t_trace = 0, t_results = 0, t_refine = 0, t_write = 0;
+    auto lap = [](std::chrono::steady_clock::time_point &t) {
+        auto now = std::chrono::steady_clock::now();
+        double d = std::chrono::duration<double>(now - t).count();
+        t = now;
+        return d;
+    };
     auto t_total = std::chrono::steady_clock::now();
     while (!mesh->IsFinished()) {
         screen_output("Starting new integration loop.", Level::Proc);
         auto t_iter = std::chrono::steady_clock::now();
+        auto t = t_iter;
         const uint64_t n = mesh->getCurNrGeodesics();
         screen_output("Integrating " + std::to_string(static_cast<long long>(n)) + " geodesics on the GPU...",
                       Level::Proc);
         out.PrepareForOutput(n);
+        t_write += lap(t);
         std::vector<uint64_t> idx(2 * n);
         mesh->screenIndices(idx.data());
         BatchResults res(n);
+        t_index += lap(t);
         foort_geodesic_results view = res.view();
         const uint64_t side = mesh->rowColumnSize();
         int rcode = FOORT_OK;
@@ -184,6 +264,38 @@ int run(const char *cfg_path, const foort_program_options &o)
                 if (rcode)
                     set_error("batch tracer failed with code " + std::to_string(rcode));
             } else {
+                // tune on (a sample of at most 65536 pixels of) a representative batch: the
+                // only iteration of a single-pass mesh, or the first refinement iteration of an
+                // adaptive mesh with at least 65536 pixels (its pixels follow the image's edges,
+                // unlike the uniform initial grid)
+                const bool adaptive = rc.mesh.type == FOORT_MESH_SQUARESUBDIVISION ||
+                                      rc.mesh.type == FOORT_MESH_SQUARESUBDIVISIONV2;
+                if (rc.autotune && !tuned && (!adaptive || (iteration > 0 && n >= 65536))) {
+                    tuned = true;
+                    const uint64_t stride = std::max<uint64_t>(1, n / 65536);
+                    std::vector<uint64_t> sample;
+                    for (uint64_t i = 0; i < n; i += stride)
+                        sample.insert(sample.end(), {idx[2 * i], idx[2 * i + 1]});
+                    foort_run_desc chosen;
+                    foort_stats tst{};
+                    const std::string key = tune_key(rc.desc, rc.autotune);
+                    if (foort_gpu_autotune(gpu, side, sample.size() / 2, sample.data(),
+                                           rc.autotune == 2 ? FOORT_TUNE_SCHEDULE : 0u, &chosen, &tst) == FOORT_OK) {
+                        tune_store(key, chosen);
+                        screen_output("Autotuner: " + std::string(chosen.schedule == FOORT_SCHEDULE_ONESHOT
+                                                                      ? "one-shot schedule, block size " +
+                                                                            std::to_string(chosen.block_size)
+                                                                      : "persistent schedule, refill every " +
+                                                                            std::to_string(chosen.refill_interval) +
+                                                                            " steps once " +
+                                                                            std::to_string(chosen.refill_lanes) +
+                                                                            " lanes are idle") +
+                                          " (" + std::to_string(tst.device_ms) + " ms on " +
+                                          std::to_string(sample.size() / 2) + " pixels).",
+                                      Level::Subproc);
+                    } else
+                        screen_output(std::string("Autotuner failed: ") + foort_gpu_last_error(gpu), Level::Warning);
+                }
                 foort_stats st{};
                 rcode = grid ? foort_gpu_trace_grid(gpu, side, 0, n, &view, &st)
                              : foort_gpu_trace_pixels(gpu, side, n, idx.data(), &view, &st);
@@ -199,26 +311,42 @@ int run(const char *cfg_path, const foort_program_options &o)
             screen_output(g_error, Level::Warning);
             return rcode;
         }
+        t_trace += lap(t);
         const uint32_t vd = order.front();
-        std::vector<std::string> texts(order.size());
-        for (uint64_t i = 0; i < n; ++i) {
-            mesh->GeodesicFinished(i, res.value(vd, i));
+        // GeodesicFinished and NewGeodesicOutput write the slot of their own index only
+        // (as in the reference's parallel loop), so the geodesics are handed over in parallel
+#pragma omp parallel for schedule(static)
+        for (long long i = 0; i < static_cast<long long>(n); ++i) {
+            const uint64_t u = static_cast<uint64_t>(i);
+            mesh->GeodesicFinished(u, res.value(vd, u));
+            double vals[4];
             for (size_t k = 0; k < order.size(); ++k)
-                texts[k] = res.text(order[k], i);
-            out.NewGeodesicOutput(i, idx[2 * i], idx[2 * i + 1], texts);
+                vals[k] = res.value(order[k], u);
+            out.NewGeodesicValues(u, idx[2 * i], idx[2 * i + 1], vals);
         }
+        t_results += lap(t);
         const double it = seconds_since(t_iter), tt = seconds_since(t_total);
         screen_output("Integration loop done. Time taken for integration loop: " + std::to_string(it) + "s (" +
                           std::to_string(it / 60) + "m); total time elapsed: " + std::to_string(tt) + "s (" +
                           std::to_string(tt / 60) + "m).",
                       Level::Proc);
+        t = std::chrono::steady_clock::now();
         mesh->EndCurrentLoop();
+        t_refine += lap(t);
+        ++iteration;
     }
     const double tt = seconds_since(t_total);
     screen_output("All integration finished! Total time elapsed: " + std::to_string(tt) + "s (" +
                       std::to_string(tt / 60) + "m).",
                   Level::Proc);
+    auto t = std::chrono::steady_clock::now();
     out.OutputFinished();
+    t_write += lap(t);
+    screen_output("Time by phase: screen indices " + std::to_string(t_index) + "s, integration " +
+                      std::to_string(t_trace) + "s (kernels " + std::to_string(total.device_ms / 1e3) +
+                      "s), results to mesh and output " + std::to_string(t_results) + "s, mesh refinement " +
+                      std::to_string(t_refine) + "s, writing files " + std::to_string(t_write) + "s.",
+                  Level::Subproc);
     screen_output("FOORT finished. Goodbye!", Level::Proc);
     std::cout.flush();
     if (o.stats)

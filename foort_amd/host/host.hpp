@@ -50,6 +50,9 @@ struct RunConfig {
     unsigned long per_file = ~0UL;
     bool first_line_on = true;
     int screen_output_level = 4;
+    // GPU group: tune the execution options on the first mesh iteration (0 none, 1 the
+    // schedule's own options, 2 also the schedule)
+    int autotune = 0;
 };
 
 // Config::InitializeScreenOutput ... Config::GetOutputHandler in the order of
@@ -104,15 +107,24 @@ public:
     void PrepareForOutput(uint64_t count);
     // screen index text (row, column) and one text per diagnostic
     void NewGeodesicOutput(uint64_t index, uint64_t row, uint64_t col, std::vector<std::string> texts);
+    // the same with the diagnostics' values, written as texts when the files are written:
+    // kind[d] = kInt (std::to_string of an int) or kFixed (std::to_string of a double, "%f")
+    enum Kind : char { kInt = 1, kFixed = 2 };
+    void setValueKinds(std::vector<char> kinds) { m_kinds = std::move(kinds); }
+    void NewGeodesicValues(uint64_t index, uint64_t row, uint64_t col, const double *values);
     void OutputFinished();
     std::string getFullDescriptionStr() const;
     bool writesToConsole() const { return m_console; }
 
 private:
-    struct Entry {
-        uint64_t row = 0, col = 0;
-        std::vector<std::string> texts;
+    static constexpr int kMaxValues = 4;
+    static constexpr uint64_t kNoText = ~uint64_t(0);
+    struct Entry {              // plain data, so the cache grows by copying bytes
+        uint64_t row, col;
+        double values[kMaxValues];
+        uint64_t text;          // index into m_texts, or kNoText: the entry holds values
     };
+    void AppendText(std::string &b, const Entry &e, int d) const;
     void WriteCached();
     void OpenForFirstTime(const std::string &name);
     std::string FileName(int diag, unsigned short filenr) const;
@@ -128,6 +140,8 @@ private:
     unsigned long m_in_file = 0;
     unsigned short m_full_files = 0;
     std::vector<Entry> m_cache;
+    std::vector<std::vector<std::string>> m_texts; // texts given through NewGeodesicOutput
+    std::vector<char> m_kinds;
 };
 
 std::string set_error(const std::string &msg); // records the thread's last host error

@@ -10,13 +10,18 @@
 // file's own: pixels live in one vector and refer to each other by index;
 // V1's linear searches of the pixel lists (FOORT/Mesh.cpp:290-326, :393-406)
 // become hash lookups on (row, column), which find the same pixel because a
-// screen index occurs at most once in each list.
+// screen index occurs at most once in each list; its passes over all pixels to
+// update neighbours and weights (FOORT/Mesh.cpp:292-383) visit only the pixels
+// whose neighbours or weight were reset, the only ones those passes change (each
+// pixel's update reads other pixels' indices and values, never their weights).
 #include "host.hpp"
 
 #include <algorithm>
 #include <array>
 #include <cmath>
 #include <stdexcept>
+
+#include <omp.h>
 
 namespace foort_host {
 
@@ -43,6 +48,80 @@ uint64_t exp2i(int e)
 uint64_t isqrt_trunc(uint64_t n) { return static_cast<uint64_t>(std::sqrt(static_cast<double>(n))); }
 
 inline uint64_t key(uint64_t row, uint64_t col) { return (row << 32) ^ col; }
+
+// Open-addressing hash map from a screen index key to a pixel number (linear
+// probing, power-of-two capacity, at most half full).
+class IndexMap {
+public:
+    IndexMap() { rehash(1024); }
+    // empties the map, keeping its capacity
+    void clear()
+    {
+        if (m_size) {
+            std::fill(m_keys.begin(), m_keys.end(), kEmpty);
+            m_size = 0;
+        }
+    }
+    // room for n keys without rehashing
+    void reserve(uint64_t n)
+    {
+        uint64_t cap = 1024;
+        while (cap < 2 * n)
+            cap *= 2;
+        if (cap > m_keys.size())
+            rehash(cap);
+    }
+    uint64_t find(uint64_t k) const
+    {
+        for (uint64_t h = hash(k);; h = (h + 1) & m_mask) {
+            if (m_keys[h] == kEmpty)
+                return UINT64_MAX;
+            if (m_keys[h] == k)
+                return m_vals[h];
+        }
+    }
+    // inserts k -> v unless k is present
+    void insert(uint64_t k, uint64_t v)
+    {
+        if (2 * (m_size + 1) > m_keys.size())
+            rehash(2 * m_keys.size());
+        for (uint64_t h = hash(k);; h = (h + 1) & m_mask) {
+            if (m_keys[h] == kEmpty) {
+                m_keys[h] = k;
+                m_vals[h] = v;
+                ++m_size;
+                return;
+            }
+            if (m_keys[h] == k)
+                return;
+        }
+    }
+
+private:
+    static constexpr uint64_t kEmpty = UINT64_MAX;
+    std::vector<uint64_t> m_keys, m_vals;
+    uint64_t m_mask = 0, m_size = 0;
+    uint64_t hash(uint64_t k) const
+    {
+        k ^= k >> 33;
+        k *= 0xff51afd7ed558ccdULL;
+        k ^= k >> 33;
+        return k & m_mask;
+    }
+    void rehash(uint64_t cap)
+    {
+        std::vector<uint64_t> ok, ov;
+        ok.swap(m_keys);
+        ov.swap(m_vals);
+        m_keys.assign(cap, kEmpty);
+        m_vals.assign(cap, 0);
+        m_mask = cap - 1;
+        m_size = 0;
+        for (size_t i = 0; i < ok.size(); ++i)
+            if (ok[i] != kEmpty)
+                insert(ok[i], ov[i]);
+    }
+};
 
 // ---------------------------------------------------------------------------
 // SimpleSquareMesh, FOORT/Mesh.cpp:25-73 and FOORT/Mesh.h:68-103
@@ -144,6 +223,13 @@ public:
         const uint64_t n = isqrt_trunc(S.initial);
         const uint64_t step = exp2i(S.max_sub - 1);
         m_queue.reserve(S.initial);
+        m_queue_pos.reserve(S.initial);
+        if (!S.infinite) { // the pixel list never exceeds MaxPixels (nor InitialPixels)
+            const uint64_t most = std::min<uint64_t>(std::max(S.max_pixels, S.initial), uint64_t(1) << 26);
+            m_all.reserve(most);
+            m_is_dirty.reserve(most);
+            m_all_pos.reserve(most);
+        }
         for (uint64_t i = 0; i < S.initial; ++i) {
             uint64_t row = i / n, col = i - (i / n) * n;
             int sub = (row == n - 1 || col == n - 1) ? 0 : 1;
@@ -175,8 +261,10 @@ public:
             screen_output("Not all pixels have been integrated!", Level::Warning);
         // the integrated queue joins the list of all pixels (FOORT/Mesh.cpp:546-560)
         for (const Pixel &p : m_queue) {
-            m_all_pos.emplace(key(p.row, p.col), m_all.size());
+            m_all_pos.insert(key(p.row, p.col), m_all.size());
             m_all.push_back(p);
+            m_is_dirty.push_back(0);
+            mark_dirty(m_all.size() - 1);
         }
         m_queue.clear();
         m_queue_pos.clear();
@@ -188,13 +276,32 @@ public:
         if (S.infinite || m_left > 0) {
             update_neighbors();
             update_weights();
-            // candidates (FOORT/Mesh.cpp:579-598)
+            for (uint64_t i : m_dirty)
+                m_is_dirty[i] = 0;
+            m_dirty.clear();
+            // candidates in pixel order (FOORT/Mesh.cpp:579-598), filtered in parallel chunks
             std::vector<uint64_t> cand;
-            cand.reserve(m_all.size());
-            for (uint64_t i = 0; i < m_all.size(); ++i) {
-                const Pixel &p = m_all[i];
-                if (p.sub > 0 && p.sub < S.max_sub && (p.weight > 0 || (S.to_final && p.sub > 1)))
-                    cand.push_back(i);
+            {
+                const uint64_t N = m_all.size();
+                const int T = omp_get_max_threads();
+                std::vector<std::vector<uint64_t>> part(static_cast<size_t>(T));
+#pragma omp parallel num_threads(T)
+                {
+                    const int t = omp_get_thread_num();
+                    const uint64_t lo = N * static_cast<uint64_t>(t) / static_cast<uint64_t>(T);
+                    const uint64_t hi = N * static_cast<uint64_t>(t + 1) / static_cast<uint64_t>(T);
+                    for (uint64_t i = lo; i < hi; ++i) {
+                        const Pixel &p = m_all[i];
+                        if (p.sub > 0 && p.sub < S.max_sub && (p.weight > 0 || (S.to_final && p.sub > 1)))
+                            part[static_cast<size_t>(t)].push_back(i);
+                    }
+                }
+                size_t total = 0;
+                for (const auto &v : part)
+                    total += v.size();
+                cand.reserve(total);
+                for (const auto &v : part)
+                    cand.insert(cand.end(), v.begin(), v.end());
             }
             // most important first; equal weights prefer the less subdivided pixel (FOORT/Mesh.cpp:603-614)
             std::sort(cand.begin(), cand.end(), [this](uint64_t a, uint64_t b) {
@@ -207,11 +314,8 @@ public:
             m_queue.reserve(5 * cand.size());
             for (uint64_t i : cand)
                 subdivide(i);
-            if (!S.infinite && m_queue.size() > m_left) {
-                for (uint64_t j = m_left; j < m_queue.size(); ++j)
-                    m_queue_pos.erase(key(m_queue[j].row, m_queue[j].col));
-                m_queue.resize(m_left);
-            }
+            if (!S.infinite && m_queue.size() > m_left)
+                m_queue.resize(m_left); // no iteration follows (m_left becomes 0)
             if (!S.infinite) {
                 m_left -= m_queue.size();
                 screen_output("Mesh says: Still max. " + std::to_string(static_cast<unsigned long>(m_left)) +
@@ -237,34 +341,51 @@ private:
     uint64_t m_left;
     std::vector<Pixel> m_queue, m_all;
     std::vector<char> m_done;
-    std::unordered_map<uint64_t, uint64_t> m_queue_pos, m_all_pos;
+    IndexMap m_queue_pos, m_all_pos;
+    std::vector<uint64_t> m_dirty; // pixels whose neighbours and weight were reset (each once)
+    std::vector<char> m_is_dirty;
+
+    void mark_dirty(uint64_t i)
+    {
+        if (!m_is_dirty[i]) {
+            m_is_dirty[i] = 1;
+            m_dirty.push_back(i);
+        }
+    }
 
     void enqueue(uint64_t row, uint64_t col, int sub)
     {
-        m_queue_pos.emplace(key(row, col), m_queue.size());
+        m_queue_pos.insert(key(row, col), m_queue.size());
         m_queue.push_back(Pixel{row, col, sub});
     }
 
-    uint64_t find_all(uint64_t row, uint64_t col) const
-    {
-        auto it = m_all_pos.find(key(row, col));
-        return it == m_all_pos.end() ? UINT64_MAX : it->second;
-    }
+    uint64_t find_all(uint64_t row, uint64_t col) const { return m_all_pos.find(key(row, col)); }
 
     // UpdateAllNeighbors, FOORT/Mesh.cpp:292-331
     void update_neighbors()
     {
         screen_output("Updating all pixel neighbor information...", Level::AllDetail);
-        for (Pixel &p : m_all) {
+        // each pixel's update reads only the position map, so the pixels update in parallel
+        long long missing = -1;
+#pragma omp parallel for schedule(static)
+        for (long long k = 0; k < static_cast<long long>(m_dirty.size()); ++k) {
+            Pixel &p = m_all[m_dirty[static_cast<size_t>(k)]];
             if (p.sub > 0 && p.lower == 0 && p.right == 0) {
                 const uint64_t d = exp2i(S.max_sub - p.sub);
                 const uint64_t r = find_all(p.row, p.col + d), l = find_all(p.row + d, p.col);
-                if (r == UINT64_MAX || l == UINT64_MAX)
-                    throw std::runtime_error("SquareSubdivisionMesh: pixel (" + std::to_string(p.row) + ", " +
-                                             std::to_string(p.col) + ") lacks a neighbour");
-                p.right = r;
-                p.lower = l;
+                if (r == UINT64_MAX || l == UINT64_MAX) {
+#pragma omp critical(foort_mesh_missing)
+                    missing = k;
+                } else {
+                    p.right = r;
+                    p.lower = l;
+                }
             }
+        }
+        if (missing >= 0) {
+            const Pixel &p = m_all[m_dirty[static_cast<size_t>(missing)]];
+            throw std::runtime_error("SquareSubdivisionMesh: pixel (" + std::to_string(p.row) + ", " +
+                                     std::to_string(p.col) + ") lacks a neighbour");
         }
         screen_output("Done updating pixel neighbor information.", Level::AllDetail);
     }
@@ -273,7 +394,10 @@ private:
     void update_weights()
     {
         screen_output("Updating all pixel weights...", Level::AllDetail);
-        for (Pixel &p : m_all) {
+        // each pixel's weight reads other pixels' neighbours and values, never their weights
+#pragma omp parallel for schedule(static)
+        for (long long k = 0; k < static_cast<long long>(m_dirty.size()); ++k) {
+            Pixel &p = m_all[m_dirty[static_cast<size_t>(k)]];
             if (p.weight < 0 && p.sub > 0 && p.sub < S.max_sub) {
                 std::array<double, 3> dist{};
                 dist[0] = distance(p.value, m_all[p.right].value);
@@ -296,23 +420,24 @@ private:
     {
         const uint64_t at = find_all(row, col);
         if (at == UINT64_MAX) {
-            auto it = m_queue_pos.find(key(row, col));
-            if (it == m_queue_pos.end())
+            const uint64_t q = m_queue_pos.find(key(row, col));
+            if (q == UINT64_MAX)
                 enqueue(row, col, sub);
             else
-                m_queue[it->second].sub = std::max(m_queue[it->second].sub, sub);
+                m_queue[q].sub = std::max(m_queue[q].sub, sub);
         } else {
             Pixel &p = m_all[at];
             p.sub = sub;
             p.weight = -1;
             p.lower = p.right = 0;
+            mark_dirty(at);
         }
     }
 
     // a pixel on the far edge of the subdivided square: queued at level 0 if new
     void place_edge(uint64_t row, uint64_t col)
     {
-        if (find_all(row, col) == UINT64_MAX && m_queue_pos.find(key(row, col)) == m_queue_pos.end())
+        if (find_all(row, col) == UINT64_MAX && m_queue_pos.find(key(row, col)) == UINT64_MAX)
             enqueue(row, col, 0);
     }
 
@@ -323,6 +448,7 @@ private:
         m_all[ind].sub = sub;
         m_all[ind].lower = m_all[ind].right = 0;
         m_all[ind].weight = -1;
+        mark_dirty(ind);
         const uint64_t d = exp2i(S.max_sub - sub);
         const uint64_t row = m_all[ind].row, col = m_all[ind].col;
         place_corner(row, col + d, sub);
@@ -343,7 +469,8 @@ public:
         // InitializeFirstGrid, FOORT/Mesh.cpp:726-790
         const uint64_t n = isqrt_trunc(S.initial);
         const uint64_t step = exp2i(S.max_sub - 1);
-        m_px.reserve(S.initial);
+        m_px.reserve(S.infinite ? S.initial : std::min<uint64_t>(std::max(S.max_pixels, S.initial) + 5 * S.iteration,
+                                                                 uint64_t(1) << 26));
         for (uint64_t row = 0; row < n; ++row)
             for (uint64_t col = 0; col < n; ++col)
                 m_px.push_back(Pixel{step * row, step * col, (row == n - 1 || col == n - 1) ? S.max_sub : 1});

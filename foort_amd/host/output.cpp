@@ -8,6 +8,7 @@
 #include "host.hpp"
 
 #include <algorithm>
+#include <charconv>
 #include <cstdio>
 #include <filesystem>
 #include <iostream>
@@ -51,7 +52,7 @@ void OutputHandler::PrepareForOutput(uint64_t count)
     if (m_cache.size() + count > m_to_cache)
         WriteCached();
     m_prev_cached = m_cache.size();
-    m_cache.resize(m_cache.size() + count);
+    m_cache.resize(m_cache.size() + count, Entry{0, 0, {0, 0, 0, 0}, kNoText});
 }
 
 void OutputHandler::NewGeodesicOutput(uint64_t index, uint64_t row, uint64_t col, std::vector<std::string> texts)
@@ -59,7 +60,34 @@ void OutputHandler::NewGeodesicOutput(uint64_t index, uint64_t row, uint64_t col
     Entry &e = m_cache[m_prev_cached + index];
     e.row = row;
     e.col = col;
-    e.texts = std::move(texts);
+    e.text = m_texts.size();
+    m_texts.push_back(std::move(texts));
+}
+
+void OutputHandler::NewGeodesicValues(uint64_t index, uint64_t row, uint64_t col, const double *values)
+{
+    Entry &e = m_cache[m_prev_cached + index];
+    e.row = row;
+    e.col = col;
+    for (size_t d = 0; d < m_kinds.size(); ++d)
+        e.values[d] = values[d];
+    e.text = kNoText;
+}
+
+void OutputHandler::AppendText(std::string &b, const Entry &e, int d) const
+{
+    if (e.text != kNoText) {
+        b += m_texts[e.text][d];
+        return;
+    }
+    char num[512];
+    if (m_kinds[d] == kInt) {
+        b.append(num, std::to_chars(num, num + sizeof num, static_cast<int>(e.values[d])).ptr);
+    } else {
+        // what std::to_string(double) writes ("%f"): correctly rounded to 6 decimals
+        auto r = std::to_chars(num, num + sizeof num, e.values[d], std::chars_format::fixed, 6);
+        b.append(num, r.ptr);
+    }
 }
 
 void OutputHandler::OutputFinished() { WriteCached(); }
@@ -111,11 +139,12 @@ void OutputHandler::WriteCached()
             ++nfiles;
         unsigned short cur = 1;
         unsigned long next = 0, last_count = 0;
-        const int ndiag = static_cast<int>(m_cache[0].texts.size());
+        const int ndiag =
+            static_cast<int>(m_cache[0].text == kNoText ? m_kinds.size() : m_texts[m_cache[0].text].size());
         if (m_in_file == 0)
             for (int i = 0; i < ndiag && !m_console; ++i)
                 OpenForFirstTime(FileName(i, static_cast<unsigned short>(m_full_files + cur)));
-        std::string buf;
+        std::vector<std::string> bufs;
         while (cur <= nfiles && !m_console) {
             unsigned long n = cur == 1 ? std::min(total, m_per_file - m_in_file) : std::min(total - next, m_per_file);
             for (int d = 0; d < ndiag; ++d) {
@@ -128,21 +157,33 @@ void OutputHandler::WriteCached()
                     m_console = true;
                     continue;
                 }
-                buf.clear();
-                for (unsigned long j = next; j < next + n; ++j) {
-                    const Entry &e = m_cache[j];
-                    buf += std::to_string(static_cast<unsigned long>(e.row));
-                    buf += ' ';
-                    buf += std::to_string(static_cast<unsigned long>(e.col));
-                    buf += "  ";
-                    buf += e.texts[d];
-                    buf += '\n';
-                    if (buf.size() > (1u << 22)) {
-                        std::fwrite(buf.data(), 1, buf.size(), f);
-                        buf.clear();
+                // lines "row col  text", formatted in parallel chunks and written in order
+                const unsigned long kChunk = 1ul << 16;
+                const long long chunks = static_cast<long long>((n + kChunk - 1) / kChunk);
+                const long long group = 64;
+                for (long long c0 = 0; c0 < chunks; c0 += group) {
+                    const long long c1 = std::min(chunks, c0 + group);
+                    bufs.resize(static_cast<size_t>(c1 - c0));
+#pragma omp parallel for schedule(dynamic, 1)
+                    for (long long c = c0; c < c1; ++c) {
+                        std::string &b = bufs[static_cast<size_t>(c - c0)];
+                        b.clear();
+                        const unsigned long lo = next + static_cast<unsigned long>(c) * kChunk;
+                        const unsigned long hi = std::min(next + n, lo + kChunk);
+                        char num[48];
+                        for (unsigned long j = lo; j < hi; ++j) {
+                            const Entry &e = m_cache[j];
+                            b.append(num, std::to_chars(num, num + sizeof num, e.row).ptr);
+                            b += ' ';
+                            b.append(num, std::to_chars(num, num + sizeof num, e.col).ptr);
+                            b += "  ";
+                            AppendText(b, e, d);
+                            b += '\n';
+                        }
                     }
+                    for (const std::string &b : bufs)
+                        std::fwrite(b.data(), 1, b.size(), f);
                 }
-                std::fwrite(buf.data(), 1, buf.size(), f);
                 std::fclose(f);
             }
             ++cur;
@@ -166,16 +207,22 @@ void OutputHandler::WriteCached()
         screen_output("Done writing cached geodesic output to file(s).", Level::Subproc);
     }
     if (m_console) {
+        std::string t;
         for (const Entry &e : m_cache) {
             screen_output(std::to_string(static_cast<unsigned long>(e.row)) + " " +
                               std::to_string(static_cast<unsigned long>(e.col)) + " ",
                           Level::Proc);
-            for (const std::string &t : e.texts)
+            const int nd = static_cast<int>(e.text == kNoText ? m_kinds.size() : m_texts[e.text].size());
+            for (int d = 0; d < nd; ++d) {
+                t.clear();
+                AppendText(t, e, d);
                 screen_output(t, Level::Proc);
+            }
         }
         std::cout.flush();
     }
     m_cache.clear();
+    m_texts.clear();
 }
 
 } // namespace foort_host

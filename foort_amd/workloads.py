@@ -37,3 +37,43 @@ C1_SIDE = 1024
 C1_TEXT = ("Kerr a=0.99, observer r=1000M, inclination 85 deg, 1024x1024 SimpleSquareMesh over a 30Mx30M "
            "screen, RK4 adaptive step (eps 0.03), FP64, horizon / boundary r=1000M / time-out 1e5 steps, "
            "FourColorScreen + EquatorialPasses + ClosestRadius, RLogScale off")
+
+
+# C2: adaptive SquareSubdivisionMesh, Kerr a=0.94, inclination 60 deg: a 256x256
+# initial grid refined where neighbouring FourColorScreen values differ, 8 levels
+# of subdivision (MaxSubdivide 9), at most 4,000,000 pixels; RLogScale on, so
+# geodesics take ~10^2 steps and photon-ring orbits up to the 10^5-step time-out.
+C2_TEXT = ("C2: SquareSubdivisionMeshV2 on Kerr a=0.94, 60 deg, 30Mx30M screen: 256x256 initial grid, MaxSubdivide 9 "
+           "(8 subdivision levels), IterationPixels 1,000,000, MaxPixels 4,000,000, refining on FourColorScreen; "
+           "RK4 adaptive step, FP64, RLogScale on, horizon / boundary r=1000M / time-out 1e5 steps, "
+           "FourColorScreen + EquatorialPasses + ClosestRadius")
+
+
+def c2_config(mesh="SquareSubdivisionMeshV2", christoffel="FD", precision="FP64", schedule="OneShot",
+              prefix="c2", initial=65536, max_pixels=4000000, iteration=1000000, max_subdivide=9,
+              autotune="None", use_graph=False):
+    """The C2 run as a configuration file text for the foort program."""
+    return f"""
+Metric = {{ Name = "Kerr"; a = 0.94; RLogScale = true; }};
+Diagnostics = {{
+  FourColorScreen = {{ On = true; UseForMesh = true; }};
+  EquatorialPasses = {{ On = true; }};
+  ClosestRadius = {{ On = true; }};
+}};
+Terminations = {{
+  Horizon = {{ On = true; Epsilon_Horizon = 0.01; }};
+  BoundarySphere = {{ On = true; SphereRadius = 1000.0; }};
+  TimeOut = {{ On = true; MaxSteps = 100000; }};
+}};
+ViewScreen = {{
+  Position = {{ t = 0.0; r = 1000.0; theta = {math.radians(60.0)!r}; phi = 0.0; }};
+  ScreenSize = {{ x = 30.0; y = 30.0; }};
+  Mesh = {{ Type = "{mesh}"; InitialPixels = {initial}; MaxPixels = {max_pixels}; IterationPixels = {iteration};
+           MaxSubdivide = {max_subdivide}; InitialSubdivisionToFinal = false; }};
+}};
+Source = {{ Name = "NoSource"; }};
+Integrator = {{ Type = "RK4"; StepSize = 0.03; }};
+GPU = {{ Christoffel = "{christoffel}"; Precision = "{precision}"; Schedule = "{schedule}";
+        Autotune = "{autotune}"; UseGraph = {"true" if use_graph else "false"}; }};
+Output = {{ FilePrefix = "{prefix}"; FileExtension = "dat"; TimeStamp = false; ScreenOutputLevel = 2; }};
+"""

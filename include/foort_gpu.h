@@ -135,7 +135,9 @@ typedef struct foort_run_desc {
     double mixed_pole_margin;    /* ... and where min(theta, pi - theta) > this (0.1) */
     uint32_t launch_us;          /* persistent schedule: end each kernel launch after about this many
                                     microseconds (0 = no time limit; with both limits 0 a batch is one launch) */
-    uint32_t reserved1;
+    uint32_t use_graph;          /* host entry points: run each batch that is one launch (one-shot, or
+                                    persistent without limits) as a captured HIP graph -- index upload,
+                                    kernels and result copies -- replayed while the batch shape is unchanged */
 } foort_run_desc;
 
 /*
@@ -200,6 +202,19 @@ int foort_gpu_trace_pixels(foort_gpu_ctx *ctx, uint64_t row_column_size, uint64_
 int foort_gpu_trace_device(foort_gpu_ctx *ctx, uint64_t row_column_size, uint64_t first,
                            uint64_t count, const uint64_t *screen_index,
                            const foort_geodesic_results *out, void *stream, foort_stats *stats);
+
+/* Autotuner (no counterpart in the reference): time a sample batch -- screen_index as for
+ * foort_gpu_trace_pixels (host memory), or NULL for grid indices 0 .. count-1 -- under
+ * candidate execution options and keep the fastest in the context: the one-shot block size
+ * (64, 128, 256), the persistent schedule's steps between refills (8, 32, 128) and refill
+ * threshold (1, 16, 64), each batch as one launch; with FOORT_TUNE_SCHEDULE in flags also
+ * the schedule, else the context's own.  Launch limits (steps_per_launch, launch_us) are kept.
+ * *chosen (if not NULL) receives the context's descriptor afterwards, *best the timing of the
+ * chosen options on the sample. */
+#define FOORT_TUNE_SCHEDULE 0x1u
+int foort_gpu_autotune(foort_gpu_ctx *ctx, uint64_t row_column_size, uint64_t count,
+                       const uint64_t *screen_index, uint32_t flags, foort_run_desc *chosen,
+                       foort_stats *best);
 
 /* Device memory helpers for callers that keep batches resident on the GPU
  * (no counterpart in the reference): allocate, free, copy, and wait for all

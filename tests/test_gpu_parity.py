@@ -150,3 +150,51 @@ def test_persistent_pixel_list_verlet():
     o = O.trace("fm", desc, 250, screen_index=px)
     for k in FIELDS:
         assert bitwise_equal(g[k], o[k]), k
+
+
+@pytest.mark.parametrize("schedule", [A.SCHEDULE_ONESHOT, A.SCHEDULE_PERSISTENT])
+def test_graph_batches_replay_bitexact(schedule):
+    """use_graph: each batch runs as a captured HIP graph (index upload, kernels, result
+    copies).  The same batch replayed three times, with eager work of another context between
+    the replays, then a batch of another shape (captured again), all equal the oracle."""
+    from tests.cases import sample_pixels
+    desc = c1(rlog=True, max_steps=10000, schedule=schedule, use_graph=1)
+    eager = c1(rlog=True, max_steps=10000)
+    px = sample_pixels(96, 900, 11)
+    o = O.trace("fm", desc, 96, screen_index=px)
+    print(f"[graph] schedule={schedule}", flush=True)
+    F = _gpu()
+    with F.Context(desc) as ctx, F.Context(eager) as ectx:
+        for rep in range(3):
+            g = ctx.trace_pixels(96, px)
+            for k in FIELDS:
+                assert bitwise_equal(g[k], o[k]), (rep, k)
+            e = ectx.trace_pixels(96, px[::-1])
+            assert bitwise_equal(e["quadrant"], o["quadrant"][::-1])
+        g2 = ctx.trace_grid(40)
+        o2 = O.trace("fm", desc, 40)
+        for k in FIELDS:
+            assert bitwise_equal(g2[k], o2[k]), k
+        g3 = ctx.trace_pixels(96, px[:100])
+        for k in FIELDS:
+            assert bitwise_equal(g3[k], o[k][:100]), k
+
+
+def test_autotune_picks_options_and_keeps_results():
+    from tests.cases import sample_pixels
+    desc = c1(rlog=True, max_steps=10000)
+    px = sample_pixels(64, 1500, 5)
+    print("[autotune]", flush=True)
+    F = _gpu()
+    with F.Context(desc) as ctx:
+        chosen, st = ctx.autotune(64, screen_index=px, tune_schedule=True)
+        assert chosen.schedule in (A.SCHEDULE_ONESHOT, A.SCHEDULE_PERSISTENT)
+        if chosen.schedule == A.SCHEDULE_ONESHOT:
+            assert chosen.block_size in (64, 128, 256)
+        else:
+            assert chosen.refill_interval in (8, 32, 128) and chosen.refill_lanes in (1, 16, 64)
+        assert st.geodesics == len(px) and st.device_ms > 0
+        g = ctx.trace_grid(48)
+    o = O.trace("fm", desc, 48)
+    for k in FIELDS:
+        assert bitwise_equal(g[k], o[k]), k

@@ -427,3 +427,33 @@ def test_host_library_exports_every_declared_symbol():
     L = P.lib()
     for n in names:
         assert hasattr(L, n), n
+
+
+def test_program_writes_value_texts_like_to_string(tmp_path):
+    """The driver keeps the diagnostics as numbers and formats them when it writes the
+    files; the texts must be std::to_string's: "%d" for the quadrant and the passes,
+    "%f" (six decimals, correctly rounded) for the closest radius."""
+    vals = np.array([0.0, -1.0, 2.5e-7, 5e-7, 1.5e-6, 0.1234565, 2.0000005, 999.9999995, 1e-300, 123456789.123456789,
+                     1e22, 3.141592653589793, 0.3, 7.25, 1000.0, 6.0000004999], np.float64)
+    side = 4
+
+    def tracer(desc, s, si):
+        n = len(si)
+        assert s == side and n == side * side
+        return {"term": np.full(n, 2, np.int32), "quadrant": np.arange(n, dtype=np.int32) % 5,
+                "equat_passes": np.arange(n, dtype=np.int32) - 7, "closest_radius": vals[:n]}
+
+    cfg = tmp_path / "v.cfg"
+    cfg.write_text(f"""
+    Diagnostics = {{ FourColorScreen = {{ On = true; UseForMesh = true; }}; EquatorialPasses = {{ On = true; }};
+                    ClosestRadius = {{ On = true; }}; }};
+    ViewScreen = {{ Mesh = {{ Type = "SimpleSquareMesh"; TotalPixels = {side * side}; }}; }};
+    Output = {{ FilePrefix = "{tmp_path}/v"; FileExtension = "dat"; TimeStamp = false; FirstLineInfo = false;
+               ScreenOutputLevel = 0; }};
+    """)
+    P.run(str(cfg), tracer=tracer)
+    rows = [(i // side, i % side) for i in range(side * side)]
+    cr = (tmp_path / "v_ClosestRadius.dat").read_text().splitlines()
+    assert cr == [f"{r} {c}  {v:f}" for (r, c), v in zip(rows, vals)]
+    ep = (tmp_path / "v_EquatPasses.dat").read_text().splitlines()
+    assert ep == [f"{r} {c}  {i - 7}" for i, (r, c) in enumerate(rows)]
