@@ -170,10 +170,84 @@ def run_c2(args):
         shutil.rmtree(tmp, ignore_errors=True)
 
 
+def run_c3(args):
+    """C3: precision study.  Kerr a=0.998 at inclinations 17 and 85 deg, 2048x2048 uniform grid
+    (otherwise as C1).  The mixed path (binary32 steps far from the hole and the poles) and the
+    analytic binary64 path against the finite-difference binary64 path, which is bit for bit the
+    CPU oracle: integer-diagnostic mismatch rates over all pixels and over the pixels none of
+    whose four neighbours differ in the oracle image (the photon-ring boundary is chaotic), and
+    relative errors of the final positions."""
+    import numpy as np
+    import foort_amd as F
+    import foort_amd.abi as A
+    from foort_amd import workloads as W
+    side = args.side if args.side != 1024 else 2048
+    n = side * side
+    paths = [("fd_fp64", dict()),
+             ("analytic_fp64", dict(christoffel=A.CHRISTOFFEL_ANALYTIC)),
+             ("mixed", dict(christoffel=A.CHRISTOFFEL_ANALYTIC, precision=A.PRECISION_MIXED))]
+    res = F.device_results(n, fields=("term", "quadrant", "equat_passes", "final_pos"))
+    study = {}
+    for inc in (17.0, 85.0):
+        out = {}
+        for name, kw in paths:
+            with F.Context(W.c1(params=(0.998,), inclination_deg=inc, **kw)) as ctx:
+                st = ctx.trace_device(side, 0, n, res)
+            out[name] = {k: v.numpy() for k, v in res.items()}
+            out[name]["stats"] = st
+            log(f"c3 {inc:.0f} deg {name}: {st.device_ms / 1e3:.2f} s kernels, {st.steps} steps")
+        ref = out["fd_fp64"]
+        ints = np.stack([ref["term"], ref["quadrant"], ref["equat_passes"]], -1).reshape(side, side, 3)
+        edge = np.zeros((side, side), bool)
+        for ax in (0, 1):
+            d = np.any(np.diff(ints, axis=ax) != 0, axis=-1)
+            if ax == 0:
+                edge[1:, :] |= d
+                edge[:-1, :] |= d
+            else:
+                edge[:, 1:] |= d
+                edge[:, :-1] |= d
+        edge = edge.reshape(-1)
+        entry = {"boundary_pixel_fraction": float(edge.mean())}
+        for name in ("analytic_fp64", "mixed"):
+            o = out[name]
+            mism = (o["term"] != ref["term"]) | (o["quadrant"] != ref["quadrant"]) | \
+                   (o["equat_passes"] != ref["equat_passes"])
+            same = ~mism
+            rel = np.abs(o["final_pos"] - ref["final_pos"]) / np.maximum(np.abs(ref["final_pos"]), 1e-300)
+            relmax = rel.max(axis=1)[same]
+            st = o["stats"]
+            entry[name] = {
+                "integer_diag_mismatch_rate": float(mism.mean()),
+                "mismatch_rate_off_boundary": float(mism[~edge].mean()),
+                "mismatched_pixels": int(mism.sum()),
+                "mismatched_off_boundary": int(mism[~edge].sum()),
+                "pos_rel_err_median": float(np.median(relmax)),
+                "pos_rel_err_p99": float(np.quantile(relmax, 0.99)),
+                "pos_rel_err_max": float(relmax.max()),
+                "kernel_s": st.device_ms / 1e3,
+                "pixels_per_s_kernel": n / (st.device_ms / 1e3),
+                "speedup_vs_fd_kernel": out["fd_fp64"]["stats"].device_ms / st.device_ms,
+            }
+        entry["fd_fp64"] = {"kernel_s": out["fd_fp64"]["stats"].device_ms / 1e3,
+                            "geodesic_steps": out["fd_fp64"]["stats"].steps}
+        study[f"{inc:.0f}deg"] = entry
+    mixed85 = study["85deg"]["mixed"]
+    line = {"metric": "integer-diagnostic mismatch rate of the mixed path vs the FP64 oracle",
+            "value": max(study[k]["mixed"]["integer_diag_mismatch_rate"] for k in study), "unit": "fraction",
+            "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": mixed85["kernel_s"] * 1e3,
+            "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+            "data": "synthetic",
+            "config": {"workload": f"C3: Kerr a=0.998, 17 and 85 deg, {side}x{side}, otherwise as C1"},
+            "study": study}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["c1", "c2"], default="c1",
-                    help="c1: the 1024^2 drop-in image (default); c2: the adaptive-mesh run")
+    ap.add_argument("--workload", choices=["c1", "c2", "c3"], default="c1",
+                    help="c1: the 1024^2 drop-in image (default); c2: the adaptive-mesh run; "
+                         "c3: the precision study")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
@@ -198,6 +272,9 @@ def main():
 
     if args.workload == "c2":
         run_c2(args)
+        return
+    if args.workload == "c3":
+        run_c3(args)
         return
 
     import foort_amd as F
