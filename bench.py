@@ -243,11 +243,54 @@ def run_c3(args):
     print(json.dumps(line), flush=True)
 
 
+def run_c4(args):
+    """C4: the render server under a seeded open-loop request stream (Poisson arrivals):
+    latency (submission to completion) percentiles and throughput."""
+    import numpy as np
+    from foort_amd import program as P
+    from foort_amd import workloads as W
+    n = args.requests
+    reqs = W.c4_requests(seed=args.seed, n=n, rate=args.rate)
+    with P.Server(workers=args.workers, hbm_budget_bytes=args.hbm_budget << 20) as srv:
+        t0 = time.perf_counter()
+        for k, (t, desc, mesh, _) in enumerate(reqs):
+            wait = t - (time.perf_counter() - t0)
+            if wait > 0:
+                time.sleep(wait)
+            srv.submit(desc, mesh, k)
+        srv.drain()
+        res = srv.results(cap=n + 16)
+    assert len(res) == n and all(r.status == 0 for r in res), [r.status for r in res if r.status]
+    sub = np.array([r.submit_s for r in res])
+    lat = np.array([r.end_s - r.submit_s for r in res])
+    wait = np.array([r.start_s - r.submit_s for r in res])
+    span = max(r.end_s for r in res) - sub.min()
+    pixels = sum(r.pixels for r in res)
+    kernel_s = sum(r.kernel_ms for r in res) / 1e3
+    line = {"metric": "render-server request latency p99", "value": float(np.quantile(lat, 0.99)), "unit": "s",
+            "n_gpus": 1, "steps": n, "warmup": 0, "ms_per_step": float(np.quantile(lat, 0.5)) * 1e3,
+            "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+            "data": "synthetic",
+            "config": {"workload": W.C4_TEXT.format(seed=args.seed, rate=args.rate), "workers": args.workers,
+                       "hbm_budget_bytes": args.hbm_budget << 20},
+            "latency_p50_s": float(np.quantile(lat, 0.5)), "latency_p99_s": float(np.quantile(lat, 0.99)),
+            "latency_max_s": float(lat.max()), "queue_wait_p99_s": float(np.quantile(wait, 0.99)),
+            "offered_rate_per_s": args.rate, "completed_rate_per_s": n / span,
+            "pixels_per_s": pixels / span, "pixels": int(pixels), "geodesic_steps": int(sum(r.steps for r in res)),
+            "gpu_kernel_s_summed_over_streams": kernel_s, "span_s": span}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["c1", "c2", "c3"], default="c1",
+    ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4"], default="c1",
                     help="c1: the 1024^2 drop-in image (default); c2: the adaptive-mesh run; "
-                         "c3: the precision study")
+                         "c3: the precision study; c4: the render server")
+    ap.add_argument("--requests", type=int, default=2000, help="c4: requests in the stream")
+    ap.add_argument("--rate", type=float, default=40.0, help="c4: offered request rate (per s)")
+    ap.add_argument("--seed", type=int, default=2024, help="c4: seed of the request stream")
+    ap.add_argument("--workers", type=int, default=4, help="c4: requests served at once")
+    ap.add_argument("--hbm-budget", type=int, default=4096, help="c4: admission budget in MiB")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
@@ -275,6 +318,9 @@ def main():
         return
     if args.workload == "c3":
         run_c3(args)
+        return
+    if args.workload == "c4":
+        run_c4(args)
         return
 
     import foort_amd as F

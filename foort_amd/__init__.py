@@ -22,7 +22,7 @@ EXPORTS = [
     "foort_desc_init", "foort_desc_size", "foort_gpu_open", "foort_gpu_close", "foort_gpu_last_error",
     "foort_gpu_trace_grid", "foort_gpu_trace_pixels", "foort_gpu_trace_device", "foort_gpu_camera",
     "foort_gpu_device_alloc", "foort_gpu_device_free", "foort_gpu_copy_to_device", "foort_gpu_copy_to_host",
-    "foort_gpu_synchronize", "foort_gpu_autotune",
+    "foort_gpu_synchronize", "foort_gpu_autotune", "foort_gpu_configure",
 ]
 
 
@@ -49,6 +49,7 @@ def lib():
         L.foort_gpu_trace_device.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_void_p,
                                              C.POINTER(GeodesicResults), C.c_void_p, C.POINTER(Stats)]
         L.foort_gpu_camera.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.foort_gpu_configure.argtypes = [C.c_void_p, C.POINTER(RunDesc)]
         L.foort_gpu_autotune.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p, C.c_uint32,
                                          C.POINTER(RunDesc), C.POINTER(Stats)]
         L.foort_gpu_device_alloc.argtypes = [C.c_size_t, C.POINTER(C.c_void_p)]
@@ -147,6 +148,10 @@ class Context:
                                                    C.byref(st) if st is not None else None),
                     "foort_gpu_trace_device")
         return st
+
+    def configure(self, desc):
+        """foort_gpu_configure: another run description on this context's buffers."""
+        self._check(self._L.foort_gpu_configure(self._h, C.byref(desc)), "foort_gpu_configure")
 
     def autotune(self, side, screen_index=None, count=None, tune_schedule=False):
         """foort_gpu_autotune on a sample batch (a (row, column) list, or grid indices

@@ -76,7 +76,7 @@ def build_opcount(verbose=False):
         f.write(res.stdout)
 
 
-HOST_SOURCES = ["cfg.cpp", "config.cpp", "mesh.cpp", "output.cpp", "driver.cpp"]
+HOST_SOURCES = ["cfg.cpp", "config.cpp", "mesh.cpp", "output.cpp", "driver.cpp", "server.cpp"]
 HOST_LIB = os.path.join(PKG, "libfoort_host.so")
 CLI = os.path.join(PKG, "foort")
 

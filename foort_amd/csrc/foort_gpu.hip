@@ -450,17 +450,18 @@ int trace_host(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, uint64_t count
 
     const bool graph = ctx->desc.use_graph && single_launch(ctx) && count > 0;
     if (!graph) {
+        // everything on the context's own stream, so contexts on other host threads run concurrently
         if (si)
-            HIP_TRY(ctx, hipMemcpy(base + o_si, si, count * 16, hipMemcpyHostToDevice));
-        HIP_TRY(ctx, hipEventRecord(ctx->ev0, nullptr));
-        rc = launch(ctx, side, first, count, d_si, d, nullptr);
+            HIP_TRY(ctx, hipMemcpyAsync(base + o_si, si, count * 16, hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+        rc = launch(ctx, side, first, count, d_si, d, ctx->stream);
         if (rc)
             return rc;
-        HIP_TRY(ctx, hipEventRecord(ctx->ev1, nullptr));
-        HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+        HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
         for (const Field &x : f)
             if (x.host)
-                HIP_TRY(ctx, hipMemcpy(x.host, base + x.off, x.bytes, hipMemcpyDeviceToHost));
+                HIP_TRY(ctx, hipMemcpyAsync(x.host, base + x.off, x.bytes, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     } else {
         rc = ensure_pinned(ctx, off);
         if (rc)
@@ -528,7 +529,8 @@ int trace_host(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, uint64_t count
     }
     if (stats) {
         unsigned long long steps = 0;
-        HIP_TRY(ctx, hipMemcpy(&steps, ctx->d_steps, sizeof(steps), hipMemcpyDeviceToHost));
+        HIP_TRY(ctx, hipMemcpyAsync(&steps, ctx->d_steps, sizeof(steps), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
         float ms = 0;
         HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
         stats->geodesics = count;
@@ -536,6 +538,95 @@ int trace_host(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, uint64_t count
         stats->launches = ctx->last_launches;
         stats->device_ms = ms;
     }
+    return FOORT_OK;
+}
+
+// Validate a run description and derive what the kernels take (the run constants and
+// the kernel pair).  Nothing of ctx changes unless the description is valid.
+int configure(foort_gpu_ctx *ctx, const foort_run_desc &d)
+{
+    if (d.integrator != FOORT_INTEGRATOR_RK4 && d.integrator != FOORT_INTEGRATOR_VERLET)
+        return fail(ctx, FOORT_EINVAL, "unknown integrator");
+    if (d.christoffel != FOORT_CHRISTOFFEL_FD && d.christoffel != FOORT_CHRISTOFFEL_ANALYTIC)
+        return fail(ctx, FOORT_EINVAL, "unknown Christoffel policy");
+    if (d.precision != FOORT_PRECISION_FP64 && d.precision != FOORT_PRECISION_MIXED)
+        return fail(ctx, FOORT_EINVAL, "unknown precision");
+    if (d.precision == FOORT_PRECISION_MIXED && d.christoffel != FOORT_CHRISTOFFEL_ANALYTIC)
+        return fail(ctx, FOORT_EINVAL, "mixed precision needs the analytic Christoffel policy");
+    if (d.schedule != FOORT_SCHEDULE_ONESHOT && d.schedule != FOORT_SCHEDULE_PERSISTENT)
+        return fail(ctx, FOORT_EINVAL, "unknown schedule");
+    if (d.diag_flags & FOORT_DIAGFLAG_GEODESICPOSITION)
+        return fail(ctx, FOORT_EINVAL, "GeodesicPosition is not supported by the GPU path");
+    if (!(d.derivative_h > 0))
+        return fail(ctx, FOORT_EINVAL, "DerivativeH must be positive");
+    if (d.step_cap > 0x7fffffffull)
+        return fail(ctx, FOORT_EINVAL, "step_cap above 2^31-1");
+    if (!(d.camera_position[1] > 0))
+        return fail(ctx, FOORT_EINVAL, "camera radius must be positive");
+
+    RunConst R;
+    std::memset(&R, 0, sizeof(R));
+    std::string err;
+    if (!init_metric(d, R.M, err))
+        return fail(ctx, FOORT_EINVAL, err);
+    R.eps = d.step_size;
+    R.minstep = d.smallest_step;
+    R.h_fd = d.derivative_h;
+    R.inv2h = 1 / (2 * d.derivative_h); // FOORT/Metric.cpp:30: / (2 * h) is * (1 / (2 * h))
+    R.verlet_tol = d.verlet_tolerance;
+    R.terms = d.term_flags & 0xfu;
+    if (!R.M.has_horizon) // Config turns Horizon off for metrics without one (FOORT/Config.cpp:652-658)
+        R.terms &= ~FOORT_TERMFLAG_HORIZON;
+    R.diags = d.diag_flags & (FOORT_DIAGFLAG_FOURCOLORSCREEN | FOORT_DIAGFLAG_EQUATORIALPASSES | FOORT_DIAGFLAG_CLOSESTRADIUS);
+    R.h_eps = d.horizon_epsilon;
+    R.sphere_R = d.sphere_radius;
+    R.theta_eps = d.theta_epsilon;
+    R.h_n = clamp_u32(d.horizon_update_n);
+    R.s_n = clamp_u32(d.sphere_update_n);
+    R.t_n = clamp_u32(d.timeout_update_n);
+    R.th_n = clamp_u32(d.theta_update_n);
+    R.max_steps = clamp_u32(d.timeout_max_steps);
+    R.step_cap = d.step_cap ? static_cast<uint32_t>(d.step_cap) : 0x7fffffffu;
+    R.ep_thr = d.ep_threshold;
+    R.ep_n = clamp_u32(d.ep_update.update_n_steps);
+    R.ep_start = d.ep_update.update_start;
+    R.ep_finish = d.ep_update.update_finish;
+    R.cr_n = clamp_u32(d.cr_update.update_n_steps);
+    R.cr_start = d.cr_update.update_start;
+    R.cr_finish = d.cr_update.update_finish;
+    R.cam_t = d.camera_position[0];
+    R.cam_r = d.camera_position[1];
+    R.cam_u = R.M.rlog ? std::log(d.camera_position[1]) : d.camera_position[1];
+    R.cam_theta = d.camera_position[2];
+    R.cam_phi = d.camera_position[3];
+    R.sin_theta0 = fm_sin(d.camera_position[2]);
+    R.size_x = d.screen_size[0];
+    R.size_y = d.screen_size[1];
+    R.center_x = d.screen_center[0];
+    R.center_y = d.screen_center[1];
+    R.mixed_u = R.M.rlog ? std::log(d.mixed_switch_radius) : d.mixed_switch_radius;
+    R.mixed_pole = d.mixed_pole_margin;
+    build_camera(d.metric, R, d.camera_position);
+    // kernels without update counters when every termination and diagnostic updates every step
+    // (UpdateFrequency 1; for a termination 0 means the same, FOORT/Terminations.cpp:71-90)
+    auto term_every = [&R](uint32_t flag, uint32_t n) { return !(R.terms & flag) || n <= 1; };
+    const bool every = term_every(FOORT_TERMFLAG_HORIZON, R.h_n) && term_every(FOORT_TERMFLAG_BOUNDARYSPHERE, R.s_n) &&
+                       term_every(FOORT_TERMFLAG_TIMEOUT, R.t_n) && term_every(FOORT_TERMFLAG_THETASINGULARITY, R.th_n) &&
+                       (!(R.diags & FOORT_DIAGFLAG_EQUATORIALPASSES) || R.ep_n == 1) &&
+                       (!(R.diags & FOORT_DIAGFLAG_CLOSESTRADIUS) || R.cr_n == 1);
+    const KernelPair kernels = d.christoffel == FOORT_CHRISTOFFEL_FD
+                       ? foort::pick_policy<foort::RhsFD>(d.metric, d.integrator, every)
+                       : foort::pick_analytic(d.metric, d.integrator, static_cast<int>(d.precision), every);
+    if (ctx->kernels.persistent != kernels.persistent && ctx->d_slots) {
+        // the lane count follows the persistent kernel's occupancy: size the slots again
+        (void)hipFree(ctx->d_slots);
+        ctx->d_slots = nullptr;
+    }
+    drop_graph(ctx);
+    ctx->desc = d;
+    ctx->rc = R;
+    ctx->kernels = kernels;
+    ctx->budget = d.steps_per_launch ? d.steps_per_launch : UINT64_MAX;
     return FOORT_OK;
 }
 
@@ -590,84 +681,13 @@ int foort_gpu_open(const foort_run_desc *desc, foort_gpu_ctx **out)
     if (!desc || !out)
         return fail(nullptr, FOORT_EINVAL, "NULL descriptor or context pointer");
     *out = nullptr;
-    const foort_run_desc &d = *desc;
-    if (d.integrator != FOORT_INTEGRATOR_RK4 && d.integrator != FOORT_INTEGRATOR_VERLET)
-        return fail(nullptr, FOORT_EINVAL, "unknown integrator");
-    if (d.christoffel != FOORT_CHRISTOFFEL_FD && d.christoffel != FOORT_CHRISTOFFEL_ANALYTIC)
-        return fail(nullptr, FOORT_EINVAL, "unknown Christoffel policy");
-    if (d.precision != FOORT_PRECISION_FP64 && d.precision != FOORT_PRECISION_MIXED)
-        return fail(nullptr, FOORT_EINVAL, "unknown precision");
-    if (d.precision == FOORT_PRECISION_MIXED && d.christoffel != FOORT_CHRISTOFFEL_ANALYTIC)
-        return fail(nullptr, FOORT_EINVAL, "mixed precision needs the analytic Christoffel policy");
-    if (d.schedule != FOORT_SCHEDULE_ONESHOT && d.schedule != FOORT_SCHEDULE_PERSISTENT)
-        return fail(nullptr, FOORT_EINVAL, "unknown schedule");
-    if (d.diag_flags & FOORT_DIAGFLAG_GEODESICPOSITION)
-        return fail(nullptr, FOORT_EINVAL, "GeodesicPosition is not supported by the GPU path");
-    if (!(d.derivative_h > 0))
-        return fail(nullptr, FOORT_EINVAL, "DerivativeH must be positive");
-    if (d.step_cap > 0x7fffffffull)
-        return fail(nullptr, FOORT_EINVAL, "step_cap above 2^31-1");
-    if (!(d.camera_position[1] > 0))
-        return fail(nullptr, FOORT_EINVAL, "camera radius must be positive");
-
     foort_gpu_ctx *ctx = new foort_gpu_ctx();
-    ctx->desc = d;
-    RunConst &R = ctx->rc;
-    std::memset(&R, 0, sizeof(R));
-    std::string err;
-    if (!init_metric(d, R.M, err)) {
+    int rc = configure(ctx, *desc);
+    if (rc) {
+        g_open_error = ctx->err;
         delete ctx;
-        return fail(nullptr, FOORT_EINVAL, err);
+        return rc;
     }
-    R.eps = d.step_size;
-    R.minstep = d.smallest_step;
-    R.h_fd = d.derivative_h;
-    R.inv2h = 1 / (2 * d.derivative_h); // FOORT/Metric.cpp:30: / (2 * h) is * (1 / (2 * h))
-    R.verlet_tol = d.verlet_tolerance;
-    R.terms = d.term_flags & 0xfu;
-    if (!R.M.has_horizon) // Config turns Horizon off for metrics without one (FOORT/Config.cpp:652-658)
-        R.terms &= ~FOORT_TERMFLAG_HORIZON;
-    R.diags = d.diag_flags & (FOORT_DIAGFLAG_FOURCOLORSCREEN | FOORT_DIAGFLAG_EQUATORIALPASSES | FOORT_DIAGFLAG_CLOSESTRADIUS);
-    R.h_eps = d.horizon_epsilon;
-    R.sphere_R = d.sphere_radius;
-    R.theta_eps = d.theta_epsilon;
-    R.h_n = clamp_u32(d.horizon_update_n);
-    R.s_n = clamp_u32(d.sphere_update_n);
-    R.t_n = clamp_u32(d.timeout_update_n);
-    R.th_n = clamp_u32(d.theta_update_n);
-    R.max_steps = clamp_u32(d.timeout_max_steps);
-    R.step_cap = d.step_cap ? static_cast<uint32_t>(d.step_cap) : 0x7fffffffu;
-    R.ep_thr = d.ep_threshold;
-    R.ep_n = clamp_u32(d.ep_update.update_n_steps);
-    R.ep_start = d.ep_update.update_start;
-    R.ep_finish = d.ep_update.update_finish;
-    R.cr_n = clamp_u32(d.cr_update.update_n_steps);
-    R.cr_start = d.cr_update.update_start;
-    R.cr_finish = d.cr_update.update_finish;
-    R.cam_t = d.camera_position[0];
-    R.cam_r = d.camera_position[1];
-    R.cam_u = R.M.rlog ? std::log(d.camera_position[1]) : d.camera_position[1];
-    R.cam_theta = d.camera_position[2];
-    R.cam_phi = d.camera_position[3];
-    R.sin_theta0 = fm_sin(d.camera_position[2]);
-    R.size_x = d.screen_size[0];
-    R.size_y = d.screen_size[1];
-    R.center_x = d.screen_center[0];
-    R.center_y = d.screen_center[1];
-    R.mixed_u = R.M.rlog ? std::log(d.mixed_switch_radius) : d.mixed_switch_radius;
-    R.mixed_pole = d.mixed_pole_margin;
-    build_camera(d.metric, R, d.camera_position);
-    // kernels without update counters when every termination and diagnostic updates every step
-    // (UpdateFrequency 1; for a termination 0 means the same, FOORT/Terminations.cpp:71-90)
-    auto term_every = [&R](uint32_t flag, uint32_t n) { return !(R.terms & flag) || n <= 1; };
-    const bool every = term_every(FOORT_TERMFLAG_HORIZON, R.h_n) && term_every(FOORT_TERMFLAG_BOUNDARYSPHERE, R.s_n) &&
-                       term_every(FOORT_TERMFLAG_TIMEOUT, R.t_n) && term_every(FOORT_TERMFLAG_THETASINGULARITY, R.th_n) &&
-                       (!(R.diags & FOORT_DIAGFLAG_EQUATORIALPASSES) || R.ep_n == 1) &&
-                       (!(R.diags & FOORT_DIAGFLAG_CLOSESTRADIUS) || R.cr_n == 1);
-    ctx->kernels = d.christoffel == FOORT_CHRISTOFFEL_FD
-                       ? foort::pick_policy<foort::RhsFD>(d.metric, d.integrator, every)
-                       : foort::pick_analytic(d.metric, d.integrator, static_cast<int>(d.precision), every);
-    ctx->budget = d.steps_per_launch ? d.steps_per_launch : UINT64_MAX;
 
     hipError_t e = hipGetDevice(&ctx->device);
     if (e == hipSuccess)
@@ -682,6 +702,8 @@ int foort_gpu_open(const foort_run_desc *desc, foort_gpu_ctx **out)
         e = hipEventCreate(&ctx->ev0);
     if (e == hipSuccess)
         e = hipEventCreate(&ctx->ev1);
+    if (e == hipSuccess)
+        e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         std::string msg = std::string("HIP initialisation: ") + hipGetErrorString(e);
         foort_gpu_close(ctx);
@@ -689,6 +711,13 @@ int foort_gpu_open(const foort_run_desc *desc, foort_gpu_ctx **out)
     }
     *out = ctx;
     return FOORT_OK;
+}
+
+int foort_gpu_configure(foort_gpu_ctx *ctx, const foort_run_desc *desc)
+{
+    if (!ctx || !desc)
+        return fail(ctx, FOORT_EINVAL, "NULL context or descriptor");
+    return configure(ctx, *desc);
 }
 
 void foort_gpu_close(foort_gpu_ctx *ctx)

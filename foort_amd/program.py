@@ -32,6 +32,8 @@ EXPORTS = [
     "foort_config_run_desc", "foort_config_mesh_desc", "foort_config_first_line", "foort_config_diag_names",
     "foort_output_create", "foort_output_destroy", "foort_output_prepare", "foort_output_geodesic",
     "foort_output_finished", "foort_program_options_init", "foort_program_run",
+    "foort_server_config_init", "foort_server_create", "foort_server_submit", "foort_server_results",
+    "foort_server_drain", "foort_server_destroy",
 ]
 
 
@@ -56,6 +58,23 @@ class ProgramOptions(C.Structure):
         ("schedule", C.c_int32), ("screen_output_level", C.c_int32), ("tracer", TRACE_FN),
         ("tracer_user", C.c_void_p), ("stats", C.POINTER(Stats)),
     ]
+
+
+class Request(C.Structure):
+    """foort_request (include/foort_host.h)"""
+    _fields_ = [("desc", RunDesc), ("mesh", MeshDesc), ("tag", C.c_uint64)]
+
+
+class RequestResult(C.Structure):
+    """foort_request_result (include/foort_host.h)"""
+    _fields_ = [("tag", C.c_uint64), ("status", C.c_int32), ("iterations", C.c_uint32), ("pixels", C.c_uint64),
+                ("steps", C.c_uint64), ("submit_s", C.c_double), ("start_s", C.c_double), ("end_s", C.c_double),
+                ("kernel_ms", C.c_double), ("footprint_bytes", C.c_uint64)]
+
+
+class ServerConfig(C.Structure):
+    """foort_server_config (include/foort_host.h)"""
+    _fields_ = [("workers", C.c_uint32), ("reserved", C.c_uint32), ("hbm_budget_bytes", C.c_uint64)]
 
 
 class HostError(RuntimeError):
@@ -92,6 +111,12 @@ def lib():
         L.foort_output_geodesic.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.POINTER(C.c_char_p)]
         L.foort_output_finished.argtypes = [C.c_void_p]
         L.foort_program_options_init.argtypes = [C.POINTER(ProgramOptions)]
+        L.foort_server_config_init.argtypes = [C.POINTER(ServerConfig)]
+        L.foort_server_create.argtypes = [C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]
+        L.foort_server_submit.argtypes = [C.c_void_p, C.POINTER(Request)]
+        L.foort_server_results.argtypes = [C.c_void_p, C.POINTER(RequestResult), C.c_int, C.c_double]
+        L.foort_server_drain.argtypes = [C.c_void_p]
+        L.foort_server_destroy.argtypes = [C.c_void_p]
         L.foort_program_run.argtypes = [C.c_char_p, C.POINTER(ProgramOptions)]
         _lib = L
     return _lib
@@ -248,3 +273,46 @@ def run(cfg_path, output_prefix=None, christoffel=-1, precision=-1, schedule=-1,
     rc = lib().foort_program_run(cfg_path.encode(), C.byref(o))
     _check(rc)
     return st
+
+
+class Server:
+    """The render server (foort_server_*): submit requests, collect their results."""
+
+    def __init__(self, workers=0, hbm_budget_bytes=0):
+        cfg = ServerConfig()
+        lib().foort_server_config_init(C.byref(cfg))
+        if workers:
+            cfg.workers = workers
+        if hbm_budget_bytes:
+            cfg.hbm_budget_bytes = hbm_budget_bytes
+        self.config = cfg
+        self._h = C.c_void_p()
+        _check(lib().foort_server_create(C.byref(cfg), C.byref(self._h)))
+
+    def submit(self, desc, mesh, tag):
+        req = Request(desc=desc, mesh=mesh, tag=tag)
+        _check(lib().foort_server_submit(self._h, C.byref(req)))
+
+    def results(self, cap=4096, timeout=0.0):
+        arr = (RequestResult * cap)()
+        n = lib().foort_server_results(self._h, arr, cap, timeout)
+        if n < 0:
+            _check(n)
+        return [arr[i] for i in range(n)]
+
+    def drain(self):
+        _check(lib().foort_server_drain(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().foort_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        self.close()

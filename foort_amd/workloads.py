@@ -77,3 +77,44 @@ GPU = {{ Christoffel = "{christoffel}"; Precision = "{precision}"; Schedule = "{
         Autotune = "{autotune}"; UseGraph = {"true" if use_graph else "false"}; }};
 Output = {{ FilePrefix = "{prefix}"; FileExtension = "dat"; TimeStamp = false; ScreenOutputLevel = 2; }};
 """
+
+
+# C4: render-server request stream.
+C4_TEXT = ("C4: 2000 requests from seed {seed}, Poisson arrivals at {rate:g}/s; final image side log-uniform in "
+           "[64, 1024]; adaptive SquareSubdivisionMeshV2 of 2-6 iterations (MaxSubdivide = iterations, initial "
+           "side (side-1)/2^(it-1)+1, every pixel with differing neighbours subdivided each iteration); Kerr "
+           "a ~ U[0, 0.998] (80%), else Johannsen (a ~ U[0, 0.9], alpha13 ~ U[-0.5, 2]), Manko-Novikov "
+           "(a ~ U[0, 0.9], alpha3 ~ U[0, 5]) or Rasheed-Larsen (m = 1, a ~ U[0, 0.9], p = 2, q ~ U[2, 4]); "
+           "inclination ~ U[5, 90] deg; 30Mx30M screen, RLogScale on, mixed precision, RK4, horizon / boundary "
+           "r=1000M / time-out 1e5 steps, FourColorScreen (mesh value) + EquatorialPasses + ClosestRadius")
+
+
+def c4_requests(seed=2024, n=2000, rate=40.0):
+    """The C4 request stream: [(arrival time in s, RunDesc, MeshDesc, info dict)]."""
+    import numpy as np
+    from . import program as P
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for k in range(n):
+        t += rng.exponential(1.0 / rate)
+        side = int(round(math.exp(rng.uniform(math.log(64), math.log(1024)))))
+        iters = int(rng.integers(2, 7))
+        init_side = max(2, int(round((side - 1) / 2 ** (iters - 1))) + 1)
+        final_side = (init_side - 1) * 2 ** (iters - 1) + 1
+        inc = float(rng.uniform(5.0, 90.0))
+        kind = "kerr" if rng.uniform() < 0.8 else ["johannsen", "manko_novikov", "rasheed_larsen"][int(rng.integers(3))]
+        if kind == "kerr":
+            metric, params = A.METRIC_KERR, (float(rng.uniform(0.0, 0.998)),)
+        elif kind == "johannsen":
+            metric, params = A.METRIC_JOHANNSEN, (float(rng.uniform(0, 0.9)), float(rng.uniform(-0.5, 2.0)), 0, 0, 0)
+        elif kind == "manko_novikov":
+            metric, params = A.METRIC_MANKONOVIKOV, (float(rng.uniform(0, 0.9)), float(rng.uniform(0, 5.0)))
+        else:
+            metric, params = A.METRIC_RASHEEDLARSEN, (1.0, float(rng.uniform(0, 0.9)), 2.0, float(rng.uniform(2.0, 4.0)))
+        desc = c0(metric=metric, params=params, rlog=True, inclination_deg=inc,
+                  christoffel=A.CHRISTOFFEL_ANALYTIC, precision=A.PRECISION_MIXED)
+        mesh = P.mesh_desc(P.MESH_SQUARESUBDIVISIONV2, A.DIAGFLAG_FOURCOLORSCREEN, initial_pixels=init_side ** 2,
+                           max_pixels=final_side ** 2, iteration_pixels=final_side ** 2, max_subdivide=iters)
+        out.append((t, desc, mesh, {"kind": kind, "side": final_side, "iterations": iters}))
+    return out

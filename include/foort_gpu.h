@@ -13,7 +13,8 @@
  * Plain C types only; no exceptions cross the boundary; every entry point
  * returns FOORT_OK (0) or a negative FOORT_E* code and records a message that
  * foort_gpu_last_error() returns.  A context is used by one host thread at a
- * time; several contexts may run concurrently on different streams.
+ * time; several contexts may run concurrently: the host-pointer entry points run
+ * on the context's own stream.
  */
 #ifndef FOORT_GPU_H
 #define FOORT_GPU_H
@@ -177,6 +178,11 @@ int foort_desc_size(void);
  * of FOORT/Main.cpp:205-256. */
 int foort_gpu_open(const foort_run_desc *desc, foort_gpu_ctx **ctx);
 
+/* Give a context another run description (no counterpart in the reference): the
+ * constants and kernels change, its device buffers, stream and lane state stay allocated,
+ * so a server can move one context from request to request.  On failure (FOORT_EINVAL) the
+ * context keeps its previous description. */
+int foort_gpu_configure(foort_gpu_ctx *ctx, const foort_run_desc *desc);
 void foort_gpu_close(foort_gpu_ctx *ctx);
 
 /* Message of the last failure on ctx (or of the last failed open when ctx is NULL). */

@@ -128,6 +128,47 @@ void foort_program_options_init(foort_program_options *opts);
 /* Run the whole program on a configuration file: returns FOORT_OK after writing the output. */
 int foort_program_run(const char *cfg_path, const foort_program_options *opts);
 
+/* ---- render server (no counterpart in the reference) ----
+ * Requests -- each a run description and its mesh (a grid or an adaptive subdivision
+ * mesh) -- run concurrently on one GPU, `workers` at a time, each worker with its own GPU
+ * context and stream kept from request to request.  A request is admitted in arrival order
+ * once the device memory of its largest possible batch (36 bytes per pixel: indices and
+ * results) fits in hbm_budget_bytes next to the running requests (always when none runs). */
+typedef struct foort_request {
+    foort_run_desc desc;
+    foort_mesh_desc mesh;         /* pixel lists (InputCertainPixelsMesh) are not accepted */
+    uint64_t tag;                 /* the caller's identifier, returned with the result */
+} foort_request;
+
+typedef struct foort_request_result {
+    uint64_t tag;
+    int32_t status;               /* FOORT_OK or the failing call's code */
+    uint32_t iterations;          /* mesh iterations run */
+    uint64_t pixels, steps;       /* geodesics integrated and their steps */
+    double submit_s, start_s, end_s; /* seconds since the server was created */
+    double kernel_ms;             /* GPU time of the request's batches */
+    uint64_t footprint_bytes;     /* device memory it was admitted with */
+} foort_request_result;
+
+typedef struct foort_server_config {
+    uint32_t workers;             /* requests running at once (one stream each); default 4 */
+    uint32_t reserved;
+    uint64_t hbm_budget_bytes;    /* admission budget; default 4 GiB */
+} foort_server_config;
+
+typedef struct foort_server foort_server;
+
+void foort_server_config_init(foort_server_config *cfg);
+int foort_server_create(const foort_server_config *cfg, foort_server **srv);
+/* queue a request (returns at once; thread-safe) */
+int foort_server_submit(foort_server *srv, const foort_request *req);
+/* move up to cap finished results to out, waiting up to timeout_s for the first; returns their number */
+int foort_server_results(foort_server *srv, foort_request_result *out, int cap, double timeout_s);
+/* wait until every submitted request has finished */
+int foort_server_drain(foort_server *srv);
+/* finish the queued requests and stop the workers */
+void foort_server_destroy(foort_server *srv);
+
 #ifdef __cplusplus
 }
 #endif

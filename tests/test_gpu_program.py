@@ -99,3 +99,35 @@ def test_persistent_schedule_writes_the_same_files(tmp_path):
     P.run(str(cfg), output_prefix=str(tmp_path / "per" / "run"), schedule=1)
     one, per = _tree(tmp_path / "one"), _tree(tmp_path / "per")
     assert one == per
+
+
+def test_server_requests_match_direct_runs():
+    """The render server runs requests concurrently (4 workers, each with its own context and
+    stream, moved between requests); every request's pixel and step counts equal those of the
+    same adaptive mesh driven directly, one batch at a time, through a fresh context."""
+    import foort_amd as F
+    from foort_amd import workloads as W
+    reqs = W.c4_requests(seed=7, n=12, rate=1000.0)
+    reqs = [r for r in reqs if r[3]["side"] <= 300][:8]
+    print(f"[server] {len(reqs)} requests", flush=True)
+    with P.Server(workers=4, hbm_budget_bytes=64 << 20) as srv:
+        for k, (_, desc, mesh, _) in enumerate(reqs):
+            srv.submit(desc, mesh, k)
+        srv.drain()
+        got = {r.tag: r for r in srv.results()}
+    assert sorted(got) == list(range(len(reqs)))
+    for k, (_, desc, mesh, info) in enumerate(reqs):
+        r = got[k]
+        assert r.status == 0 and 2 <= r.iterations <= info["iterations"]
+        m = P.Mesh(mesh)
+        pixels = steps = 0
+        with F.Context(desc) as ctx:
+            while not m.is_finished:
+                si = m.screen_indices()
+                out = ctx.trace_pixels(m.side, si)
+                m.finished(out["quadrant"].astype(np.float64))
+                m.end_loop()
+                pixels += len(si)
+                steps += int(out["stats"].steps)
+        assert (r.pixels, r.steps) == (pixels, steps), (k, info)
+        assert r.end_s >= r.start_s >= r.submit_s
