@@ -251,7 +251,8 @@ def run_c4(args):
     from foort_amd import workloads as W
     n = args.requests
     reqs = W.c4_requests(seed=args.seed, n=n, rate=args.rate)
-    with P.Server(workers=args.workers, hbm_budget_bytes=args.hbm_budget << 20) as srv:
+    with P.Server(workers=args.workers, hbm_budget_bytes=args.hbm_budget << 20,
+                  launch_us=args.launch_us or 2000) as srv:
         t0 = time.perf_counter()
         for k, (t, desc, mesh, _) in enumerate(reqs):
             wait = t - (time.perf_counter() - t0)
@@ -272,12 +273,20 @@ def run_c4(args):
             "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
             "data": "synthetic",
             "config": {"workload": W.C4_TEXT.format(seed=args.seed, rate=args.rate), "workers": args.workers,
-                       "hbm_budget_bytes": args.hbm_budget << 20},
+                       "ring_bytes_per_group": args.hbm_budget << 20, "launch_us": args.launch_us or 2000},
             "latency_p50_s": float(np.quantile(lat, 0.5)), "latency_p99_s": float(np.quantile(lat, 0.99)),
             "latency_max_s": float(lat.max()), "queue_wait_p99_s": float(np.quantile(wait, 0.99)),
             "offered_rate_per_s": args.rate, "completed_rate_per_s": n / span,
             "pixels_per_s": pixels / span, "pixels": int(pixels), "geodesic_steps": int(sum(r.steps for r in res)),
-            "gpu_kernel_s_summed_over_streams": kernel_s, "span_s": span}
+            "gpu_kernel_s_shared_out": kernel_s, "span_s": span}
+    by_kind = {}
+    for r in res:
+        by_kind.setdefault(reqs[r.tag][3]["kind"], []).append(r)
+    line["by_kind"] = {
+        k: {"requests": len(v), "latency_p50_s": float(np.quantile([r.end_s - r.submit_s for r in v], 0.5)),
+            "latency_p99_s": float(np.quantile([r.end_s - r.submit_s for r in v], 0.99)),
+            "steps_per_pixel": sum(r.steps for r in v) / max(1, sum(r.pixels for r in v))}
+        for k, v in sorted(by_kind.items())}
     print(json.dumps(line), flush=True)
 
 
@@ -289,8 +298,8 @@ def main():
     ap.add_argument("--requests", type=int, default=2000, help="c4: requests in the stream")
     ap.add_argument("--rate", type=float, default=40.0, help="c4: offered request rate (per s)")
     ap.add_argument("--seed", type=int, default=2024, help="c4: seed of the request stream")
-    ap.add_argument("--workers", type=int, default=4, help="c4: requests served at once")
-    ap.add_argument("--hbm-budget", type=int, default=4096, help="c4: admission budget in MiB")
+    ap.add_argument("--workers", type=int, default=4, help="c4: host threads for mesh work")
+    ap.add_argument("--hbm-budget", type=int, default=4096, help="c4: device ring size per request group in MiB")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)

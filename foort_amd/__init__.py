@@ -23,6 +23,9 @@ EXPORTS = [
     "foort_gpu_trace_grid", "foort_gpu_trace_pixels", "foort_gpu_trace_device", "foort_gpu_camera",
     "foort_gpu_device_alloc", "foort_gpu_device_free", "foort_gpu_copy_to_device", "foort_gpu_copy_to_host",
     "foort_gpu_synchronize", "foort_gpu_autotune", "foort_gpu_configure",
+    "foort_gpu_queue_open", "foort_gpu_queue_close", "foort_gpu_queue_last_error", "foort_gpu_queue_add",
+    "foort_gpu_queue_remove", "foort_gpu_queue_push", "foort_gpu_queue_run", "foort_gpu_queue_progress",
+    "foort_gpu_queue_results",
 ]
 
 
@@ -56,6 +59,17 @@ def lib():
         L.foort_gpu_device_free.argtypes = [C.c_void_p]
         L.foort_gpu_copy_to_device.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
         L.foort_gpu_copy_to_host.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+        L.foort_gpu_queue_open.argtypes = [C.POINTER(RunDesc), C.c_uint64, C.c_uint32, C.POINTER(C.c_void_p)]
+        L.foort_gpu_queue_close.argtypes = [C.c_void_p]
+        L.foort_gpu_queue_last_error.restype = C.c_char_p
+        L.foort_gpu_queue_last_error.argtypes = [C.c_void_p]
+        L.foort_gpu_queue_add.argtypes = [C.c_void_p, C.POINTER(RunDesc), C.c_uint64, C.POINTER(C.c_uint32)]
+        L.foort_gpu_queue_remove.argtypes = [C.c_void_p, C.c_uint32]
+        L.foort_gpu_queue_push.argtypes = [C.c_void_p, C.c_uint32, C.c_uint64, C.c_void_p, C.POINTER(C.c_uint64)]
+        L.foort_gpu_queue_run.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(Stats)]
+        L.foort_gpu_queue_progress.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64),
+                                               C.POINTER(C.c_uint64)]
+        L.foort_gpu_queue_results.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(GeodesicResults)]
         if L.foort_desc_size() != C.sizeof(RunDesc):
             raise FoortError(f"foort_run_desc layout mismatch: library {L.foort_desc_size()} bytes, "
                              f"binding {C.sizeof(RunDesc)}")
@@ -171,6 +185,78 @@ class Context:
         self._check(self._L.foort_gpu_camera(self._h, V.ctypes.data_as(C.c_void_p), g0.ctypes.data_as(C.c_void_p)),
                     "foort_gpu_camera")
         return V.reshape(4, 4), g0
+
+
+class Queue:
+    """Request queue (foort_gpu_queue_*): many requests of one group -- run descriptions
+    that differ only in metric parameters, camera and screen -- share one device ring of
+    pixels and one persistent kernel."""
+
+    def __init__(self, group_desc, capacity=1 << 20, max_requests=256):
+        L = lib()
+        self._L = L
+        self.max_requests = max_requests
+        h = C.c_void_p()
+        rc = L.foort_gpu_queue_open(C.byref(group_desc), capacity, max_requests, C.byref(h))
+        if rc != 0:
+            raise FoortError(f"foort_gpu_queue_open failed ({rc}): {L.foort_gpu_queue_last_error(None).decode()}")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.foort_gpu_queue_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise FoortError(f"{what} failed ({rc}): {self._L.foort_gpu_queue_last_error(self._h).decode()}")
+
+    def add(self, desc, side):
+        slot = C.c_uint32()
+        self._check(self._L.foort_gpu_queue_add(self._h, C.byref(desc), side, C.byref(slot)), "foort_gpu_queue_add")
+        return slot.value
+
+    def remove(self, slot):
+        self._check(self._L.foort_gpu_queue_remove(self._h, slot), "foort_gpu_queue_remove")
+
+    def push(self, slot, screen_index):
+        si = np.ascontiguousarray(screen_index, dtype=np.uint64).reshape(-1, 2)
+        first = C.c_uint64()
+        self._check(self._L.foort_gpu_queue_push(self._h, slot, si.shape[0], si.ctypes.data_as(C.c_void_p),
+                                                 C.byref(first)), "foort_gpu_queue_push")
+        return first.value, si.shape[0]
+
+    def run(self, launch_us=0):
+        st = Stats()
+        self._check(self._L.foort_gpu_queue_run(self._h, launch_us, C.byref(st)), "foort_gpu_queue_run")
+        return st
+
+    def progress(self):
+        """(per-slot [done, steps] array, pixels pushed, pixels done)."""
+        ds = np.zeros((self.max_requests, 2), np.uint64)
+        pushed, fin = C.c_uint64(), C.c_uint64()
+        self._check(self._L.foort_gpu_queue_progress(self._h, ds.ctypes.data_as(C.c_void_p), C.byref(pushed),
+                                                     C.byref(fin)), "foort_gpu_queue_progress")
+        return ds, pushed.value, fin.value
+
+    def results(self, first, n):
+        out = {"term": np.zeros(n, np.int32), "quadrant": np.zeros(n, np.int32),
+               "equat_passes": np.zeros(n, np.int32), "closest_radius": np.zeros(n, np.float64)}
+        res = _results_struct(out, lambda a: a.ctypes.data_as(C.c_void_p))
+        self._check(self._L.foort_gpu_queue_results(self._h, first, n, C.byref(res)), "foort_gpu_queue_results")
+        return out
 
 
 class DeviceArray:

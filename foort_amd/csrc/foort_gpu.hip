@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -46,6 +47,14 @@ __global__ void __launch_bounds__(64) zero_counters(unsigned long long *p, int n
         p[threadIdx.x] = 0;
 }
 
+// After a queue launch: lanes that found the ring empty advanced the claim counter past
+// the pixels pushed so far without taking any; later pushes start at `count`.
+__global__ void __launch_bounds__(64) clamp_claims(unsigned long long *next, unsigned long long count)
+{
+    if (threadIdx.x == 0 && *next > count)
+        *next = count;
+}
+
 thread_local std::string g_open_error;
 
 uint32_t clamp_u32(uint64_t v)
@@ -79,6 +88,10 @@ bool init_metric(const foort_run_desc &d, foort::MetricParams &M, std::string &e
         M.m = mP / ((pP + qP) / 4.0);
         M.p = pP / ((pP + qP) / 4.0);
         M.q = qP / ((pP + qP) / 4.0);
+        // the constant square roots of FOORT/Metric.cpp:288-298 (and :333-343), in this unit's
+        // contraction-free arithmetic
+        M.rl_root_pq = std::sqrt(M.p * M.q);
+        M.rl_root_pm = std::sqrt((M.p * M.p - 4. * M.m * M.m) * (M.q * M.q - 4. * M.m * M.m));
         M.horizon = (mP + std::sqrt(mP * mP - aP * aP)) / ((pP + qP) / 4.0);
         break;
     }
@@ -541,34 +554,50 @@ int trace_host(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, uint64_t count
     return FOORT_OK;
 }
 
-// Validate a run description and derive what the kernels take (the run constants and
-// the kernel pair).  Nothing of ctx changes unless the description is valid.
-int configure(foort_gpu_ctx *ctx, const foort_run_desc &d)
+// Validate a run description and derive what the kernels take: the run constants
+// and the kernels.  Returns FOORT_OK, or FOORT_EINVAL with the reason in err.
+int build_run(const foort_run_desc &d, RunConst &R, KernelPair &kernels, std::string &err)
 {
-    if (d.integrator != FOORT_INTEGRATOR_RK4 && d.integrator != FOORT_INTEGRATOR_VERLET)
-        return fail(ctx, FOORT_EINVAL, "unknown integrator");
-    if (d.christoffel != FOORT_CHRISTOFFEL_FD && d.christoffel != FOORT_CHRISTOFFEL_ANALYTIC)
-        return fail(ctx, FOORT_EINVAL, "unknown Christoffel policy");
-    if (d.precision != FOORT_PRECISION_FP64 && d.precision != FOORT_PRECISION_MIXED)
-        return fail(ctx, FOORT_EINVAL, "unknown precision");
-    if (d.precision == FOORT_PRECISION_MIXED && d.christoffel != FOORT_CHRISTOFFEL_ANALYTIC)
-        return fail(ctx, FOORT_EINVAL, "mixed precision needs the analytic Christoffel policy");
-    if (d.schedule != FOORT_SCHEDULE_ONESHOT && d.schedule != FOORT_SCHEDULE_PERSISTENT)
-        return fail(ctx, FOORT_EINVAL, "unknown schedule");
-    if (d.diag_flags & FOORT_DIAGFLAG_GEODESICPOSITION)
-        return fail(ctx, FOORT_EINVAL, "GeodesicPosition is not supported by the GPU path");
-    if (!(d.derivative_h > 0))
-        return fail(ctx, FOORT_EINVAL, "DerivativeH must be positive");
-    if (d.step_cap > 0x7fffffffull)
-        return fail(ctx, FOORT_EINVAL, "step_cap above 2^31-1");
-    if (!(d.camera_position[1] > 0))
-        return fail(ctx, FOORT_EINVAL, "camera radius must be positive");
+    if (d.integrator != FOORT_INTEGRATOR_RK4 && d.integrator != FOORT_INTEGRATOR_VERLET) {
+        err = "unknown integrator";
+        return FOORT_EINVAL;
+    }
+    if (d.christoffel != FOORT_CHRISTOFFEL_FD && d.christoffel != FOORT_CHRISTOFFEL_ANALYTIC) {
+        err = "unknown Christoffel policy";
+        return FOORT_EINVAL;
+    }
+    if (d.precision != FOORT_PRECISION_FP64 && d.precision != FOORT_PRECISION_MIXED) {
+        err = "unknown precision";
+        return FOORT_EINVAL;
+    }
+    if (d.precision == FOORT_PRECISION_MIXED && d.christoffel != FOORT_CHRISTOFFEL_ANALYTIC) {
+        err = "mixed precision needs the analytic Christoffel policy";
+        return FOORT_EINVAL;
+    }
+    if (d.schedule != FOORT_SCHEDULE_ONESHOT && d.schedule != FOORT_SCHEDULE_PERSISTENT) {
+        err = "unknown schedule";
+        return FOORT_EINVAL;
+    }
+    if (d.diag_flags & FOORT_DIAGFLAG_GEODESICPOSITION) {
+        err = "GeodesicPosition is not supported by the GPU path";
+        return FOORT_EINVAL;
+    }
+    if (!(d.derivative_h > 0)) {
+        err = "DerivativeH must be positive";
+        return FOORT_EINVAL;
+    }
+    if (d.step_cap > 0x7fffffffull) {
+        err = "step_cap above 2^31-1";
+        return FOORT_EINVAL;
+    }
+    if (!(d.camera_position[1] > 0)) {
+        err = "camera radius must be positive";
+        return FOORT_EINVAL;
+    }
 
-    RunConst R;
     std::memset(&R, 0, sizeof(R));
-    std::string err;
     if (!init_metric(d, R.M, err))
-        return fail(ctx, FOORT_EINVAL, err);
+        return FOORT_EINVAL;
     R.eps = d.step_size;
     R.minstep = d.smallest_step;
     R.h_fd = d.derivative_h;
@@ -614,9 +643,20 @@ int configure(foort_gpu_ctx *ctx, const foort_run_desc &d)
                        term_every(FOORT_TERMFLAG_TIMEOUT, R.t_n) && term_every(FOORT_TERMFLAG_THETASINGULARITY, R.th_n) &&
                        (!(R.diags & FOORT_DIAGFLAG_EQUATORIALPASSES) || R.ep_n == 1) &&
                        (!(R.diags & FOORT_DIAGFLAG_CLOSESTRADIUS) || R.cr_n == 1);
-    const KernelPair kernels = d.christoffel == FOORT_CHRISTOFFEL_FD
+    kernels = d.christoffel == FOORT_CHRISTOFFEL_FD
                        ? foort::pick_policy<foort::RhsFD>(d.metric, d.integrator, every)
                        : foort::pick_analytic(d.metric, d.integrator, static_cast<int>(d.precision), every);
+    return FOORT_OK;
+}
+
+// Validate and apply a run description.  Nothing of ctx changes unless it is valid.
+int configure(foort_gpu_ctx *ctx, const foort_run_desc &d)
+{
+    RunConst R;
+    KernelPair kernels{};
+    std::string err;
+    if (build_run(d, R, kernels, err) != FOORT_OK)
+        return fail(ctx, FOORT_EINVAL, err);
     if (ctx->kernels.persistent != kernels.persistent && ctx->d_slots) {
         // the lane count follows the persistent kernel's occupancy: size the slots again
         (void)hipFree(ctx->d_slots);
@@ -630,7 +670,45 @@ int configure(foort_gpu_ctx *ctx, const foort_run_desc &d)
     return FOORT_OK;
 }
 
+// What must be equal for requests to share a queue: everything the step loop
+// reads except the metric parameters (the camera and screen only start geodesics).
+std::string group_key(const foort_run_desc &d, const RunConst &R, const KernelPair &k)
+{
+    std::string key;
+    auto add = [&key](const void *p, size_t n) { key.append(static_cast<const char *>(p), n); };
+    add(&d.metric, sizeof d.metric);
+    add(&k.queue, sizeof k.queue);
+    const double dv[] = {R.eps, R.minstep, R.h_fd, R.inv2h, R.verlet_tol, R.h_eps, R.sphere_R, R.theta_eps,
+                         R.ep_thr, R.mixed_u, R.mixed_pole};
+    add(dv, sizeof dv);
+    const uint32_t uv[] = {R.terms, R.diags, R.h_n, R.s_n, R.t_n, R.th_n, R.max_steps, R.step_cap, R.ep_n, R.cr_n,
+                           static_cast<uint32_t>(R.ep_start), static_cast<uint32_t>(R.ep_finish),
+                           static_cast<uint32_t>(R.cr_start), static_cast<uint32_t>(R.cr_finish),
+                           static_cast<uint32_t>(R.M.rlog), static_cast<uint32_t>(R.M.has_horizon)};
+    add(uv, sizeof uv);
+    return key;
+}
+
 } // namespace
+
+struct foort_gpu_queue {
+    foort_gpu_ctx *ctx = nullptr; // the group's kernels, stream, counters and lane state
+    std::string key;
+    uint64_t cap = 0, mask = 0;   // ring capacity (a power of two)
+    uint64_t count = 0;           // pixels pushed so far (a monotonic position)
+    std::map<uint64_t, uint64_t> unread; // pushed ranges whose results were not read yet: first -> count
+    uint32_t max_req = 0;
+    std::vector<char> used;
+    RunConst *d_reqc = nullptr;
+    uint64_t *d_req_side = nullptr;
+    uint32_t *d_req_of = nullptr;
+    uint64_t *d_si = nullptr;
+    unsigned long long *d_req_done = nullptr;
+    int32_t *d_term = nullptr, *d_quad = nullptr, *d_pass = nullptr;
+    double *d_close = nullptr;
+    void *d_block = nullptr;      // one allocation holding all of the above
+    bool lanes_reset = false;
+};
 
 extern "C" {
 
@@ -888,6 +966,244 @@ int foort_gpu_autotune(foort_gpu_ctx *ctx, uint64_t side, uint64_t count, const 
         *chosen = ctx->desc;
     if (best)
         *best = top;
+    return FOORT_OK;
+}
+
+int foort_gpu_queue_open(const foort_run_desc *group, uint64_t capacity, uint32_t max_requests,
+                         foort_gpu_queue **out)
+{
+    if (!group || !out)
+        return fail(nullptr, FOORT_EINVAL, "NULL argument");
+    *out = nullptr;
+    if (capacity < 256 || capacity > (uint64_t(1) << 32) || max_requests == 0 || max_requests > (1u << 20))
+        return fail(nullptr, FOORT_EINVAL, "queue capacity must be 256 .. 2^32 pixels and 1 .. 2^20 requests");
+    foort_run_desc d = *group;
+    d.schedule = FOORT_SCHEDULE_PERSISTENT;
+    d.steps_per_launch = 0;
+    d.use_graph = 0;
+    foort_gpu_ctx *ctx = nullptr;
+    int rc = foort_gpu_open(&d, &ctx);
+    if (rc)
+        return rc;
+    auto *q = new foort_gpu_queue();
+    q->ctx = ctx;
+    q->key = group_key(d, ctx->rc, ctx->kernels);
+    uint64_t cap = 256;
+    while (cap < capacity)
+        cap *= 2;
+    q->cap = cap;
+    q->mask = cap - 1;
+    q->max_req = max_requests;
+    q->used.assign(max_requests, 0);
+    auto align = [](size_t v) { return (v + 255) & ~size_t(255); };
+    size_t off = 0, o_reqc = off;
+    off = align(off + max_requests * sizeof(RunConst));
+    size_t o_side = off;
+    off = align(off + max_requests * 8);
+    size_t o_done = off;
+    off = align(off + max_requests * 16);
+    size_t o_of = off;
+    off = align(off + cap * 4);
+    size_t o_si = off;
+    off = align(off + cap * 16);
+    size_t o_term = off;
+    off = align(off + cap * 4);
+    size_t o_quad = off;
+    off = align(off + cap * 4);
+    size_t o_pass = off;
+    off = align(off + cap * 4);
+    size_t o_close = off;
+    off = align(off + cap * 8);
+    hipError_t e = hipMalloc(&q->d_block, off);
+    if (e != hipSuccess) {
+        foort_gpu_queue_close(q);
+        return fail(nullptr, FOORT_ENOMEM, std::string("hipMalloc of the queue: ") + hipGetErrorString(e));
+    }
+    char *b = static_cast<char *>(q->d_block);
+    q->d_reqc = reinterpret_cast<RunConst *>(b + o_reqc);
+    q->d_req_side = reinterpret_cast<uint64_t *>(b + o_side);
+    q->d_req_done = reinterpret_cast<unsigned long long *>(b + o_done);
+    q->d_req_of = reinterpret_cast<uint32_t *>(b + o_of);
+    q->d_si = reinterpret_cast<uint64_t *>(b + o_si);
+    q->d_term = reinterpret_cast<int32_t *>(b + o_term);
+    q->d_quad = reinterpret_cast<int32_t *>(b + o_quad);
+    q->d_pass = reinterpret_cast<int32_t *>(b + o_pass);
+    q->d_close = reinterpret_cast<double *>(b + o_close);
+    *out = q;
+    return FOORT_OK;
+}
+
+void foort_gpu_queue_close(foort_gpu_queue *q)
+{
+    if (!q)
+        return;
+    if (q->d_block)
+        (void)hipFree(q->d_block);
+    foort_gpu_close(q->ctx);
+    delete q;
+}
+
+const char *foort_gpu_queue_last_error(const foort_gpu_queue *q)
+{
+    return q ? q->ctx->err.c_str() : g_open_error.c_str();
+}
+
+int foort_gpu_queue_add(foort_gpu_queue *q, const foort_run_desc *desc, uint64_t side, uint32_t *slot)
+{
+    if (!q || !desc || !slot)
+        return fail(nullptr, FOORT_EINVAL, "NULL argument");
+    if (side < 2)
+        return fail(q->ctx, FOORT_EINVAL, "row_column_size must be at least 2");
+    RunConst R;
+    KernelPair k{};
+    std::string err;
+    if (build_run(*desc, R, k, err) != FOORT_OK)
+        return fail(q->ctx, FOORT_EINVAL, err);
+    if (group_key(*desc, R, k) != q->key)
+        return fail(q->ctx, FOORT_EINVAL, "request does not match the queue's group (only the metric parameters, "
+                                          "the camera and the screen may differ)");
+    uint32_t s = 0;
+    while (s < q->max_req && q->used[s])
+        ++s;
+    if (s == q->max_req)
+        return fail(q->ctx, FOORT_ENOMEM, "no free request slot");
+    const unsigned long long zero[2] = {0, 0};
+    hipStream_t st = q->ctx->stream;
+    HIP_TRY(q->ctx, hipMemcpyAsync(q->d_reqc + s, &R, sizeof R, hipMemcpyHostToDevice, st));
+    HIP_TRY(q->ctx, hipMemcpyAsync(q->d_req_side + s, &side, 8, hipMemcpyHostToDevice, st));
+    HIP_TRY(q->ctx, hipMemcpyAsync(q->d_req_done + 2 * s, zero, 16, hipMemcpyHostToDevice, st));
+    HIP_TRY(q->ctx, hipStreamSynchronize(st));
+    q->used[s] = 1;
+    *slot = s;
+    return FOORT_OK;
+}
+
+int foort_gpu_queue_remove(foort_gpu_queue *q, uint32_t slot)
+{
+    if (!q || slot >= q->max_req || !q->used[slot])
+        return fail(q ? q->ctx : nullptr, FOORT_EINVAL, "no such request slot");
+    q->used[slot] = 0;
+    return FOORT_OK;
+}
+
+int foort_gpu_queue_push(foort_gpu_queue *q, uint32_t slot, uint64_t n, const uint64_t *screen_index,
+                         uint64_t *first)
+{
+    if (!q || (n && !screen_index) || !first)
+        return fail(q ? q->ctx : nullptr, FOORT_EINVAL, "NULL argument");
+    if (slot >= q->max_req || !q->used[slot])
+        return fail(q->ctx, FOORT_EINVAL, "no such request slot");
+    const uint64_t tail = q->unread.empty() ? q->count : q->unread.begin()->first;
+    if (q->count + n - tail > q->cap)
+        return fail(q->ctx, FOORT_ENOMEM, "queue full: read the results of earlier batches first");
+    *first = q->count;
+    if (n == 0)
+        return FOORT_OK;
+    std::vector<uint32_t> of(n, slot);
+    hipStream_t st = q->ctx->stream;
+    // the ring may wrap: at most two contiguous pieces
+    uint64_t done = 0;
+    while (done < n) {
+        const uint64_t at = (q->count + done) & q->mask;
+        const uint64_t len = std::min(n - done, q->cap - at);
+        HIP_TRY(q->ctx, hipMemcpyAsync(q->d_si + 2 * at, screen_index + 2 * done, len * 16, hipMemcpyHostToDevice, st));
+        HIP_TRY(q->ctx, hipMemcpyAsync(q->d_req_of + at, of.data() + done, len * 4, hipMemcpyHostToDevice, st));
+        done += len;
+    }
+    HIP_TRY(q->ctx, hipStreamSynchronize(st));
+    q->unread[q->count] = n;
+    q->count += n;
+    return FOORT_OK;
+}
+
+int foort_gpu_queue_run(foort_gpu_queue *q, uint32_t launch_us, foort_stats *stats)
+{
+    if (!q)
+        return fail(nullptr, FOORT_EINVAL, "NULL queue");
+    foort_gpu_ctx *ctx = q->ctx;
+    ctx->desc.launch_us = launch_us;
+    int rc = setup_persistent(ctx);
+    if (rc)
+        return rc;
+    const LaneSlots &S = ctx->slots;
+    hipStream_t st = ctx->stream;
+    if (!q->lanes_reset) {
+        hipLaunchKernelGGL(reset_lanes, dim3(S.L / 256), dim3(256), 0, st, S, ctx->d_steps);
+        HIP_TRY(ctx, hipGetLastError());
+        q->lanes_reset = true;
+    }
+    hipLaunchKernelGGL(zero_counters, dim3(1), dim3(64), 0, st, ctx->d_steps, 1);
+    HIP_TRY(ctx, hipGetLastError());
+    foort::QueueArgs Q{q->d_reqc, q->d_req_side, q->d_req_of, q->d_si, q->d_req_done, q->count, q->mask};
+    DevResults d{q->d_term, q->d_quad, q->d_pass, q->d_close, nullptr, nullptr, nullptr, nullptr};
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
+    hipLaunchKernelGGL(ctx->kernels.queue, dim3(S.L / 256), dim3(256), 0, st, ctx->rc, Q, d, ctx->d_steps, S,
+                       UINT64_MAX);
+    HIP_TRY(ctx, hipGetLastError());
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
+    hipLaunchKernelGGL(clamp_claims, dim3(1), dim3(64), 0, st, S.next, static_cast<unsigned long long>(q->count));
+    HIP_TRY(ctx, hipGetLastError());
+    unsigned long long c[3] = {0, 0, 0};
+    HIP_TRY(ctx, hipMemcpyAsync(c, ctx->d_counters, sizeof c, hipMemcpyDeviceToHost, st));
+    HIP_TRY(ctx, hipStreamSynchronize(st));
+    if (stats) {
+        float ms = 0;
+        HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        stats->geodesics = c[2];
+        stats->steps = c[0];
+        stats->launches = 1;
+        stats->device_ms = ms;
+    }
+    return FOORT_OK;
+}
+
+int foort_gpu_queue_progress(foort_gpu_queue *q, uint64_t *done_steps, uint64_t *pushed, uint64_t *finished)
+{
+    if (!q)
+        return fail(nullptr, FOORT_EINVAL, "NULL queue");
+    foort_gpu_ctx *ctx = q->ctx;
+    hipStream_t st = ctx->stream;
+    unsigned long long c[3] = {0, 0, 0};
+    if (done_steps)
+        HIP_TRY(ctx, hipMemcpyAsync(done_steps, q->d_req_done, q->max_req * 16, hipMemcpyDeviceToHost, st));
+    HIP_TRY(ctx, hipMemcpyAsync(c, ctx->d_counters, sizeof c, hipMemcpyDeviceToHost, st));
+    HIP_TRY(ctx, hipStreamSynchronize(st));
+    if (pushed)
+        *pushed = q->count;
+    if (finished)
+        *finished = q->lanes_reset ? c[2] : 0;
+    return FOORT_OK;
+}
+
+int foort_gpu_queue_results(foort_gpu_queue *q, uint64_t first, uint64_t n, const foort_geodesic_results *out)
+{
+    if (!q || !out)
+        return fail(q ? q->ctx : nullptr, FOORT_EINVAL, "NULL argument");
+    auto it = q->unread.find(first);
+    if (it == q->unread.end() || it->second != n)
+        return fail(q->ctx, FOORT_EINVAL, "not a pushed batch whose results are unread");
+    if (out->final_pos || out->final_vel || out->lambda || out->steps)
+        return fail(q->ctx, FOORT_EINVAL, "a queue keeps termination and the three diagnostics only");
+    hipStream_t st = q->ctx->stream;
+    struct F {
+        void *host;
+        const void *dev;
+        size_t size;
+    } f[4] = {{out->term, q->d_term, 4}, {out->quadrant, q->d_quad, 4}, {out->equat_passes, q->d_pass, 4},
+              {out->closest_radius, q->d_close, 8}};
+    uint64_t done = 0;
+    while (done < n) {
+        const uint64_t at = (first + done) & q->mask;
+        const uint64_t len = std::min(n - done, q->cap - at);
+        for (const F &x : f)
+            if (x.host)
+                HIP_TRY(q->ctx, hipMemcpyAsync(static_cast<char *>(x.host) + done * x.size,
+                                               static_cast<const char *>(x.dev) + at * x.size, len * x.size,
+                                               hipMemcpyDeviceToHost, st));
+        done += len;
+    }
+    HIP_TRY(q->ctx, hipStreamSynchronize(st));
+    q->unread.erase(it);
     return FOORT_OK;
 }
 

@@ -16,8 +16,23 @@
 #ifndef FOORT_WPE_PERSIST
 #define FOORT_WPE_PERSIST 2
 #endif
+// Manko-Novikov and Rasheed-Larsen: their potentials with derivatives need more
+// registers than two waves per SIMD leave, and spills to scratch memory stall a
+// lone wave (the time-outs that end an image) on every step.  Measured on MI355X:
+// one wave per SIMD halves the time of Manko-Novikov steps and takes about a sixth
+// off Rasheed-Larsen's; Johannsen does as well with two.
+#ifndef FOORT_WPE_HEAVY
+#define FOORT_WPE_HEAVY 1
+#endif
 
 namespace foort {
+
+template <class RHS>
+constexpr int waves_per_eu(int wpe)
+{
+    return (RHS::metric == FOORT_METRIC_MANKONOVIKOV || RHS::metric == FOORT_METRIC_RASHEEDLARSEN) ? FOORT_WPE_HEAVY
+                                                                                                  : wpe;
+}
 
 struct DevResults {
     int32_t *term;
@@ -33,7 +48,7 @@ struct DevResults {
 // Per-lane state of the persistent schedule, structure of arrays over L lanes,
 // kept in HBM between launches.
 constexpr int kSlotD = 11; // pos[4] vel[4] lambda prevtheta closest
-constexpr int kSlotU = 11; // term steps hc bc tc thc tsteps epc crc quadrant passes
+constexpr int kSlotU = 12; // term steps hc bc tc thc tsteps epc crc quadrant passes request
 struct LaneSlots {
     double *d;                 // kSlotD * L
     uint32_t *u;               // kSlotU * L
@@ -44,6 +59,23 @@ struct LaneSlots {
     uint32_t refill;           // refill idle lanes once at least this many of a wavefront are idle
     uint32_t interval;         // steps between refills (the length of the inner step loop)
     uint64_t ticks;            // wall-clock ticks after which a launch ends (0 = no time limit)
+};
+
+// A ring of pixels from many requests that share one kernel and every run
+// constant except the metric parameters and the camera (a render server's
+// queue): request slot r has its own run constants reqc[r] (camera, screen,
+// metric parameters; the rest equal to the group's) and grid side req_side[r];
+// pixel k (a monotonic counter; storage index k & mask) belongs to request
+// req_of[k & mask] and has screen index si[2 (k & mask)], si[2 (k & mask) + 1].
+// req_done[2 r] counts the finished pixels of request r, req_done[2 r + 1] their steps.
+struct QueueArgs {
+    const RunConst *reqc;
+    const uint64_t *req_side;
+    const uint32_t *req_of;
+    const uint64_t *si;
+    unsigned long long *req_done;
+    uint64_t count;            // pixels queued so far
+    uint64_t mask;             // ring capacity - 1 (a power of two)
 };
 
 __device__ __forceinline__ void store_result(const DevResults &out, uint64_t i, const GeoState &g)
@@ -102,7 +134,7 @@ __device__ __forceinline__ void add_steps(unsigned long long *total, unsigned lo
 // One lane per pixel, integrated to termination: the drop-in for the
 // per-thread loop of FOORT/Main.cpp:373-400.
 template <class RHS, int INTEG, bool EVERY>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FOORT_WPE_ONESHOT, FOORT_WPE_ONESHOT)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(waves_per_eu<RHS>(FOORT_WPE_ONESHOT), waves_per_eu<RHS>(FOORT_WPE_ONESHOT))))
 trace_oneshot(const RunConst R, uint64_t side, uint64_t first, uint64_t count,
                                                      const uint64_t *__restrict__ si, DevResults out,
                                                      unsigned long long *__restrict__ total_steps, LaneSlots,
@@ -182,7 +214,7 @@ __device__ __forceinline__ void slot_save(const LaneSlots &S, uint32_t l, const 
 // leaves as soon as none of its lanes has work.  The host relaunches until `done` reaches the
 // batch size.  The steps counted are those of the geodesics that finish.
 template <class RHS, int INTEG, bool EVERY>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FOORT_WPE_PERSIST, FOORT_WPE_PERSIST)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(waves_per_eu<RHS>(FOORT_WPE_PERSIST), waves_per_eu<RHS>(FOORT_WPE_PERSIST))))
 trace_persistent(const RunConst R, uint64_t side, uint64_t first,
                                                         uint64_t count, const uint64_t *__restrict__ si,
                                                         DevResults out, unsigned long long *__restrict__ total_steps,
@@ -246,12 +278,94 @@ trace_persistent(const RunConst R, uint64_t side, uint64_t first,
     add_steps(total_steps, mine);
 }
 
+// Persistent schedule over a request queue (QueueArgs): as trace_persistent, with
+// each lane carrying its pixel's request -- the camera when the geodesic starts,
+// the metric parameters on every step -- and counting finished pixels per request.
+// Results go to storage index k & mask of the ring.
+template <class RHS, int INTEG, bool EVERY>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(waves_per_eu<RHS>(FOORT_WPE_PERSIST), waves_per_eu<RHS>(FOORT_WPE_PERSIST))))
+trace_queue(const RunConst R, const QueueArgs Q, DevResults out, unsigned long long *__restrict__ total_steps,
+            LaneSlots S, uint64_t budget)
+{
+    const uint32_t l = blockIdx.x * blockDim.x + threadIdx.x; // grid covers exactly S.L lanes
+    const uint32_t lane_id = threadIdx.x & 63;
+    GeoState g;
+    MetricParams M = R.M;
+    uint32_t req = 0;
+    long long pix = S.pix[l];
+    if (pix >= 0) {
+        slot_load(S, l, g);
+        req = S.u[11 * S.L + l];
+        M = Q.reqc[req].M;
+    } else {
+        g.term = -1; // no geodesic: the step loop below skips the lane
+    }
+    bool exhausted = false;
+    unsigned long long mine = 0;
+    uint64_t it = 0;
+    const uint64_t t0 = S.ticks ? wall_clock64() : 0;
+    for (;;) {
+        const unsigned long long need = __ballot(pix < 0 && !exhausted);
+        if (need && (static_cast<uint32_t>(__popcll(need)) >= S.refill || !__any(pix >= 0))) {
+            const int leader = __ffsll(static_cast<long long>(need)) - 1;
+            unsigned long long base = 0;
+            if (static_cast<int>(lane_id) == leader)
+                base = atomicAdd(S.next, static_cast<unsigned long long>(__popcll(need)));
+            base = __shfl(base, leader);
+            if (pix < 0 && !exhausted) {
+                const unsigned long long k = base + __popcll(need & ((1ull << lane_id) - 1ull));
+                if (k < Q.count) {
+                    pix = static_cast<long long>(k);
+                    const uint64_t at = k & Q.mask;
+                    req = Q.req_of[at];
+                    const RunConst &Rq = Q.reqc[req];
+                    M = Rq.M;
+                    double ux, uy;
+                    pixel_unit(Q.req_side[req], 0, Q.si, at, ux, uy);
+                    geo_init<EVERY>(Rq, ux, uy, g);
+                } else {
+                    exhausted = true;
+                }
+            }
+        }
+        if (!__any(pix >= 0) || it >= budget ||
+            (S.ticks && static_cast<uint64_t>(wall_clock64()) - t0 >= S.ticks))
+            break;
+        const uint32_t n = budget - it < S.interval ? static_cast<uint32_t>(budget - it) : S.interval;
+        RunConst Rl = R;
+        Rl.M = M;
+        for (uint32_t k = 0; k < n && g.term == FOORT_TERM_CONTINUE; ++k)
+            geo_update<RHS, INTEG, EVERY>(Rl, g);
+        it += n;
+        const bool finished = pix >= 0 && g.term != FOORT_TERM_CONTINUE;
+        if (finished) {
+            store_result(out, static_cast<uint64_t>(pix) & Q.mask, g);
+            atomicAdd(&Q.req_done[2 * req], 1ull);
+            atomicAdd(&Q.req_done[2 * req + 1], static_cast<unsigned long long>(g.steps));
+            mine += g.steps;
+            pix = -1;
+        }
+        const unsigned long long fin = __ballot(finished);
+        if (fin && static_cast<int>(lane_id) == __ffsll(static_cast<long long>(fin)) - 1)
+            atomicAdd(S.done, static_cast<unsigned long long>(__popcll(fin)));
+    }
+    S.pix[l] = pix;
+    if (pix >= 0) {
+        slot_save(S, l, g);
+        S.u[11 * S.L + l] = req;
+    }
+    add_steps(total_steps, mine);
+}
+
 using KernelFn = void (*)(const RunConst, uint64_t, uint64_t, uint64_t, const uint64_t *, DevResults,
                           unsigned long long *, LaneSlots, uint64_t);
+using QueueKernelFn = void (*)(const RunConst, const QueueArgs, DevResults, unsigned long long *, LaneSlots,
+                               uint64_t);
 
 struct KernelPair {
     KernelFn oneshot;
     KernelFn persistent;
+    QueueKernelFn queue;
 };
 
 // `every`: every termination and diagnostic updates on every step (the kernels
@@ -259,8 +373,10 @@ struct KernelPair {
 template <class RHS, int INTEG>
 KernelPair kernel_pair(bool every)
 {
-    return every ? KernelPair{trace_oneshot<RHS, INTEG, true>, trace_persistent<RHS, INTEG, true>}
-                 : KernelPair{trace_oneshot<RHS, INTEG, false>, trace_persistent<RHS, INTEG, false>};
+    return every ? KernelPair{trace_oneshot<RHS, INTEG, true>, trace_persistent<RHS, INTEG, true>,
+                              trace_queue<RHS, INTEG, true>}
+                 : KernelPair{trace_oneshot<RHS, INTEG, false>, trace_persistent<RHS, INTEG, false>,
+                              trace_queue<RHS, INTEG, false>};
 }
 
 template <template <int> class POLICY, int M>

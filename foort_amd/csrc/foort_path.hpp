@@ -105,6 +105,8 @@ FM_HD_INLINE D2<S> fpow8(const D2<S> &a)
 struct MetricParams {
     double a;               // Kerr / Johannsen / MN spin, RL rescaled a
     double m, p, q;         // Rasheed-Larsen (rescaled by (p+q)/4)
+    double rl_root_pq;      // sqrt(p q)                        (computed once on the host, where no
+    double rl_root_pm;      // sqrt((p^2 - 4m^2) (q^2 - 4m^2))   multiply-add can make p = 2m negative)
     double a13, a22, a52, eps3;  // Johannsen
     double alpha3, mn_alpha, mn_k;  // Manko-Novikov
     double horizon;         // SphericalHorizonMetric::m_HorizonRadius
@@ -216,12 +218,12 @@ FM_HD_INLINE void rl_common(const MetricParams &M, T u, T sint, T cost, RLCommon
     T cost2 = cost * cost;
     T delta = r2 + a * a - 2. * r * m;
     T H3 = r2 - 2. * r * m + a * a * cost2;
-    T Bp = __builtin_sqrt(pp * qq) * a * sint2 *
+    T Bp = M.rl_root_pq * a * sint2 *
                 (((pp * qq + 4. * m * m) * r - m * (pp - 2. * m) * (qq - 2. * m)) / (2. * m * (pp + qq) * H3));
     T H1 = r2 + a * a * cost2 + r * (pp - 2. * m) + (pp / (pp + qq)) * ((pp - 2. * m) * (qq - 2. * m) / 2.) -
-                (pp / (2. * m * (pp + qq))) * __builtin_sqrt((pp * pp - 4. * m * m) * (qq * qq - 4. * m * m)) * a * cost;
+                (pp / (2. * m * (pp + qq))) * M.rl_root_pm * a * cost;
     T H2 = r2 + a * a * cost2 + r * (qq - 2. * m) + (qq / (pp + qq)) * ((pp - 2. * m) * (qq - 2. * m) / 2.) +
-                (qq / (2. * m * (pp + qq))) * __builtin_sqrt((pp * pp - 4. * m * m) * (qq * qq - 4. * m * m)) * a * cost;
+                (qq / (2. * m * (pp + qq))) * M.rl_root_pm * a * cost;
     c.r = r; c.sint2 = sint2; c.delta = delta; c.H3 = H3; c.Bp = Bp; c.H1 = H1; c.H2 = H2;
     c.sqH1H2 = fsqrt(H1 * H2);
 }
@@ -338,10 +340,8 @@ FM_HD_INLINE void mn_common(const MetricParams &M, T u, T sint, T cost, MNCommon
 }
 
 template <class T>
-FM_HD_INLINE void mn_dd(const MetricParams &M, T u, T sint, T cost, T *g)
+FM_HD_INLINE void mn_dd_from(const MetricParams &M, const MNCommon<T> &c, T sint, T *g)
 {
-    MNCommon<T> c;
-    mn_common(M, u, sint, cost, c);
     T g00 = -c.f;
     T g22 = c.exp_2gamma * c.rho_sq / c.f;
     T g11 = g22 / c.delta;
@@ -353,10 +353,16 @@ FM_HD_INLINE void mn_dd(const MetricParams &M, T u, T sint, T cost, T *g)
 }
 
 template <class T>
-FM_HD_INLINE void mn_uu(const MetricParams &M, T u, T sint, T cost, T *g)
+FM_HD_INLINE void mn_dd(const MetricParams &M, T u, T sint, T cost, T *g)
 {
     MNCommon<T> c;
     mn_common(M, u, sint, cost, c);
+    mn_dd_from(M, c, sint, g);
+}
+
+template <class T>
+FM_HD_INLINE void mn_uu_from(const MetricParams &M, const MNCommon<T> &c, T sint, T *g)
+{
     T beta_d = c.omega * c.f;
     T gamma_dd = -c.f * c.omega * c.omega + c.delta * sint * sint / c.f;
     T beta_u = beta_d / gamma_dd;
@@ -369,6 +375,14 @@ FM_HD_INLINE void mn_uu(const MetricParams &M, T u, T sint, T cost, T *g)
     if (M.rlog)
         g11 *= 1.0 / (c.r * c.r);
     g[0] = g00; g[1] = g03; g[2] = g11; g[3] = g22; g[4] = g33;
+}
+
+template <class T>
+FM_HD_INLINE void mn_uu(const MetricParams &M, T u, T sint, T cost, T *g)
+{
+    MNCommon<T> c;
+    mn_common(M, u, sint, cost, c);
+    mn_uu_from(M, c, sint, g);
 }
 
 template <int METRIC, class T>
@@ -555,15 +569,28 @@ FM_HD_INLINE void rhs_an(const RunConst &R, const T *p, const T *v, T *out)
     ssincos(p[2], &s, &c);
     const D2<S> U(static_cast<S>(p[1]), S(1), S(0));
     const D2<S> ST(s, S(0), c), CT(c, S(0), -s);
-    D2<S> g[5], gu[5];
-    metric_dd<METRIC, D2<S>>(R.M, U, ST, CT, g);
-    metric_uu<METRIC, D2<S>>(R.M, U, ST, CT, gu);
+    D2<S> g[5];
     S d1[5], d2[5], uu[5], vv[4], o[4];
+    if constexpr (METRIC == FOORT_METRIC_MANKONOVIKOV) {
+        // g_dd and g^uu share one evaluation of the Manko-Novikov potentials
+        // (their most expensive part); g^uu needs only its values
+        MNCommon<D2<S>> cd;
+        mn_common(R.M, U, ST, CT, cd);
+        mn_dd_from(R.M, cd, ST, g);
+        const MNCommon<S> cv{cd.r.v, cd.sint.v, cd.f.v, cd.exp_2gamma.v, cd.omega.v, cd.rho_sq.v, cd.delta.v};
+        mn_uu_from(R.M, cv, s, uu);
+    } else {
+        D2<S> gu[5];
+        metric_dd<METRIC, D2<S>>(R.M, U, ST, CT, g);
+        metric_uu<METRIC, D2<S>>(R.M, U, ST, CT, gu);
+#pragma unroll
+        for (int k = 0; k < 5; ++k)
+            uu[k] = gu[k].v;
+    }
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
         d1[k] = g[k].du;
         d2[k] = g[k].dt;
-        uu[k] = gu[k].v;
     }
 #pragma unroll
     for (int k = 0; k < 4; ++k)

@@ -1,20 +1,25 @@
 // server.cpp -- render server (no counterpart in the reference): requests of
 // any metric, camera and resolution, each an adaptive mesh of a few
-// iterations, run concurrently on one GPU.
+// iterations, run concurrently on one GPU by continuous batching.
 //
-// A fixed set of workers each owns a GPU context (its stream, device staging
-// buffers and persistent-lane state stay allocated from request to request;
-// foort_gpu_configure switches the run description) and runs one request at a
-// time, iteration by iteration: the mesh hands out its pixels, the GPU
-// integrates them on the worker's stream, the value diagnostic goes back to the
-// mesh.  Requests are admitted in arrival order once the device memory they
-// may need -- their largest possible batch of pixel indices and results --
-// fits in the budget next to the requests already running.
+// Requests whose run descriptions differ only in the metric parameters, the
+// camera and the screen form a group; each group has one device queue
+// (foort_gpu_queue_*): a ring of pixels that one persistent kernel drains, its
+// lanes refilled from whichever request pushed the next pixel.  A group thread
+// owns the queue: it pushes the iteration batches that are ready, runs
+// time-sliced launches on the queue's stream, and collects the batches that are
+// done.  Mesh work -- building a request's mesh, feeding it an iteration's
+// values, queueing the next iteration -- runs on a pool of host threads, so
+// the GPU keeps integrating other requests' pixels meanwhile.  Each request has
+// at most one batch on the device: its mesh needs those values before it can
+// choose the next pixels.
 #include "host.hpp"
 
 #include <chrono>
 #include <condition_variable>
 #include <deque>
+#include <map>
+#include <memory>
 #include <mutex>
 #include <thread>
 
@@ -23,31 +28,55 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
-// device bytes per pixel of a batch: screen index (16) + termination, quadrant,
-// passes (4 each) + closest radius (8)
-constexpr uint64_t kBytesPerPixel = 16 + 4 + 4 + 4 + 8;
+// device bytes per ring pixel: screen index (16), request slot (4), termination,
+// quadrant, passes (4 each), closest radius (8)
+constexpr uint64_t kBytesPerPixel = 16 + 4 + 4 + 4 + 4 + 8;
+constexpr uint32_t kSlotsPerQueue = 1024;
 
-uint64_t max_batch_pixels(const foort_mesh_desc &m)
+// The fields a queue's group shares (the queue checks the same on the device).
+std::string group_key(const foort_run_desc &d)
 {
-    switch (m.type) {
-    case FOORT_MESH_SIMPLESQUARE: return m.total_pixels;
-    case FOORT_MESH_INPUTCERTAINPIXELS: return m.pixel_count;
-    default: {
-        // the initial grid, or an iteration's queue: at most 5 new pixels per subdivided
-        // square, and never more than MaxPixels in all (0 = no limit)
-        const uint64_t per_iter = m.iteration_pixels > (uint64_t(1) << 40) ? (uint64_t(1) << 40) : 5 * m.iteration_pixels;
-        uint64_t most = std::max<uint64_t>(m.initial_pixels, per_iter);
-        if (m.max_pixels)
-            most = std::min<uint64_t>(most, std::max(m.max_pixels, m.initial_pixels));
-        return most;
-    }
-    }
+    std::string k;
+    auto add = [&k](const auto &v) { k.append(reinterpret_cast<const char *>(&v), sizeof v); };
+    add(d.metric), add(d.rlog_scale), add(d.integrator), add(d.christoffel), add(d.step_size);
+    add(d.smallest_step), add(d.derivative_h), add(d.verlet_tolerance), add(d.term_flags), add(d.precision);
+    add(d.horizon_epsilon), add(d.horizon_update_n), add(d.sphere_radius), add(d.sphere_update_n);
+    add(d.timeout_max_steps), add(d.timeout_update_n), add(d.theta_epsilon), add(d.theta_update_n);
+    add(d.diag_flags), add(d.ep_threshold), add(d.ep_update.update_n_steps), add(d.ep_update.update_start);
+    add(d.ep_update.update_finish), add(d.cr_update.update_n_steps), add(d.cr_update.update_start);
+    add(d.cr_update.update_finish), add(d.step_cap), add(d.lanes_per_cu), add(d.refill_lanes);
+    add(d.refill_interval), add(d.mixed_switch_radius), add(d.mixed_pole_margin);
+    return k;
 }
 
-struct Job {
-    foort_request req;
+struct Group;
+
+struct Req {
+    foort_request req{};
     foort_request_result res{};
-    uint64_t need = 0;
+    Group *group = nullptr;
+    std::unique_ptr<Mesh> mesh;
+    uint32_t vd = 0;
+    bool finished = false;         // the mesh is done (or the request failed)
+    uint32_t slot = UINT32_MAX;
+    std::vector<uint64_t> idx;     // the batch to push: (row, column) pairs
+    uint64_t n = 0, first = 0;     // its pixels and, once pushed, its ring position
+    uint64_t pushed = 0;           // pixels pushed for the slot since it was added
+    uint64_t steps_seen = 0;       // the slot's step count at the last look
+    std::vector<int32_t> term, quad, passes;
+    std::vector<double> closest;
+};
+
+struct Group {
+    std::string key;
+    foort_run_desc desc{};
+    foort_gpu_queue *q = nullptr;
+    uint64_t capacity = 0;
+    std::thread thr;
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<Req *> events;      // requests with a batch ready, or finished
+    bool stop = false;
 };
 
 } // namespace
@@ -59,100 +88,231 @@ struct foort_server {
     foort_server_config cfg{};
     Clock::time_point t0 = Clock::now();
     std::mutex mu;
-    std::condition_variable cv_work, cv_done;
-    std::deque<Job> pending;             // arrival order
+    std::condition_variable cv_task, cv_done;
+    std::deque<std::pair<Req *, bool>> tasks; // (request, true: build its mesh; false: take its values)
     std::deque<foort_request_result> done;
-    uint64_t in_flight = 0;              // admitted bytes
-    uint64_t running = 0, submitted = 0, finished = 0;
-    bool stopping = false;
-    std::vector<std::thread> workers;
+    uint64_t submitted = 0, finished = 0;
+    bool stopping = false;  // no new requests
+    bool pool_stop = false; // every request finished: the pool may end
+    std::vector<std::thread> pool;
+    std::map<std::string, std::unique_ptr<Group>> groups;
 
     double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
 
-    void run_job(Job &j, foort_gpu_ctx *&ctx)
+    void post_task(Req *r, bool build)
     {
-        foort_request_result &r = j.res;
-        r.start_s = now();
-        int rc = FOORT_OK;
-        if (!ctx)
-            rc = foort_gpu_open(&j.req.desc, &ctx);
-        else
-            rc = foort_gpu_configure(ctx, &j.req.desc);
-        if (rc != FOORT_OK) {
-            r.status = rc;
-            return;
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            tasks.emplace_back(r, build);
         }
+        cv_task.notify_one();
+    }
+
+    static void post_event(Req *r)
+    {
+        Group *g = r->group;
+        {
+            std::lock_guard<std::mutex> lk(g->mu);
+            g->events.push_back(r);
+        }
+        g->cv.notify_one();
+    }
+
+    void complete(Req *r)
+    {
+        r->res.end_s = now();
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            done.push_back(r->res);
+            ++finished;
+        }
+        cv_done.notify_all();
+        delete r;
+    }
+
+    // ---- host pool: mesh work ----
+    // Take the last batch's values (unless `build`), then move the mesh on to its next
+    // non-empty iteration and hand the batch to the group thread (or finish).
+    void mesh_step(Req *r, bool build)
+    {
         try {
-            std::unique_ptr<Mesh> mesh = make_mesh(j.req.mesh);
-            const uint32_t vd = diagnostic_order(j.req.desc.diag_flags, j.req.desc.value_diag).front();
-            std::vector<uint64_t> idx;
-            std::vector<int32_t> term, quad, passes;
-            std::vector<double> closest, value;
-            while (!mesh->IsFinished()) {
-                const uint64_t n = mesh->getCurNrGeodesics();
-                idx.resize(2 * n);
-                term.resize(n), quad.resize(n), passes.resize(n), closest.resize(n), value.resize(n);
-                mesh->screenIndices(idx.data());
-                foort_geodesic_results out{term.data(), quad.data(), passes.data(), closest.data(),
-                                           nullptr,     nullptr,     nullptr,       nullptr};
-                foort_stats st{};
-                if (n) {
-                    rc = foort_gpu_trace_pixels(ctx, mesh->rowColumnSize(), n, idx.data(), &out, &st);
-                    if (rc != FOORT_OK)
-                        break;
+            if (build) {
+                r->res.start_s = now();
+                r->mesh = make_mesh(r->req.mesh);
+                r->vd = diagnostic_order(r->req.desc.diag_flags, r->req.desc.value_diag).front();
+            } else {
+                for (uint64_t i = 0; i < r->n; ++i)
+                    r->mesh->GeodesicFinished(i, r->vd == FOORT_DIAGFLAG_FOURCOLORSCREEN   ? r->quad[i]
+                                                 : r->vd == FOORT_DIAGFLAG_EQUATORIALPASSES ? r->passes[i]
+                                                                                            : r->closest[i]);
+                r->mesh->EndCurrentLoop();
+                r->res.pixels += r->n;
+                ++r->res.iterations;
+            }
+            for (;;) {
+                if (r->mesh->IsFinished()) {
+                    r->finished = true;
+                    break;
                 }
-                for (uint64_t i = 0; i < n; ++i)
-                    mesh->GeodesicFinished(i, vd == FOORT_DIAGFLAG_FOURCOLORSCREEN   ? quad[i]
-                                              : vd == FOORT_DIAGFLAG_EQUATORIALPASSES ? passes[i]
-                                                                                      : closest[i]);
-                mesh->EndCurrentLoop();
-                r.pixels += n;
-                r.steps += st.steps;
-                r.kernel_ms += st.device_ms;
-                ++r.iterations;
+                r->n = r->mesh->getCurNrGeodesics();
+                if (r->n) {
+                    r->idx.resize(2 * r->n);
+                    r->mesh->screenIndices(r->idx.data());
+                    r->res.footprint_bytes = std::max(r->res.footprint_bytes, r->n * kBytesPerPixel);
+                    break;
+                }
+                r->mesh->EndCurrentLoop(); // an empty iteration
+                ++r->res.iterations;
             }
         } catch (const std::exception &e) {
             set_error(e.what());
-            rc = FOORT_EINVAL;
+            r->res.status = FOORT_EINVAL;
+            r->finished = true;
         }
-        r.status = rc;
+        if (r->finished && r->slot == UINT32_MAX)
+            complete(r); // the group thread holds nothing of it
+        else
+            post_event(r);
     }
 
-    void worker()
+    void pool_thread()
     {
-        foort_gpu_ctx *ctx = nullptr;
         for (;;) {
-            Job j;
+            std::pair<Req *, bool> t;
             {
                 std::unique_lock<std::mutex> lk(mu);
-                // admit the oldest request once its memory fits (always when nothing runs)
-                cv_work.wait(lk, [&] {
-                    return stopping || (!pending.empty() &&
-                                        (running == 0 || in_flight + pending.front().need <= cfg.hbm_budget_bytes));
-                });
-                if (pending.empty() && stopping)
-                    break;
-                if (pending.empty())
-                    continue;
-                j = pending.front();
-                pending.pop_front();
-                in_flight += j.need;
-                ++running;
+                cv_task.wait(lk, [&] { return pool_stop || !tasks.empty(); });
+                if (tasks.empty())
+                    return;
+                t = tasks.front();
+                tasks.pop_front();
             }
-            cv_work.notify_all();
-            run_job(j, ctx);
-            j.res.end_s = now();
-            {
-                std::lock_guard<std::mutex> lk(mu);
-                in_flight -= j.need;
-                --running;
-                ++finished;
-                done.push_back(j.res);
-            }
-            cv_work.notify_all();
-            cv_done.notify_all();
+            mesh_step(t.first, t.second);
         }
-        foort_gpu_close(ctx);
+    }
+
+    // ---- group thread: the device queue ----
+    void fail_request(Group &g, Req *r, int status)
+    {
+        r->res.status = status;
+        if (r->slot != UINT32_MAX && g.q)
+            (void)foort_gpu_queue_remove(g.q, r->slot);
+        r->slot = UINT32_MAX;
+        complete(r);
+    }
+
+    void group_thread(Group &g)
+    {
+        int broken = foort_gpu_queue_open(&g.desc, g.capacity, kSlotsPerQueue, &g.q);
+        if (broken)
+            set_error(std::string("render server: ") + foort_gpu_queue_last_error(nullptr));
+        std::deque<Req *> waiting;          // batches not yet pushed, in arrival order
+        std::vector<Req *> on_device;       // one batch each on the device
+        std::vector<uint64_t> done_steps(2 * kSlotsPerQueue);
+        for (;;) {
+            std::deque<Req *> ev;
+            {
+                std::unique_lock<std::mutex> lk(g.mu);
+                if (on_device.empty())
+                    g.cv.wait(lk, [&] { return g.stop || !g.events.empty(); });
+                ev.swap(g.events);
+                if (g.stop && ev.empty() && waiting.empty() && on_device.empty())
+                    break;
+            }
+            for (Req *r : ev) {
+                if (r->finished) {
+                    if (r->slot != UINT32_MAX)
+                        (void)foort_gpu_queue_remove(g.q, r->slot);
+                    r->slot = UINT32_MAX;
+                    complete(r);
+                } else {
+                    waiting.push_back(r);
+                }
+            }
+            if (broken) {
+                for (Req *r : waiting)
+                    fail_request(g, r, broken);
+                waiting.clear();
+                for (Req *r : on_device)
+                    fail_request(g, r, broken);
+                on_device.clear();
+                continue;
+            }
+            // push ready batches in arrival order while slots and ring room last
+            while (!waiting.empty()) {
+                Req *r = waiting.front();
+                if (r->n > g.capacity) {
+                    waiting.pop_front();
+                    set_error("render server: an iteration batch exceeds the ring");
+                    fail_request(g, r, FOORT_ENOMEM);
+                    continue;
+                }
+                if (r->slot == UINT32_MAX) {
+                    uint32_t s = 0;
+                    const int rc = foort_gpu_queue_add(g.q, &r->req.desc, r->mesh->rowColumnSize(), &s);
+                    if (rc == FOORT_ENOMEM)
+                        break; // every slot taken: wait for a request to finish
+                    if (rc) {
+                        waiting.pop_front();
+                        set_error(std::string("render server: ") + foort_gpu_queue_last_error(g.q));
+                        fail_request(g, r, rc);
+                        continue;
+                    }
+                    r->slot = s;
+                    r->pushed = r->steps_seen = 0;
+                }
+                const int rc = foort_gpu_queue_push(g.q, r->slot, r->n, r->idx.data(), &r->first);
+                if (rc == FOORT_ENOMEM)
+                    break; // the ring is full: wait for batches to finish
+                waiting.pop_front();
+                if (rc) {
+                    fail_request(g, r, rc);
+                    continue;
+                }
+                r->pushed += r->n;
+                on_device.push_back(r);
+            }
+            if (on_device.empty())
+                continue;
+            foort_stats st{};
+            uint64_t pushed = 0, fin = 0;
+            int rc = foort_gpu_queue_run(g.q, cfg.launch_us, &st);
+            if (!rc)
+                rc = foort_gpu_queue_progress(g.q, done_steps.data(), &pushed, &fin);
+            if (rc) {
+                set_error(std::string("render server: ") + foort_gpu_queue_last_error(g.q));
+                broken = rc;
+                continue;
+            }
+            // the launch's time, shared out by the steps each request finished in it
+            uint64_t total = 0;
+            for (Req *r : on_device)
+                total += done_steps[2 * r->slot + 1] - r->steps_seen;
+            for (size_t i = 0; i < on_device.size();) {
+                Req *r = on_device[i];
+                const uint64_t s = done_steps[2 * r->slot + 1];
+                if (total)
+                    r->res.kernel_ms += st.device_ms * double(s - r->steps_seen) / double(total);
+                r->res.steps += s - r->steps_seen;
+                r->steps_seen = s;
+                if (done_steps[2 * r->slot] < r->pushed) {
+                    ++i;
+                    continue;
+                }
+                r->term.resize(r->n), r->quad.resize(r->n), r->passes.resize(r->n), r->closest.resize(r->n);
+                foort_geodesic_results out{r->term.data(), r->quad.data(), r->passes.data(), r->closest.data(),
+                                           nullptr,        nullptr,        nullptr,          nullptr};
+                rc = foort_gpu_queue_results(g.q, r->first, r->n, &out);
+                on_device[i] = on_device.back();
+                on_device.pop_back();
+                if (rc) {
+                    fail_request(g, r, rc);
+                    continue;
+                }
+                post_task(r, false);
+            }
+        }
+        foort_gpu_queue_close(g.q);
     }
 };
 
@@ -163,6 +323,7 @@ void foort_server_config_init(foort_server_config *cfg)
     if (!cfg)
         return;
     cfg->workers = 4;
+    cfg->launch_us = 2000;
     cfg->hbm_budget_bytes = uint64_t(4) << 30;
 }
 
@@ -177,16 +338,18 @@ int foort_server_create(const foort_server_config *cfg, foort_server **srv)
     if (cfg) {
         if (cfg->workers)
             s->cfg.workers = cfg->workers;
+        if (cfg->launch_us)
+            s->cfg.launch_us = cfg->launch_us;
         if (cfg->hbm_budget_bytes)
             s->cfg.hbm_budget_bytes = cfg->hbm_budget_bytes;
     }
-    if (s->cfg.workers > 64) {
+    if (s->cfg.workers > 256 || s->cfg.hbm_budget_bytes < 256 * kBytesPerPixel) {
         delete s;
-        set_error("at most 64 workers");
+        set_error("at most 256 workers and a budget of at least 256 pixels");
         return FOORT_EINVAL;
     }
     for (uint32_t w = 0; w < s->cfg.workers; ++w)
-        s->workers.emplace_back([s] { s->worker(); });
+        s->pool.emplace_back([s] { s->pool_thread(); });
     *srv = s;
     return FOORT_OK;
 }
@@ -197,28 +360,41 @@ int foort_server_submit(foort_server *srv, const foort_request *req)
         set_error("NULL argument");
         return FOORT_EINVAL;
     }
-    Job j;
-    j.req = *req;
-    j.req.mesh.pixels = nullptr; // a request's mesh is adaptive or a grid; pixel lists are not kept
-    j.req.mesh.pixel_count = 0;
-    if (j.req.mesh.type == FOORT_MESH_INPUTCERTAINPIXELS) {
+    if (req->mesh.type == FOORT_MESH_INPUTCERTAINPIXELS) {
         set_error("the server takes grid and subdivision meshes");
         return FOORT_EINVAL;
     }
-    j.need = max_batch_pixels(j.req.mesh) * kBytesPerPixel;
-    j.res.tag = req->tag;
-    j.res.footprint_bytes = j.need;
+    auto *r = new Req();
+    r->req = *req;
+    r->req.mesh.pixels = nullptr; // a request's mesh is adaptive or a grid; pixel lists are not kept
+    r->req.mesh.pixel_count = 0;
+    r->res.tag = req->tag;
     {
         std::lock_guard<std::mutex> lk(srv->mu);
         if (srv->stopping) {
+            delete r;
             set_error("server is shutting down");
             return FOORT_EINVAL;
         }
-        j.res.submit_s = srv->now();
-        srv->pending.push_back(j);
+        r->res.submit_s = srv->now();
+        const std::string key = group_key(req->desc);
+        std::unique_ptr<Group> &g = srv->groups[key];
+        if (!g) {
+            g = std::make_unique<Group>();
+            g->key = key;
+            g->desc = req->desc;
+            uint64_t cap = 256;
+            while (cap * 2 * kBytesPerPixel <= srv->cfg.hbm_budget_bytes && cap < (uint64_t(1) << 32))
+                cap *= 2;
+            g->capacity = cap;
+            Group *gp = g.get();
+            g->thr = std::thread([srv, gp] { srv->group_thread(*gp); });
+        }
+        r->group = g.get();
         ++srv->submitted;
+        srv->tasks.emplace_back(r, true);
     }
-    srv->cv_work.notify_all();
+    srv->cv_task.notify_one();
     return FOORT_OK;
 }
 
@@ -256,11 +432,25 @@ void foort_server_destroy(foort_server *srv)
         return;
     {
         std::lock_guard<std::mutex> lk(srv->mu);
-        srv->stopping = true;
+        srv->stopping = true; // no new requests; those submitted still finish
     }
-    srv->cv_work.notify_all();
-    for (auto &t : srv->workers)
+    foort_server_drain(srv);
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        srv->pool_stop = true;
+    }
+    srv->cv_task.notify_all();
+    for (auto &t : srv->pool)
         t.join();
+    for (auto &kv : srv->groups) {
+        Group &g = *kv.second;
+        {
+            std::lock_guard<std::mutex> lk(g.mu);
+            g.stop = true;
+        }
+        g.cv.notify_all();
+        g.thr.join();
+    }
     delete srv;
 }
 

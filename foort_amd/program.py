@@ -74,7 +74,7 @@ class RequestResult(C.Structure):
 
 class ServerConfig(C.Structure):
     """foort_server_config (include/foort_host.h)"""
-    _fields_ = [("workers", C.c_uint32), ("reserved", C.c_uint32), ("hbm_budget_bytes", C.c_uint64)]
+    _fields_ = [("workers", C.c_uint32), ("launch_us", C.c_uint32), ("hbm_budget_bytes", C.c_uint64)]
 
 
 class HostError(RuntimeError):
@@ -278,11 +278,13 @@ def run(cfg_path, output_prefix=None, christoffel=-1, precision=-1, schedule=-1,
 class Server:
     """The render server (foort_server_*): submit requests, collect their results."""
 
-    def __init__(self, workers=0, hbm_budget_bytes=0):
+    def __init__(self, workers=0, hbm_budget_bytes=0, launch_us=0):
         cfg = ServerConfig()
         lib().foort_server_config_init(C.byref(cfg))
         if workers:
             cfg.workers = workers
+        if launch_us:
+            cfg.launch_us = launch_us
         if hbm_budget_bytes:
             cfg.hbm_budget_bytes = hbm_budget_bytes
         self.config = cfg

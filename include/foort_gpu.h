@@ -222,6 +222,46 @@ int foort_gpu_autotune(foort_gpu_ctx *ctx, uint64_t row_column_size, uint64_t co
                        const uint64_t *screen_index, uint32_t flags, foort_run_desc *chosen,
                        foort_stats *best);
 
+/* Request queue (no counterpart in the reference; for the render server): one device-resident
+ * ring of pixels shared by many requests whose run descriptions differ only in the metric
+ * parameters, the camera and the screen (the group).  Each pixel carries its request slot;
+ * a persistent kernel refills its lanes from the ring, so lanes of every request stay busy
+ * together and short requests finish while long ones still run.
+ *   open:     group = a descriptor of the group (its metric, integrator, precision, Christoffel
+ *             method, terminations and diagnostics); capacity = ring size in pixels (rounded up
+ *             to a power of two); max_requests = request slots.
+ *   add:      register a request (its own descriptor, which must match the group, and its
+ *             screen side); *slot receives its slot.  remove frees the slot: only once every
+ *             pixel pushed for it is done.
+ *   push:     append n (row, column) pairs of screen_index (host memory) for a slot; *first
+ *             receives the ring position of the first.  Fails with FOORT_ENOMEM when results
+ *             of earlier batches still occupy the room the batch needs.
+ *   run:      one launch over the ring, for at most launch_us microseconds of wall clock
+ *             (0: until every pushed pixel is done); lanes keep unfinished geodesics between
+ *             runs.  stats->steps counts the steps of the geodesics finished by this run.
+ *   progress: per slot, pixels done and their steps (done_steps[2*slot], [2*slot+1], for
+ *             max_requests slots, since the slot was added); pixels pushed and done in all.
+ *   results:  copy a pushed batch's results (term, quadrant, equat_passes, closest_radius;
+ *             final state pointers must be NULL) to host memory and free its room; only after
+ *             progress shows the batch done.
+ * Host memory in and out; every call is synchronous on the queue's own stream.  One thread
+ * at a time may use a queue. */
+typedef struct foort_gpu_queue foort_gpu_queue;
+int foort_gpu_queue_open(const foort_run_desc *group, uint64_t capacity, uint32_t max_requests,
+                         foort_gpu_queue **out);
+void foort_gpu_queue_close(foort_gpu_queue *q);
+const char *foort_gpu_queue_last_error(const foort_gpu_queue *q);
+int foort_gpu_queue_add(foort_gpu_queue *q, const foort_run_desc *desc, uint64_t row_column_size,
+                        uint32_t *slot);
+int foort_gpu_queue_remove(foort_gpu_queue *q, uint32_t slot);
+int foort_gpu_queue_push(foort_gpu_queue *q, uint32_t slot, uint64_t n, const uint64_t *screen_index,
+                         uint64_t *first);
+int foort_gpu_queue_run(foort_gpu_queue *q, uint32_t launch_us, foort_stats *stats);
+int foort_gpu_queue_progress(foort_gpu_queue *q, uint64_t *done_steps, uint64_t *pushed,
+                             uint64_t *finished);
+int foort_gpu_queue_results(foort_gpu_queue *q, uint64_t first, uint64_t n,
+                            const foort_geodesic_results *out);
+
 /* Device memory helpers for callers that keep batches resident on the GPU
  * (no counterpart in the reference): allocate, free, copy, and wait for all
  * work queued on the device. */

@@ -130,10 +130,15 @@ int foort_program_run(const char *cfg_path, const foort_program_options *opts);
 
 /* ---- render server (no counterpart in the reference) ----
  * Requests -- each a run description and its mesh (a grid or an adaptive subdivision
- * mesh) -- run concurrently on one GPU, `workers` at a time, each worker with its own GPU
- * context and stream kept from request to request.  A request is admitted in arrival order
- * once the device memory of its largest possible batch (36 bytes per pixel: indices and
- * results) fits in hbm_budget_bytes next to the running requests (always when none runs). */
+ * mesh) -- run concurrently on one GPU by continuous batching.  Requests whose descriptions
+ * differ only in metric parameters, camera and screen share a group: one device request
+ * queue (foort_gpu_queue_*, see foort_gpu.h) whose persistent kernel integrates the pixels
+ * of all of them together, in time slices of launch_us.  Each request keeps one iteration
+ * batch on the device at a time; mesh work runs on `workers` host threads meanwhile.  Each
+ * group's ring holds the largest power of two of pixels whose 40 bytes each (indices,
+ * request slot and results) fit in hbm_budget_bytes; a batch larger than the ring fails
+ * its request with FOORT_ENOMEM.  At most 1024 requests of a group are on the device at
+ * once; later ones wait for a slot. */
 typedef struct foort_request {
     foort_run_desc desc;
     foort_mesh_desc mesh;         /* pixel lists (InputCertainPixelsMesh) are not accepted */
@@ -146,14 +151,15 @@ typedef struct foort_request_result {
     uint32_t iterations;          /* mesh iterations run */
     uint64_t pixels, steps;       /* geodesics integrated and their steps */
     double submit_s, start_s, end_s; /* seconds since the server was created */
-    double kernel_ms;             /* GPU time of the request's batches */
-    uint64_t footprint_bytes;     /* device memory it was admitted with */
+    double kernel_ms;             /* its share of the GPU time of the launches it took part in
+                                     (by the steps of its geodesics that finished in each) */
+    uint64_t footprint_bytes;     /* ring bytes of its largest batch */
 } foort_request_result;
 
 typedef struct foort_server_config {
-    uint32_t workers;             /* requests running at once (one stream each); default 4 */
-    uint32_t reserved;
-    uint64_t hbm_budget_bytes;    /* admission budget; default 4 GiB */
+    uint32_t workers;             /* host threads for mesh work; default 4 */
+    uint32_t launch_us;           /* time slice of a queue launch; default 2000 */
+    uint64_t hbm_budget_bytes;    /* ring size per group; default 4 GiB */
 } foort_server_config;
 
 typedef struct foort_server foort_server;

@@ -32,6 +32,9 @@ CASES = {
     "test_cfg": (lambda **k: johannsen_cfg(**k), 40),
     "rasheed_larsen": (lambda **k: c0(metric=A.METRIC_RASHEEDLARSEN, params=(1.0, 0.6, 2.5, 3.0), rlog=True,
                                       inclination_deg=60.0, max_steps=20000, **k), 24),
+    # p = 2m: the constant (p^2 - 4m^2) is exactly zero, and a multiply-add must not make it negative
+    "rasheed_larsen_p2m": (lambda **k: c0(metric=A.METRIC_RASHEEDLARSEN, params=(1.0, 0.5, 2.0, 3.0), rlog=True,
+                                          inclination_deg=60.0, max_steps=20000, **k), 24),
     "manko_novikov": (lambda **k: c0(metric=A.METRIC_MANKONOVIKOV, params=(0.5, 1.0), rlog=True,
                                      inclination_deg=70.0, max_steps=20000, **k), 24),
     "flat": (lambda **k: c0(metric=A.METRIC_FLATSPACE, params=(), inclination_deg=45.0,
@@ -49,6 +52,7 @@ def test_analytic_close_to_fd_oracle(name, precision):
     with F.Context(mk(**kw)) as ctx:
         g = ctx.trace_grid(side)
     o = O.trace("fm", mk(), side)
+    assert np.isfinite(g["final_pos"]).all()
     same, rel = _compare(g, o)
     frac = 1 - same.mean()
     print(f"[analytic] {name} precision={precision}: integer mismatch {frac:.4f}, "

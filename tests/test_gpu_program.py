@@ -102,15 +102,16 @@ def test_persistent_schedule_writes_the_same_files(tmp_path):
 
 
 def test_server_requests_match_direct_runs():
-    """The render server runs requests concurrently (4 workers, each with its own context and
-    stream, moved between requests); every request's pixel and step counts equal those of the
-    same adaptive mesh driven directly, one batch at a time, through a fresh context."""
+    """The render server runs requests concurrently: those of one metric share a device queue
+    (time-sliced launches of 300 us, so geodesics span launches); every request's pixel and
+    step counts equal those of the same adaptive mesh driven directly, one batch at a time,
+    through a fresh context."""
     import foort_amd as F
     from foort_amd import workloads as W
     reqs = W.c4_requests(seed=7, n=12, rate=1000.0)
     reqs = [r for r in reqs if r[3]["side"] <= 300][:8]
     print(f"[server] {len(reqs)} requests", flush=True)
-    with P.Server(workers=4, hbm_budget_bytes=64 << 20) as srv:
+    with P.Server(workers=4, hbm_budget_bytes=64 << 20, launch_us=300) as srv:
         for k, (_, desc, mesh, _) in enumerate(reqs):
             srv.submit(desc, mesh, k)
         srv.drain()
@@ -131,3 +132,4 @@ def test_server_requests_match_direct_runs():
                 steps += int(out["stats"].steps)
         assert (r.pixels, r.steps) == (pixels, steps), (k, info)
         assert r.end_s >= r.start_s >= r.submit_s
+        assert r.kernel_ms > 0
