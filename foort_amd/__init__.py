@@ -12,7 +12,7 @@ import os
 import numpy as np
 
 from . import abi
-from .abi import RunDesc, GeodesicResults, Stats, make_desc, default_desc  # noqa: F401
+from .abi import RunDesc, GeodesicResults, Positions, Stats, make_desc, default_desc  # noqa: F401
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("FOORT_GPU_LIB") or os.path.join(_HERE, "libfoort_gpu.so")
@@ -25,7 +25,7 @@ EXPORTS = [
     "foort_gpu_synchronize", "foort_gpu_autotune", "foort_gpu_configure",
     "foort_gpu_queue_open", "foort_gpu_queue_close", "foort_gpu_queue_last_error", "foort_gpu_queue_add",
     "foort_gpu_queue_remove", "foort_gpu_queue_push", "foort_gpu_queue_run", "foort_gpu_queue_progress",
-    "foort_gpu_queue_results",
+    "foort_gpu_queue_results", "foort_gpu_trace_positions", "foort_positions_free",
 ]
 
 
@@ -70,6 +70,9 @@ def lib():
         L.foort_gpu_queue_progress.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64),
                                                C.POINTER(C.c_uint64)]
         L.foort_gpu_queue_results.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(GeodesicResults)]
+        L.foort_gpu_trace_positions.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p,
+                                                C.POINTER(GeodesicResults), C.POINTER(Positions), C.POINTER(Stats)]
+        L.foort_positions_free.argtypes = [C.POINTER(Positions)]
         if L.foort_desc_size() != C.sizeof(RunDesc):
             raise FoortError(f"foort_run_desc layout mismatch: library {L.foort_desc_size()} bytes, "
                              f"binding {C.sizeof(RunDesc)}")
@@ -162,6 +165,28 @@ class Context:
                                                    C.byref(st) if st is not None else None),
                     "foort_gpu_trace_device")
         return st
+
+    def trace_positions(self, side, screen_index=None, count=None):
+        """GeodesicPosition batch (foort_gpu_trace_positions): the results of trace_pixels plus
+        "offsets" (count + 1) and "points" (total x 4) of the kept positions."""
+        if screen_index is not None:
+            si = np.ascontiguousarray(screen_index, dtype=np.uint64).reshape(-1, 2)
+            n, ptr = si.shape[0], si.ctypes.data_as(C.c_void_p)
+        else:
+            n, ptr = count, None
+        out = _host_results(n)
+        st, pos = Stats(), Positions()
+        res = _results_struct(out, lambda a: a.ctypes.data_as(C.c_void_p))
+        self._check(self._L.foort_gpu_trace_positions(self._h, side, n, ptr, C.byref(res), C.byref(pos), C.byref(st)),
+                    "foort_gpu_trace_positions")
+        try:
+            out["offsets"] = np.ctypeslib.as_array(pos.offsets, shape=(n + 1,)).copy()
+            out["points"] = (np.ctypeslib.as_array(pos.points, shape=(pos.total, 4)).copy() if pos.total
+                             else np.zeros((0, 4)))
+        finally:
+            self._L.foort_positions_free(C.byref(pos))
+        out["stats"] = st
+        return out
 
     def configure(self, desc):
         """foort_gpu_configure: another run description on this context's buffers."""

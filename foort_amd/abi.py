@@ -45,6 +45,7 @@ class RunDesc(C.Structure):
         ("refill_lanes", C.c_uint32), ("refill_interval", C.c_uint32),
         ("mixed_switch_radius", C.c_double), ("mixed_pole_margin", C.c_double),
         ("launch_us", C.c_uint32), ("use_graph", C.c_uint32),
+        ("gp_update", UpdateFrequency), ("gp_output_steps", C.c_uint64),
     ]
 
 
@@ -54,6 +55,11 @@ class GeodesicResults(C.Structure):
         ("closest_radius", C.c_void_p), ("final_pos", C.c_void_p), ("final_vel", C.c_void_p),
         ("lambda_", C.c_void_p), ("steps", C.c_void_p),
     ]
+
+
+class Positions(C.Structure):
+    """foort_positions (include/foort_gpu.h)"""
+    _fields_ = [("offsets", C.POINTER(C.c_uint64)), ("points", C.POINTER(C.c_double)), ("total", C.c_uint64)]
 
 
 class Stats(C.Structure):
@@ -95,6 +101,7 @@ def default_desc():
     d.schedule = SCHEDULE_ONESHOT
     d.mixed_switch_radius = 10.0
     d.mixed_pole_margin = 0.1
+    d.gp_update = UpdateFrequency(1, 0, 1)
     return d
 
 

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -578,10 +579,6 @@ int build_run(const foort_run_desc &d, RunConst &R, KernelPair &kernels, std::st
         err = "unknown schedule";
         return FOORT_EINVAL;
     }
-    if (d.diag_flags & FOORT_DIAGFLAG_GEODESICPOSITION) {
-        err = "GeodesicPosition is not supported by the GPU path";
-        return FOORT_EINVAL;
-    }
     if (!(d.derivative_h > 0)) {
         err = "DerivativeH must be positive";
         return FOORT_EINVAL;
@@ -747,6 +744,7 @@ void foort_desc_init(foort_run_desc *d)
     d->schedule = FOORT_SCHEDULE_ONESHOT;
     d->mixed_switch_radius = 10;
     d->mixed_pole_margin = 0.1;
+    d->gp_update = foort_update_frequency{1, 0, 1};
 }
 
 int foort_desc_size(void)
@@ -1205,6 +1203,183 @@ int foort_gpu_queue_results(foort_gpu_queue *q, uint64_t first, uint64_t n, cons
     HIP_TRY(q->ctx, hipStreamSynchronize(st));
     q->unread.erase(it);
     return FOORT_OK;
+}
+
+// GeodesicPosition: a counting pass (saves per pixel, and every other result), the kept
+// points per pixel and their offsets on the host, then writing passes over chunks of pixels
+// whose points fit the device buffer.
+int foort_gpu_trace_positions(foort_gpu_ctx *ctx, uint64_t side, uint64_t count, const uint64_t *screen_index,
+                              const foort_geodesic_results *out, foort_positions *pos, foort_stats *stats)
+{
+    if (!ctx)
+        return fail(nullptr, FOORT_EINVAL, "NULL context");
+    if (!out || !pos)
+        return fail(ctx, FOORT_EINVAL, "results or positions pointer is NULL");
+    pos->offsets = nullptr;
+    pos->points = nullptr;
+    pos->total = 0;
+    if (!(ctx->desc.diag_flags & FOORT_DIAGFLAG_GEODESICPOSITION))
+        return fail(ctx, FOORT_EINVAL, "GeodesicPosition is not on in the context's description");
+    int rc = check_batch(ctx, side, 0, count, screen_index == nullptr);
+    if (rc)
+        return rc;
+    if (screen_index)
+        for (uint64_t i = 0; i < 2 * count; ++i)
+            if (screen_index[i] >= side)
+                return fail(ctx, FOORT_EINVAL, "screen index outside the row_column_size grid");
+    if (count > (uint64_t(1) << 31))
+        return fail(ctx, FOORT_EINVAL, "batch too large");
+    // points per writing pass: 2^24 (512 MiB), or FOORT_GP_CHUNK_POINTS (for tests of the chunking)
+    uint64_t kChunkPoints = uint64_t(1) << 24;
+    if (const char *e = std::getenv("FOORT_GP_CHUNK_POINTS"))
+        kChunkPoints = std::max<uint64_t>(1, std::strtoull(e, nullptr, 10));
+    const foort_update_frequency &f = ctx->desc.gp_update;
+    hipStream_t st = ctx->stream;
+
+    // device: [si | term | quadrant | passes | steps | steps of the writing pass | closest | lambda |
+    //          pos | vel | saved | seen | chunk offsets]
+    auto align = [](size_t v) { return (v + 255) & ~size_t(255); };
+    size_t off = 0;
+    size_t o[13];
+    const size_t bytes[13] = {screen_index ? count * 16 : 0, count * 4, count * 4, count * 4, count * 4, count * 4,
+                              count * 8, count * 8, count * 32, count * 32, count * 8, count * 8, count * 8};
+    for (int k = 0; k < 13; ++k) {
+        o[k] = off;
+        off = align(off + bytes[k]);
+    }
+    struct Dev {
+        void *p = nullptr;
+        ~Dev()
+        {
+            if (p)
+                (void)hipFree(p);
+        }
+    } blk, pts;
+    HIP_TRY(ctx, hipMalloc(&blk.p, std::max<size_t>(off, 256)));
+    char *b = static_cast<char *>(blk.p);
+    const uint64_t *d_si = screen_index ? reinterpret_cast<const uint64_t *>(b + o[0]) : nullptr;
+    DevResults d{reinterpret_cast<int32_t *>(b + o[1]), reinterpret_cast<int32_t *>(b + o[2]),
+                 reinterpret_cast<int32_t *>(b + o[3]), reinterpret_cast<double *>(b + o[6]),
+                 reinterpret_cast<double *>(b + o[8]), reinterpret_cast<double *>(b + o[9]),
+                 reinterpret_cast<double *>(b + o[7]), reinterpret_cast<uint32_t *>(b + o[4])};
+    uint32_t *d_steps2 = reinterpret_cast<uint32_t *>(b + o[5]);
+    uint64_t *d_saved = reinterpret_cast<uint64_t *>(b + o[10]);
+    uint64_t *d_seen = reinterpret_cast<uint64_t *>(b + o[11]);
+    uint64_t *d_off = reinterpret_cast<uint64_t *>(b + o[12]);
+    if (screen_index)
+        HIP_TRY(ctx, hipMemcpyAsync(b + o[0], screen_index, count * 16, hipMemcpyHostToDevice, st));
+
+    foort::GPArgs A{f.update_n_steps, f.update_start, f.update_finish, ctx->desc.gp_output_steps,
+                    d_saved, nullptr, nullptr, nullptr};
+    hipLaunchKernelGGL(zero_counters, dim3(1), dim3(64), 0, st, ctx->d_counters, 3);
+    HIP_TRY(ctx, hipGetLastError());
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
+    if (count)
+        hipLaunchKernelGGL(ctx->kernels.gp, dim3(static_cast<unsigned>((count + 255) / 256)), dim3(256), 0, st,
+                           ctx->rc, A, side, uint64_t(0), count, d_si, d, ctx->d_steps);
+    HIP_TRY(ctx, hipGetLastError());
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
+    unsigned long long steps = 0;
+    HIP_TRY(ctx, hipMemcpyAsync(&steps, ctx->d_steps, 8, hipMemcpyDeviceToHost, st));
+    std::vector<uint64_t> saved(count), seen(count), offs(count);
+    std::vector<uint32_t> steps1(count), steps2(count);
+    HIP_TRY(ctx, hipMemcpyAsync(saved.data(), d_saved, count * 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(ctx, hipMemcpyAsync(steps1.data(), d.steps, count * 4, hipMemcpyDeviceToHost, st));
+    const struct {
+        void *host;
+        const void *dev;
+        size_t size;
+    } res[8] = {{out->term, d.term, 4},        {out->quadrant, d.quadrant, 4}, {out->equat_passes, d.passes, 4},
+                {out->closest_radius, d.closest, 8}, {out->final_pos, d.pos, 32}, {out->final_vel, d.vel, 32},
+                {out->lambda, d.lambda, 8},    {out->steps, d.steps, 4}};
+    for (const auto &x : res)
+        if (x.host)
+            HIP_TRY(ctx, hipMemcpyAsync(x.host, x.dev, count * x.size, hipMemcpyDeviceToHost, st));
+    HIP_TRY(ctx, hipStreamSynchronize(st));
+    float ms = 0;
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+
+    // kept points per pixel (the reference's decimation) and their offsets
+    const uint64_t keep = ctx->desc.gp_output_steps;
+    auto kept = [keep](uint64_t S) { return (keep > 0 && keep < S) ? (S - 1) / (S / keep) + 1 : S; };
+    pos->offsets = static_cast<uint64_t *>(std::malloc((count + 1) * sizeof(uint64_t)));
+    if (!pos->offsets)
+        return fail(ctx, FOORT_ENOMEM, "out of host memory for the position offsets");
+    uint64_t total = 0, most = 0;
+    for (uint64_t i = 0; i < count; ++i) {
+        pos->offsets[i] = total;
+        total += kept(saved[i]);
+        most = std::max(most, kept(saved[i]));
+    }
+    pos->offsets[count] = total;
+    pos->total = total;
+    if (total) {
+        pos->points = static_cast<double *>(std::malloc(total * 4 * sizeof(double)));
+        if (!pos->points) {
+            foort_positions_free(pos);
+            return fail(ctx, FOORT_ENOMEM, "out of host memory for the positions");
+        }
+    }
+    const uint64_t cap = std::min(total, std::max(kChunkPoints, most));
+    if (cap)
+        HIP_TRY(ctx, hipMalloc(&pts.p, cap * 32));
+    uint64_t launches = 1;
+    // nothing saved anywhere: no writing pass (the kernel takes a NULL points buffer for counting)
+    for (uint64_t a = 0; total > 0 && a < count;) {
+        uint64_t e = a, n_pts = 0;
+        while (e < count && (e == a || n_pts + kept(saved[e]) <= cap)) {
+            offs[e] = n_pts;
+            n_pts += kept(saved[e]);
+            ++e;
+        }
+        HIP_TRY(ctx, hipMemcpyAsync(d_off + a, offs.data() + a, (e - a) * 8, hipMemcpyHostToDevice, st));
+        foort::GPArgs W{f.update_n_steps, f.update_start, f.update_finish, keep, d_saved + a, d_off + a,
+                        static_cast<double *>(pts.p), d_seen + a};
+        DevResults w{};
+        w.steps = d_steps2 + a;
+        HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
+        hipLaunchKernelGGL(ctx->kernels.gp, dim3(static_cast<unsigned>((e - a + 255) / 256)), dim3(256), 0, st,
+                           ctx->rc, W, side, a, e - a, d_si ? d_si + 2 * a : nullptr, w, ctx->d_steps);
+        HIP_TRY(ctx, hipGetLastError());
+        HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
+        if (n_pts)
+            HIP_TRY(ctx, hipMemcpyAsync(pos->points + 4 * pos->offsets[a], pts.p, n_pts * 32, hipMemcpyDeviceToHost,
+                                        st));
+        HIP_TRY(ctx, hipStreamSynchronize(st));
+        float wms = 0;
+        HIP_TRY(ctx, hipEventElapsedTime(&wms, ctx->ev0, ctx->ev1));
+        ms += wms;
+        ++launches;
+        a = e;
+    }
+    if (total > 0) {
+        HIP_TRY(ctx, hipMemcpyAsync(seen.data(), d_seen, count * 8, hipMemcpyDeviceToHost, st));
+        HIP_TRY(ctx, hipMemcpyAsync(steps2.data(), d_steps2, count * 4, hipMemcpyDeviceToHost, st));
+        HIP_TRY(ctx, hipStreamSynchronize(st));
+    }
+    for (uint64_t i = 0; total > 0 && i < count; ++i)
+        if (seen[i] != saved[i] || steps1[i] != steps2[i]) {
+            foort_positions_free(pos);
+            return fail(ctx, FOORT_EDEVICE, "the writing pass of GeodesicPosition did not repeat the counting pass");
+        }
+    if (stats) {
+        stats->geodesics = count;
+        stats->steps = steps;
+        stats->launches = count ? launches : 0;
+        stats->device_ms = ms;
+    }
+    return FOORT_OK;
+}
+
+void foort_positions_free(foort_positions *pos)
+{
+    if (!pos)
+        return;
+    std::free(pos->offsets);
+    std::free(pos->points);
+    pos->offsets = nullptr;
+    pos->points = nullptr;
+    pos->total = 0;
 }
 
 int foort_gpu_device_alloc(size_t bytes, void **ptr)

@@ -357,6 +357,82 @@ trace_queue(const RunConst R, const QueueArgs Q, DevResults out, unsigned long l
     add_steps(total_steps, mine);
 }
 
+// GeodesicPosition (FOORT/Diagnostics.cpp:239-284).  The saved positions of a geodesic are
+// the positions at which the diagnostic's DecideUpdate (FOORT/Diagnostics.cpp:113-140) is true,
+// at Reset and after every step.  At termination the reference keeps save i when
+// i % J == 0, J = S / keep (S saves, 0 < keep < S), and puts save S - 1 in place of the last
+// kept one if it is not kept already: K = (S - 1) / J + 1 points, save i going to slot i / J
+// and save S - 1 to slot K - 1.  The counting pass (points == NULL) stores S per pixel; the
+// writing pass re-integrates with the same code and stores each kept save at its slot.
+struct GPArgs {
+    uint64_t n;                // UpdateFrequency: UpdateNSteps
+    int32_t start, finish;     //                  UpdateStart, UpdateFinish
+    uint64_t keep;             // OutputSteps (0 = all)
+    uint64_t *saved;           // counting pass: S per pixel (written); writing pass: S (read)
+    const uint64_t *offset;    // writing pass: first slot of each pixel in points
+    double *points;            // writing pass: 4 doubles per kept point; NULL when counting
+    uint64_t *seen;            // writing pass: saves seen (equal to S when the passes agree)
+};
+
+template <class RHS, int INTEG>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(waves_per_eu<RHS>(FOORT_WPE_ONESHOT), waves_per_eu<RHS>(FOORT_WPE_ONESHOT))))
+trace_gp(const RunConst R, const GPArgs A, uint64_t side, uint64_t first, uint64_t count,
+         const uint64_t *__restrict__ si, DevResults out, unsigned long long *__restrict__ total_steps)
+{
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    unsigned long long mine = 0;
+    if (i < count) {
+        double ux, uy;
+        pixel_unit(side, first, si, i, ux, uy);
+        GeoState g;
+        geo_init<false>(R, ux, uy, g);
+        const bool write = A.points != nullptr;
+        const uint64_t S = write ? A.saved[i] : 0;
+        uint64_t J = 0, K = S;
+        if (write && A.keep > 0 && A.keep < S) {
+            J = S / A.keep;
+            K = (S - 1) / J + 1;
+        }
+        double *const base = write ? A.points + 4 * A.offset[i] : nullptr;
+        uint64_t cnt = 0, s = 0;
+        for (;;) {
+            // GeodesicPositionDiagnostic::UpdateData after Reset and after every Update
+            bool save;
+            if (A.n > 0) {
+                save = ++cnt >= A.n;
+                if (save)
+                    cnt = 0;
+            } else {
+                save = (A.start && g.lambda == 0.0) || (A.finish && g.term != FOORT_TERM_CONTINUE);
+            }
+            if (save) {
+                if (write && s < S) {
+                    const uint64_t slot = J == 0 ? s : (s % J == 0 ? s / J : (s == S - 1 ? K - 1 : K));
+                    if (slot < K) {
+                        double2 *p = reinterpret_cast<double2 *>(base + 4 * slot);
+                        p[0] = make_double2(g.pos[0], g.pos[1]);
+                        p[1] = make_double2(g.pos[2], g.pos[3]);
+                    }
+                }
+                ++s;
+            }
+            if (g.term != FOORT_TERM_CONTINUE)
+                break;
+            geo_update<RHS, INTEG, false>(R, g);
+        }
+        store_result(out, i, g);
+        if (write)
+            A.seen[i] = s;
+        else
+            A.saved[i] = s;
+        mine = g.steps;
+    }
+    add_steps(total_steps, mine);
+}
+
+using GPKernelFn = void (*)(const RunConst, const GPArgs, uint64_t, uint64_t, uint64_t, const uint64_t *, DevResults,
+                            unsigned long long *);
+
 using KernelFn = void (*)(const RunConst, uint64_t, uint64_t, uint64_t, const uint64_t *, DevResults,
                           unsigned long long *, LaneSlots, uint64_t);
 using QueueKernelFn = void (*)(const RunConst, const QueueArgs, DevResults, unsigned long long *, LaneSlots,
@@ -366,6 +442,7 @@ struct KernelPair {
     KernelFn oneshot;
     KernelFn persistent;
     QueueKernelFn queue;
+    GPKernelFn gp;             // GeodesicPosition (all update frequencies general)
 };
 
 // `every`: every termination and diagnostic updates on every step (the kernels
@@ -374,9 +451,9 @@ template <class RHS, int INTEG>
 KernelPair kernel_pair(bool every)
 {
     return every ? KernelPair{trace_oneshot<RHS, INTEG, true>, trace_persistent<RHS, INTEG, true>,
-                              trace_queue<RHS, INTEG, true>}
+                              trace_queue<RHS, INTEG, true>, trace_gp<RHS, INTEG>}
                  : KernelPair{trace_oneshot<RHS, INTEG, false>, trace_persistent<RHS, INTEG, false>,
-                              trace_queue<RHS, INTEG, false>};
+                              trace_queue<RHS, INTEG, false>, trace_gp<RHS, INTEG>};
 }
 
 template <template <int> class POLICY, int M>

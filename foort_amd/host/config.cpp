@@ -125,11 +125,13 @@ std::string diag_description(uint32_t flag, const RunConfig &rc)
     }
 }
 
-double value_distance(uint32_t valdiag, double a, double b)
+double value_distance(uint32_t valdiag, const MeshValue &a, const MeshValue &b)
 {
     switch (valdiag) {
-    case FOORT_DIAGFLAG_FOURCOLORSCREEN: return std::fabs(a - b) < 1.0 ? 0.0 : 1.0;
-    default: return std::fabs(a - b); // EquatorialPasses, ClosestRadius
+    case FOORT_DIAGFLAG_FOURCOLORSCREEN: return std::fabs(a.v0 - b.v0) < 1.0 ? 0.0 : 1.0;
+    case FOORT_DIAGFLAG_GEODESICPOSITION: // arc length between (theta, phi), FOORT/Diagnostics.cpp:310-321
+        return std::acos(std::cos(a.v0) * std::cos(b.v0) + std::sin(a.v0) * std::sin(b.v0) * std::cos(a.v1 - b.v1));
+    default: return std::fabs(a.v0 - b.v0); // EquatorialPasses, ClosestRadius
     }
 }
 
@@ -341,6 +343,8 @@ void init_diagnostics(const Setting &root, RunConfig &rc)
             g->lookup_counter("OutputSteps", steps);
             rc.gp_output_steps = steps;
             rc.gp_update = {n, s, f};
+            rc.desc.gp_output_steps = steps;
+            rc.desc.gp_update = {n, s, f};
             use_for_mesh(g, FOORT_DIAGFLAG_GEODESICPOSITION);
         }
         if (const Setting *g = on("EquatorialPasses")) {

@@ -48,6 +48,41 @@ struct BatchResults {
         default: return closest[i];
         }
     }
+    // getFullDataStr of FourColorScreen, EquatorialPasses (std::to_string of the int) and
+    // ClosestRadius (std::to_string of the double), FOORT/Diagnostics.cpp:202-206, :382-386, :448-452
+    std::string text(uint32_t diag, uint64_t i) const
+    {
+        switch (diag) {
+        case FOORT_DIAGFLAG_FOURCOLORSCREEN: return std::to_string(quadrant[i]);
+        case FOORT_DIAGFLAG_EQUATORIALPASSES: return std::to_string(passes[i]);
+        default: return std::to_string(closest[i]);
+        }
+    }
+};
+
+// The kept positions of a batch (GeodesicPosition): pixel i has points offsets[i] .. offsets[i + 1].
+struct Positions {
+    std::vector<uint64_t> offsets;
+    std::vector<double> points;
+    // GeodesicPositionDiagnostic::getFullDataStr, FOORT/Diagnostics.cpp:286-301:
+    // "<points> ;; " and then "x " for every coordinate of every point
+    std::string text(uint64_t i) const
+    {
+        const uint64_t a = offsets[i], b = offsets[i + 1];
+        std::string s = std::to_string(static_cast<unsigned long long>(b - a)) + " ;; ";
+        for (uint64_t k = 4 * a; k < 4 * b; ++k)
+            s += std::to_string(points[k]) + " ";
+        return s;
+    }
+    // getFinalDataVal (:303-308): (theta, phi) of the last kept point.  The reference reads
+    // the last element of an empty list when nothing was saved; here that gives (0, 0).
+    MeshValue last_angles(uint64_t i) const
+    {
+        const uint64_t b = offsets[i + 1];
+        if (b == offsets[i])
+            return MeshValue{};
+        return MeshValue{points[4 * (b - 1) + 2], points[4 * (b - 1) + 3]};
+    }
 };
 
 int copy_out(const std::string &s, char *buf, size_t cap)
@@ -203,8 +238,9 @@ int run(const char *cfg_path, const foort_program_options &o)
     screen_output(out.getFullDescriptionStr(), L);
     screen_output("--------------------------------\n", L);
 
-    if (rc.desc.diag_flags & FOORT_DIAGFLAG_GEODESICPOSITION) {
-        set_error("the GeodesicPosition diagnostic is not available on the GPU path");
+    const bool gp = (rc.desc.diag_flags & FOORT_DIAGFLAG_GEODESICPOSITION) != 0;
+    if (gp && o.tracer && !o.positions_tracer) {
+        set_error("a run with GeodesicPosition needs a positions tracer next to the batch tracer");
         screen_output(g_error, Level::Warning);
         return FOORT_EINVAL;
     }
@@ -258,7 +294,38 @@ int run(const char *cfg_path, const foort_program_options &o)
         foort_geodesic_results view = res.view();
         const uint64_t side = mesh->rowColumnSize();
         int rcode = FOORT_OK;
-        if (n > 0) {
+        Positions positions;
+        if (n > 0 && gp) {
+            if (o.positions_tracer) {
+                positions.offsets.assign(n + 1, 0);
+                rcode = o.positions_tracer(o.tracer_user, &rc.desc, side, n, idx.data(), &view,
+                                           positions.offsets.data(), positions.points.data(),
+                                           positions.points.size() / 4);
+                if (!rcode && positions.offsets[n] > positions.points.size() / 4) {
+                    positions.points.resize(4 * positions.offsets[n]);
+                    rcode = o.positions_tracer(o.tracer_user, &rc.desc, side, n, idx.data(), &view,
+                                               positions.offsets.data(), positions.points.data(),
+                                               positions.points.size() / 4);
+                }
+                if (rcode)
+                    set_error("positions tracer failed with code " + std::to_string(rcode));
+            } else {
+                foort_positions pos{};
+                foort_stats st{};
+                rcode = foort_gpu_trace_positions(gpu, side, n, grid ? nullptr : idx.data(), &view, &pos, &st);
+                if (rcode) {
+                    set_error(std::string("GPU path: ") + foort_gpu_last_error(gpu));
+                } else {
+                    positions.offsets.assign(pos.offsets, pos.offsets + n + 1);
+                    positions.points.assign(pos.points, pos.points + 4 * pos.total);
+                }
+                foort_positions_free(&pos);
+                total.geodesics += st.geodesics;
+                total.steps += st.steps;
+                total.launches += st.launches;
+                total.device_ms += st.device_ms;
+            }
+        } else if (n > 0) {
             if (o.tracer) {
                 rcode = o.tracer(o.tracer_user, &rc.desc, side, n, idx.data(), &view);
                 if (rcode)
@@ -315,14 +382,29 @@ int run(const char *cfg_path, const foort_program_options &o)
         const uint32_t vd = order.front();
         // GeodesicFinished and NewGeodesicOutput write the slot of their own index only
         // (as in the reference's parallel loop), so the geodesics are handed over in parallel
+        if (!gp) {
 #pragma omp parallel for schedule(static)
-        for (long long i = 0; i < static_cast<long long>(n); ++i) {
-            const uint64_t u = static_cast<uint64_t>(i);
-            mesh->GeodesicFinished(u, res.value(vd, u));
-            double vals[4];
-            for (size_t k = 0; k < order.size(); ++k)
-                vals[k] = res.value(order[k], u);
-            out.NewGeodesicValues(u, idx[2 * i], idx[2 * i + 1], vals);
+            for (long long i = 0; i < static_cast<long long>(n); ++i) {
+                const uint64_t u = static_cast<uint64_t>(i);
+                mesh->GeodesicFinished(u, MeshValue{res.value(vd, u), 0});
+                double vals[4];
+                for (size_t k = 0; k < order.size(); ++k)
+                    vals[k] = res.value(order[k], u);
+                out.NewGeodesicValues(u, idx[2 * i], idx[2 * i + 1], vals);
+            }
+        } else {
+            // the texts (long ones for GeodesicPosition) are formed in parallel and handed over in order
+            std::vector<std::vector<std::string>> texts(n);
+#pragma omp parallel for schedule(dynamic, 256)
+            for (long long i = 0; i < static_cast<long long>(n); ++i) {
+                const uint64_t u = static_cast<uint64_t>(i);
+                mesh->GeodesicFinished(u, vd == FOORT_DIAGFLAG_GEODESICPOSITION ? positions.last_angles(u)
+                                                                                : MeshValue{res.value(vd, u), 0});
+                for (uint32_t d : order)
+                    texts[u].push_back(d == FOORT_DIAGFLAG_GEODESICPOSITION ? positions.text(u) : res.text(d, u));
+            }
+            for (uint64_t u = 0; u < n; ++u)
+                out.NewGeodesicOutput(u, idx[2 * u], idx[2 * u + 1], std::move(texts[u]));
         }
         t_results += lap(t);
         const double it = seconds_since(t_iter), tt = seconds_since(t_total);
@@ -443,7 +525,19 @@ int foort_mesh_finished(foort_mesh *mesh, const double *values)
     }
     const uint64_t n = mesh->m->getCurNrGeodesics();
     for (uint64_t i = 0; i < n; ++i)
-        mesh->m->GeodesicFinished(i, values[i]);
+        mesh->m->GeodesicFinished(i, MeshValue{values[i], 0});
+    return FOORT_OK;
+}
+
+int foort_mesh_finished_pairs(foort_mesh *mesh, const double *pairs)
+{
+    if (!mesh || (!pairs && mesh->m->getCurNrGeodesics())) {
+        set_error("NULL argument");
+        return FOORT_EINVAL;
+    }
+    const uint64_t n = mesh->m->getCurNrGeodesics();
+    for (uint64_t i = 0; i < n; ++i)
+        mesh->m->GeodesicFinished(i, MeshValue{pairs[2 * i], pairs[2 * i + 1]});
     return FOORT_OK;
 }
 

@@ -74,7 +74,12 @@ std::string viewscreen_description(const RunConfig &rc, const std::string &mesh_
 std::string first_line_info(const RunConfig &rc, const std::string &mesh_description);
 
 // FinalDataValDistance of the value diagnostic (FOORT/Diagnostics.cpp:214-221, :395-401, :460-464)
-double value_distance(uint32_t valdiag, double a, double b);
+// A geodesic's value for the mesh (getDiagnosticFinalValue): one number, or (theta, phi) for
+// GeodesicPosition (FOORT/Diagnostics.cpp:303-308)
+struct MeshValue {
+    double v0 = 0, v1 = 0;
+};
+double value_distance(uint32_t valdiag, const MeshValue &a, const MeshValue &b);
 
 // ---- meshes: FOORT/Mesh.h:24-61 ----
 class Mesh {
@@ -83,7 +88,7 @@ public:
     virtual ~Mesh() = default;
     virtual uint64_t getCurNrGeodesics() const = 0;
     virtual void getNewInitConds(uint64_t index, uint64_t &row, uint64_t &col) const = 0;
-    virtual void GeodesicFinished(uint64_t index, double value) = 0;
+    virtual void GeodesicFinished(uint64_t index, MeshValue value) = 0;
     virtual void EndCurrentLoop() = 0;
     virtual bool IsFinished() const = 0;
     virtual std::string getFullDescriptionStr() const = 0;
@@ -92,7 +97,7 @@ public:
     void screenIndices(uint64_t *out) const;
 
 protected:
-    double distance(double a, double b) const { return value_distance(m_valdiag, a, b); }
+    double distance(const MeshValue &a, const MeshValue &b) const { return value_distance(m_valdiag, a, b); }
     uint32_t m_valdiag;
 };
 

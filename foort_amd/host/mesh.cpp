@@ -138,7 +138,7 @@ public:
         row = index / m_side;
         col = index - row * m_side;
     }
-    void GeodesicFinished(uint64_t, double) override {}
+    void GeodesicFinished(uint64_t, MeshValue) override {}
     void EndCurrentLoop() override { m_finished = true; }
     bool IsFinished() const override { return m_finished; }
     std::string getFullDescriptionStr() const override
@@ -166,7 +166,7 @@ public:
         row = m_px[2 * index];
         col = m_px[2 * index + 1];
     }
-    void GeodesicFinished(uint64_t, double) override {}
+    void GeodesicFinished(uint64_t, MeshValue) override {}
     void EndCurrentLoop() override { m_finished = true; }
     bool IsFinished() const override { return m_finished; }
     std::string getFullDescriptionStr() const override { return "Mesh: User-input pixels"; }
@@ -246,7 +246,7 @@ public:
         row = m_queue[index].row;
         col = m_queue[index].col;
     }
-    void GeodesicFinished(uint64_t index, double value) override
+    void GeodesicFinished(uint64_t index, MeshValue value) override
     {
         m_queue[index].value = value;
         m_done[index] = 1;
@@ -333,7 +333,7 @@ private:
         uint64_t row, col;
         int sub;
         double weight = -1;
-        double value = 0;
+        MeshValue value{};
         uint64_t lower = 0, right = 0; // indices into m_all; 0 with both 0 = not yet known
     };
 
@@ -498,7 +498,7 @@ public:
         row = m_px[m_queue[index]].row;
         col = m_px[m_queue[index]].col;
     }
-    void GeodesicFinished(uint64_t index, double value) override
+    void GeodesicFinished(uint64_t index, MeshValue value) override
     {
         m_px[m_queue[index]].value = value;
         m_done[index] = 1;
@@ -554,7 +554,7 @@ private:
         uint64_t row, col;
         int sub;
         double weight = -1;
-        double value = 0;
+        MeshValue value{};
         int64_t left = -1, right = -1, up = -1, down = -1, se = -1;
     };
 

@@ -142,9 +142,11 @@ struct foort_server {
                 r->vd = diagnostic_order(r->req.desc.diag_flags, r->req.desc.value_diag).front();
             } else {
                 for (uint64_t i = 0; i < r->n; ++i)
-                    r->mesh->GeodesicFinished(i, r->vd == FOORT_DIAGFLAG_FOURCOLORSCREEN   ? r->quad[i]
-                                                 : r->vd == FOORT_DIAGFLAG_EQUATORIALPASSES ? r->passes[i]
-                                                                                            : r->closest[i]);
+                    r->mesh->GeodesicFinished(
+                        i, MeshValue{r->vd == FOORT_DIAGFLAG_FOURCOLORSCREEN    ? r->quad[i]
+                                     : r->vd == FOORT_DIAGFLAG_EQUATORIALPASSES ? r->passes[i]
+                                                                                : r->closest[i],
+                                     0});
                 r->mesh->EndCurrentLoop();
                 r->res.pixels += r->n;
                 ++r->res.iterations;
@@ -362,6 +364,10 @@ int foort_server_submit(foort_server *srv, const foort_request *req)
     }
     if (req->mesh.type == FOORT_MESH_INPUTCERTAINPIXELS) {
         set_error("the server takes grid and subdivision meshes");
+        return FOORT_EINVAL;
+    }
+    if (req->desc.diag_flags & FOORT_DIAGFLAG_GEODESICPOSITION) {
+        set_error("the server does not run GeodesicPosition");
         return FOORT_EINVAL;
     }
     auto *r = new Req();

@@ -27,7 +27,7 @@ MESH_SQUARESUBDIVISIONV2 = 3
 
 EXPORTS = [
     "foort_host_last_error", "foort_mesh_create", "foort_mesh_destroy", "foort_mesh_count",
-    "foort_mesh_row_column_size", "foort_mesh_screen_indices", "foort_mesh_finished", "foort_mesh_end_loop",
+    "foort_mesh_row_column_size", "foort_mesh_screen_indices", "foort_mesh_finished", "foort_mesh_finished_pairs", "foort_mesh_end_loop",
     "foort_mesh_is_finished", "foort_mesh_description", "foort_config_read", "foort_config_free",
     "foort_config_run_desc", "foort_config_mesh_desc", "foort_config_first_line", "foort_config_diag_names",
     "foort_output_create", "foort_output_destroy", "foort_output_prepare", "foort_output_geodesic",
@@ -49,6 +49,9 @@ class MeshDesc(C.Structure):
 
 TRACE_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(RunDesc), C.c_uint64, C.c_uint64,
                        C.POINTER(C.c_uint64), C.POINTER(GeodesicResults))
+POSITIONS_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(RunDesc), C.c_uint64, C.c_uint64,
+                           C.POINTER(C.c_uint64), C.POINTER(GeodesicResults), C.POINTER(C.c_uint64),
+                           C.POINTER(C.c_double), C.c_uint64)
 
 
 class ProgramOptions(C.Structure):
@@ -56,7 +59,7 @@ class ProgramOptions(C.Structure):
     _fields_ = [
         ("output_prefix", C.c_char_p), ("christoffel", C.c_int32), ("precision", C.c_int32),
         ("schedule", C.c_int32), ("screen_output_level", C.c_int32), ("tracer", TRACE_FN),
-        ("tracer_user", C.c_void_p), ("stats", C.POINTER(Stats)),
+        ("tracer_user", C.c_void_p), ("stats", C.POINTER(Stats)), ("positions_tracer", POSITIONS_FN),
     ]
 
 
@@ -95,6 +98,7 @@ def lib():
             getattr(L, f).argtypes = [C.c_void_p]
         L.foort_mesh_screen_indices.argtypes = [C.c_void_p, C.c_void_p]
         L.foort_mesh_finished.argtypes = [C.c_void_p, C.c_void_p]
+        L.foort_mesh_finished_pairs.argtypes = [C.c_void_p, C.c_void_p]
         L.foort_mesh_end_loop.argtypes = [C.c_void_p]
         L.foort_mesh_is_finished.argtypes = [C.c_void_p]
         L.foort_mesh_description.argtypes = [C.c_void_p, C.c_char_p, C.c_size_t]
@@ -166,6 +170,12 @@ class Mesh:
         v = np.ascontiguousarray(values, np.float64)
         assert v.shape == (self.count,)
         _check(lib().foort_mesh_finished(self._h, v.ctypes.data_as(C.c_void_p)))
+
+    def finished_pairs(self, pairs):
+        """GeodesicPosition mesh values: (theta, phi) per geodesic."""
+        v = np.ascontiguousarray(pairs, np.float64)
+        assert v.shape == (self.count, 2)
+        _check(lib().foort_mesh_finished_pairs(self._h, v.ctypes.data_as(C.c_void_p)))
 
     def end_loop(self):
         _check(lib().foort_mesh_end_loop(self._h))
@@ -253,23 +263,42 @@ def run(cfg_path, output_prefix=None, christoffel=-1, precision=-1, schedule=-1,
     st = Stats()
     o.stats = C.pointer(st)
     if tracer is not None:
+        def _fill(out_p, count, res):
+            out = out_p.contents
+            for name, ctype in (("term", C.c_int32), ("quadrant", C.c_int32), ("equat_passes", C.c_int32),
+                                ("closest_radius", C.c_double)):
+                ptr = getattr(out, name)
+                if ptr:
+                    dst = np.ctypeslib.as_array(C.cast(ptr, C.POINTER(ctype)), shape=(count,))
+                    dst[:] = res[name]
+
         def _cb(user, desc_p, side, count, si_p, out_p):
             try:
                 si = np.ctypeslib.as_array(si_p, shape=(count * 2,)).reshape(count, 2).copy()
+                _fill(out_p, count, tracer(desc_p.contents, side, si))
+                return 0
+            except Exception:  # noqa: BLE001 -- reported through the return code
+                import traceback
+                traceback.print_exc()
+                return -1
+
+        def _pos_cb(user, desc_p, side, count, si_p, out_p, off_p, pts_p, cap):
+            # the tracer returns "offsets" and "points" too when the description has GeodesicPosition
+            try:
+                si = np.ctypeslib.as_array(si_p, shape=(count * 2,)).reshape(count, 2).copy()
                 res = tracer(desc_p.contents, side, si)
-                out = out_p.contents
-                for name, ctype in (("term", C.c_int32), ("quadrant", C.c_int32), ("equat_passes", C.c_int32),
-                                    ("closest_radius", C.c_double)):
-                    ptr = getattr(out, name)
-                    if ptr:
-                        dst = np.ctypeslib.as_array(C.cast(ptr, C.POINTER(ctype)), shape=(count,))
-                        dst[:] = res[name]
+                _fill(out_p, count, res)
+                np.ctypeslib.as_array(off_p, shape=(count + 1,))[:] = res["offsets"]
+                total = int(res["offsets"][-1])
+                if total and total <= cap:
+                    np.ctypeslib.as_array(pts_p, shape=(total * 4,))[:] = np.asarray(res["points"]).reshape(-1)
                 return 0
             except Exception:  # noqa: BLE001 -- reported through the return code
                 import traceback
                 traceback.print_exc()
                 return -1
         o.tracer = TRACE_FN(_cb)
+        o.positions_tracer = POSITIONS_FN(_pos_cb)
     rc = lib().foort_program_run(cfg_path.encode(), C.byref(o))
     _check(rc)
     return st

@@ -139,6 +139,9 @@ typedef struct foort_run_desc {
     uint32_t use_graph;          /* host entry points: run each batch that is one launch (one-shot, or
                                     persistent without limits) as a captured HIP graph -- index upload,
                                     kernels and result copies -- replayed while the batch shape is unchanged */
+    /* GeodesicPosition (FOORT/Diagnostics.h:331-337): its points come from foort_gpu_trace_positions */
+    foort_update_frequency gp_update;  /* (1, false, true) */
+    uint64_t gp_output_steps;    /* OutputSteps: points kept per geodesic (0 = all) */
 } foort_run_desc;
 
 /*
@@ -261,6 +264,28 @@ int foort_gpu_queue_progress(foort_gpu_queue *q, uint64_t *done_steps, uint64_t 
                              uint64_t *finished);
 int foort_gpu_queue_results(foort_gpu_queue *q, uint64_t first, uint64_t n,
                             const foort_geodesic_results *out);
+
+/* GeodesicPosition diagnostic (FOORT/Diagnostics.cpp:239-335): the positions a geodesic saves
+ * (at Reset and after every step, as the UpdateFrequency gating decides), decimated at
+ * termination to about gp_output_steps points.  For each pixel i the kept points (t, r or log r,
+ * theta, phi) are points[4 * offsets[i] .. 4 * offsets[i + 1]); offsets has count + 1 entries.
+ * The library allocates offsets and points (malloc); release them with foort_positions_free. */
+typedef struct foort_positions {
+    uint64_t *offsets;
+    double *points;
+    uint64_t total;              /* offsets[count]: points kept in all */
+} foort_positions;
+
+/* Integrate a batch with GeodesicPosition: screen_index as for foort_gpu_trace_pixels, or NULL
+ * for grid indices 0 .. count-1.  Fills *out (host pointers, as trace_pixels) and *pos.  Each
+ * geodesic is integrated twice (once to count its saved points, once to store the kept ones);
+ * the two passes run the same code and must agree step for step, else FOORT_EDEVICE.  The
+ * writing passes cover chunks of pixels whose points fit 2^24 points of device memory
+ * (FOORT_GP_CHUNK_POINTS in the environment sets another size). */
+int foort_gpu_trace_positions(foort_gpu_ctx *ctx, uint64_t row_column_size, uint64_t count,
+                              const uint64_t *screen_index, const foort_geodesic_results *out,
+                              foort_positions *pos, foort_stats *stats);
+void foort_positions_free(foort_positions *pos);
 
 /* Device memory helpers for callers that keep batches resident on the GPU
  * (no counterpart in the reference): allocate, free, copy, and wait for all

@@ -62,6 +62,9 @@ int foort_mesh_screen_indices(const foort_mesh *mesh, uint64_t *screen_index);
 /* Mesh::GeodesicFinished for indices 0 .. count-1 (FOORT/Mesh.h:44): the value of the
  * value diagnostic of every geodesic of the current iteration */
 int foort_mesh_finished(foort_mesh *mesh, const double *values);
+/* The same with two values per geodesic, (theta, phi), for a GeodesicPosition mesh value
+ * (FOORT/Diagnostics.cpp:303-308). */
+int foort_mesh_finished_pairs(foort_mesh *mesh, const double *pairs);
 /* Mesh::EndCurrentLoop, FOORT/Mesh.h:49 */
 int foort_mesh_end_loop(foort_mesh *mesh);
 /* Mesh::IsFinished, FOORT/Mesh.h:52 */
@@ -113,6 +116,14 @@ typedef int (*foort_trace_fn)(void *user, const foort_run_desc *desc, uint64_t r
                               uint64_t count, const uint64_t *screen_index,
                               const foort_geodesic_results *out);
 
+/* A batch tracer for runs with GeodesicPosition: as foort_trace_fn, and also the kept positions
+ * (see foort_positions in foort_gpu.h): offsets[0 .. count] always, points (4 doubles each) only
+ * when offsets[count] <= capacity (the program then calls again with room enough). */
+typedef int (*foort_positions_fn)(void *user, const foort_run_desc *desc, uint64_t row_column_size,
+                                  uint64_t count, const uint64_t *screen_index,
+                                  const foort_geodesic_results *out, uint64_t *offsets, double *points,
+                                  uint64_t capacity);
+
 typedef struct foort_program_options {
     const char *output_prefix;    /* replaces Output.FilePrefix when not NULL */
     int32_t christoffel;          /* -1: from the configuration; else FOORT_CHRISTOFFEL_* */
@@ -122,6 +133,7 @@ typedef struct foort_program_options {
     foort_trace_fn tracer;        /* NULL: the GPU */
     void *tracer_user;
     foort_stats *stats;           /* if not NULL: totals over all iterations */
+    foort_positions_fn positions_tracer; /* with `tracer`, for runs with GeodesicPosition */
 } foort_program_options;
 
 void foort_program_options_init(foort_program_options *opts);

@@ -68,6 +68,41 @@ def trace(kind, desc, size, first=0, count=None, screen_index=None, nthreads=Non
     return out
 
 
+def trace_positions(kind, desc, size, first=0, count=None, screen_index=None, nthreads=None):
+    """oracle_trace with GeodesicPosition ("libm" or "fm"): the results of trace() plus
+    "kept" (points per pixel), "offsets" (count + 1) and "points" (total x 4)."""
+    L = lib(kind)
+    fn = L.oracle_trace_positions
+    if screen_index is not None:
+        si = np.ascontiguousarray(screen_index, dtype=np.uint64).reshape(-1, 2)
+        count = si.shape[0]
+        si_ptr = si.ctypes.data_as(C.c_void_p)
+    else:
+        if count is None:
+            count = size * size - first
+        si_ptr = None
+    out = {
+        "term": np.zeros(count, np.int32), "quadrant": np.zeros(count, np.int32),
+        "equat_passes": np.zeros(count, np.int32), "closest_radius": np.zeros(count, np.float64),
+        "final_pos": np.zeros((count, 4), np.float64), "final_vel": np.zeros((count, 4), np.float64),
+        "lambda": np.zeros(count, np.float64), "steps": np.zeros(count, np.uint64),
+        "kept": np.zeros(count, np.uint64),
+    }
+    nthreads = nthreads or min(16, os.cpu_count() or 1)
+    fn.restype = C.c_uint64
+    fn.argtypes = ([C.POINTER(RunDesc), C.c_uint64, C.c_uint64, C.c_uint64, C.c_void_p] + [C.c_void_p] * 8 +
+                   [C.c_int, C.c_void_p, C.c_void_p, C.c_uint64])
+    args = [C.byref(desc), size, first, count, si_ptr] + [out[k].ctypes.data_as(C.c_void_p) for k in
+            ("term", "quadrant", "equat_passes", "closest_radius", "final_pos", "final_vel", "lambda", "steps")]
+    total = fn(*args, nthreads, out["kept"].ctypes.data_as(C.c_void_p), None, 0)
+    points = np.zeros((max(total, 1), 4), np.float64)
+    total2 = fn(*args, nthreads, out["kept"].ctypes.data_as(C.c_void_p), points.ctypes.data_as(C.c_void_p), total)
+    assert total2 == total
+    out["points"] = points[:total]
+    out["offsets"] = np.concatenate([[0], np.cumsum(out["kept"])]).astype(np.uint64)
+    return out
+
+
 def _vec(a):
     return np.ascontiguousarray(a, dtype=np.float64)
 

@@ -26,6 +26,7 @@
  */
 #include <math.h>
 #include <stdint.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include "foort_gpu.h"
@@ -860,6 +861,106 @@ int oracle_trace(const foort_run_desc *d, uint64_t size, uint64_t first, uint64_
         memcpy(&fvel[4 * ii], g.vel, sizeof(g.vel));
     }
     return 0;
+}
+
+/* GeodesicPositionDiagnostic, FOORT/Diagnostics.cpp:239-284: every saved position of one
+ * geodesic in a growing array; at termination the array is cut down to about OutputSteps
+ * entries: keep index i when i % (size / OutputSteps) == 0, then put the last saved point in
+ * place of the last kept one unless it is kept already. */
+typedef struct {
+    double *p;
+    uint64_t n, cap;
+} optlist;
+
+static void optlist_push(optlist *l, const double *pos)
+{
+    if (l->n == l->cap) {
+        l->cap = l->cap ? 2 * l->cap : 64;
+        l->p = (double *)realloc(l->p, l->cap * 4 * sizeof(double));
+    }
+    memcpy(l->p + 4 * l->n, pos, 4 * sizeof(double));
+    ++l->n;
+}
+
+static void o_gp_update(const foort_update_frequency *f, uint64_t keep, const ogeo *g, uint64_t *cnt, optlist *l)
+{
+    if (diag_decide(f, cnt, g))
+        optlist_push(l, g->pos);
+    if (g->term != FOORT_TERM_CONTINUE && keep > 0 && keep < l->n) {
+        const uint64_t jettison = l->n / keep;
+        optlist tmp = {0, 0, 0};
+        for (uint64_t i = 0; i < l->n; ++i)
+            if (i % jettison == 0)
+                optlist_push(&tmp, l->p + 4 * i);
+        if ((l->n - 1) % jettison != 0) {
+            --tmp.n;
+            optlist_push(&tmp, l->p + 4 * (l->n - 1));
+        }
+        free(l->p);
+        *l = tmp;
+    }
+}
+
+/* oracle_trace with GeodesicPosition: kept[i] = the points pixel i keeps; the points of all
+ * pixels, in pixel order, go to `points` (4 doubles each) when they fit in `capacity` points.
+ * Returns the total number of kept points. */
+uint64_t oracle_trace_positions(const foort_run_desc *d, uint64_t size, uint64_t first, uint64_t count,
+                                const uint64_t *screen_index, int32_t *term, int32_t *quadrant, int32_t *passes,
+                                double *closest, double *fpos, double *fvel, double *lambda, uint64_t *steps,
+                                int nthreads, uint64_t *kept, double *points, uint64_t capacity)
+{
+    orun R;
+    o_setup(&R, d);
+    optlist *lists = (optlist *)calloc(count ? count : 1, sizeof(optlist));
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+    for (int64_t ii = 0; ii < (int64_t)count; ++ii) {
+        uint64_t row, col;
+        if (screen_index) {
+            row = screen_index[2 * ii];
+            col = screen_index[2 * ii + 1];
+        } else {
+            uint64_t index = first + (uint64_t)ii;
+            row = index / size;
+            col = index - row * size;
+        }
+        double ux = row * 1.0 / (double)(size - 1);
+        double uy = col * 1.0 / (double)(size - 1);
+        double pos[DIM], vel[DIM];
+        o_initial(&R, ux, uy, pos, vel);
+        ogeo g;
+        uint64_t gpc = 0;
+        o_reset(&R, &g, pos, vel);
+        o_gp_update(&d->gp_update, d->gp_output_steps, &g, &gpc, &lists[ii]);
+        while (g.term == FOORT_TERM_CONTINUE) {
+            o_update(&R, &g);
+            o_gp_update(&d->gp_update, d->gp_output_steps, &g, &gpc, &lists[ii]);
+        }
+        term[ii] = g.term;
+        quadrant[ii] = g.quadrant;
+        passes[ii] = g.passes;
+        closest[ii] = g.closest;
+        lambda[ii] = g.lambda;
+        steps[ii] = g.steps;
+        memcpy(&fpos[4 * ii], g.pos, sizeof(g.pos));
+        memcpy(&fvel[4 * ii], g.vel, sizeof(g.vel));
+    }
+    uint64_t total = 0;
+    for (uint64_t i = 0; i < count; ++i) {
+        kept[i] = lists[i].n;
+        total += lists[i].n;
+    }
+    if (points && total <= capacity) {
+        uint64_t at = 0;
+        for (uint64_t i = 0; i < count; ++i) {
+            if (lists[i].n)
+                memcpy(points + 4 * at, lists[i].p, lists[i].n * 4 * sizeof(double));
+            at += lists[i].n;
+        }
+    }
+    for (uint64_t i = 0; i < count; ++i)
+        free(lists[i].p);
+    free(lists);
+    return total;
 }
 
 /* Unit-level entry points for tests */

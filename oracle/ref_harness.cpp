@@ -85,8 +85,12 @@ void setup(Run &R, const foort_run_desc *d, std::unique_ptr<Mesh> mesh)
     R.source = std::unique_ptr<Source>(new NoSource(R.metric.get()));
     const auto *sph = dynamic_cast<const SphericalHorizonMetric *>(R.metric.get());
     bool rl = sph && sph->getrLogScale();
-    R.diags = static_cast<DiagBitflag>(d->diag_flags & (Diag_FourColorScreen | Diag_EquatorialPasses | Diag_ClosestRadius));
+    R.diags = static_cast<DiagBitflag>(d->diag_flags & (Diag_FourColorScreen | Diag_EquatorialPasses | Diag_ClosestRadius |
+                                                        Diag_GeodesicPosition));
     R.valdiag = static_cast<DiagBitflag>(d->value_diag);
+    GeodesicPositionDiagnostic::DiagOptions = std::make_unique<GeodesicPositionOptions>(
+        d->gp_output_steps,
+        UpdateFrequency{d->gp_update.update_n_steps, d->gp_update.update_start != 0, d->gp_update.update_finish != 0});
     EquatorialPassesDiagnostic::DiagOptions = std::make_unique<EquatorialPassesOptions>(
         d->ep_threshold, UpdateFrequency{d->ep_update.update_n_steps, d->ep_update.update_start != 0, d->ep_update.update_finish != 0});
     ClosestRadiusDiagnostic::DiagOptions = std::make_unique<ClosestRadiusOptions>(

@@ -28,6 +28,7 @@ import pytest
 import oracle as O
 from foort_amd import abi as A
 from foort_amd import program as P
+from foort_amd import workloads as W
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLD = os.path.join(HERE, "golden")
@@ -397,6 +398,101 @@ def test_program_matches_reference_program(i, tmp_path):
         assert ref[k] == mine[k], k
 
 
+def _oracle_tracer_gp(desc, side, si):
+    if desc.diag_flags & A.DIAGFLAG_GEODESICPOSITION:
+        return O.trace_positions("libm", desc, side, screen_index=si)
+    return O.trace("libm", desc, side, screen_index=si)
+
+
+GP_CASES = [
+    # (GeodesicPosition settings, other diagnostics, mesh text, integrator)
+    ("UpdateFrequency = 3; UpdateStart = false; UpdateFinish = true; OutputSteps = 7;",
+     "FourColorScreen = { On = true; UseForMesh = true; };",
+     'Type = "SimpleSquareMesh"; TotalPixels = 100;', 'Type = "RK4";'),
+    ("UpdateFrequency = 40; OutputSteps = 0; UseForMesh = true;",
+     "EquatorialPasses = { On = true; };",
+     'Type = "SquareSubdivisionMeshV2"; InitialPixels = 64; MaxPixels = 300; IterationPixels = 20; '
+     'MaxSubdivide = 3;', 'Type = "RK4";'),
+    ("UpdateFrequency = 0; UpdateStart = true; UpdateFinish = true; OutputSteps = 1;",
+     "ClosestRadius = { On = true; UseForMesh = true; }; FourColorScreen = { On = true; };",
+     'Type = "SquareSubdivisionMesh"; InitialPixels = 49; MaxPixels = 200; IterationPixels = 15; '
+     'MaxSubdivide = 3;', 'Type = "Verlet"; VerletVelocityTolerance = 0.001;'),
+]
+
+
+@need_ref
+@pytest.mark.parametrize("i", range(len(GP_CASES)))
+def test_program_with_geodesic_position_matches_reference_program(i, tmp_path):
+    """GeodesicPosition (saved positions, decimation to OutputSteps, the "N ;; x x x x ..." text and,
+    as a mesh value, (theta, phi) with the arc-length distance) against the reference program."""
+    gp, other, mesh, integ = GP_CASES[i]
+    cfg = tmp_path / "run.cfg"
+    cfg.write_text(f"""
+    Metric = {{ Name = "Kerr"; a = 0.8; RLogScale = true; }};
+    Diagnostics = {{ GeodesicPosition = {{ On = true; {gp} }}; {other} }};
+    Terminations = {{ Horizon = {{ On = true; }}; BoundarySphere = {{ On = true; SphereRadius = 1000.0; }};
+                     TimeOut = {{ On = true; MaxSteps = 3000; }}; }};
+    ViewScreen = {{ Position = {{ r = 1000.0; theta = 1.2; }}; ScreenSize = {{ x = 20.0; y = 20.0; }};
+                    Mesh = {{ {mesh} }}; }};
+    Integrator = {{ {integ} }};
+    Output = {{ FilePrefix = "{tmp_path}/mine/run"; FileExtension = "dat"; TimeStamp = false;
+               ScreenOutputLevel = 0; }};
+    """)
+    P.run(str(cfg), tracer=_oracle_tracer_gp)
+    c = P.Config(path=str(cfg))
+    d, m = c.run_desc(), c.mesh_desc()
+    assert d.diag_flags & A.DIAGFLAG_GEODESICPOSITION
+    O.ref_program_run(d, m.type, m.initial_pixels if m.type else m.total_pixels, str(tmp_path / "ref" / "run"), "dat",
+                      m.max_pixels, m.iteration_pixels, m.max_subdivide, bool(m.initial_subdivision_to_final))
+    ref, mine = _tree(tmp_path / "ref"), _tree(tmp_path / "mine")
+    assert "run_GeodesicPosition.dat" in ref
+    assert sorted(ref) == sorted(mine)
+    for k in ref:
+        assert ref[k] == mine[k], k
+
+
+def test_geodesic_position_decimation_rule():
+    """The oracle keeps what the reference's rule keeps: save i when i % (S // keep) == 0, the last
+    save in place of the last kept one; checked against the rule applied to the full list of saves."""
+    d = W.c0(params=(0.7,), rlog=True, inclination_deg=70.0, diags=A.DIAGFLAG_FOURCOLORSCREEN |
+             A.DIAGFLAG_GEODESICPOSITION, max_steps=4000)
+    si = np.array([[3, 4], [8, 8], [0, 15], [15, 2]], np.uint64)
+    d.gp_update = A.UpdateFrequency(1, 0, 1)
+    d.gp_output_steps = 0
+    full = O.trace_positions("fm", d, 16, screen_index=si)
+    assert np.array_equal(full["kept"], full["steps"] + 1)  # every call saves: Reset and each step
+    for keep in (1, 2, 5, 33, 10 ** 6):
+        d.gp_output_steps = keep
+        got = O.trace_positions("fm", d, 16, screen_index=si)
+        for p in range(len(si)):
+            pts = full["points"][full["offsets"][p]:full["offsets"][p + 1]]
+            S = len(pts)
+            if keep < S:
+                J = S // keep
+                idx = list(range(0, S, J))
+                if (S - 1) % J:
+                    idx[-1] = S - 1
+            else:
+                idx = list(range(S))
+            mine = got["points"][got["offsets"][p]:got["offsets"][p + 1]]
+            assert np.array_equal(mine, pts[idx]), (keep, p)
+    # update every 7th call, and only at the start and the end
+    d.gp_output_steps = 0
+    d.gp_update = A.UpdateFrequency(7, 1, 1)
+    got = O.trace_positions("fm", d, 16, screen_index=si)
+    for p in range(len(si)):
+        S = int(full["kept"][p])
+        assert np.array_equal(got["points"][got["offsets"][p]:got["offsets"][p + 1]],
+                              full["points"][full["offsets"][p]:full["offsets"][p + 1]][6::7][:S // 7])
+    d.gp_update = A.UpdateFrequency(0, 1, 1)
+    got = O.trace_positions("fm", d, 16, screen_index=si)
+    for p in range(len(si)):
+        pts = full["points"][full["offsets"][p]:full["offsets"][p + 1]]
+        assert np.array_equal(got["points"][got["offsets"][p]:got["offsets"][p + 1]], pts[[0, -1]])
+    d.gp_update = A.UpdateFrequency(0, 0, 0)
+    assert O.trace_positions("fm", d, 16, screen_index=si)["points"].shape == (0, 4)
+
+
 # ---------------------------------------------------------------------------
 # the command line program and the exported symbols
 # ---------------------------------------------------------------------------
@@ -412,17 +508,9 @@ def test_cli_early_exits_like_the_reference(tmp_path):
     assert r.returncode == 0 and "parse error at" in r.stderr and ":1 -" in r.stderr
 
 
-def test_cli_reports_unsupported_diagnostic(tmp_path):
-    cfg = tmp_path / "gp.cfg"
-    cfg.write_text('Diagnostics = { GeodesicPosition = { On = true; UseForMesh = true; }; };\n'
-                   'Output = { ScreenOutputLevel = 0; };\n')
-    r = subprocess.run([P.CLI_PATH, str(cfg)], capture_output=True, text=True, timeout=60)
-    assert r.returncode == 1 and "GeodesicPosition" in r.stderr
-
-
 def test_host_library_exports_every_declared_symbol():
     decl = open(os.path.join(os.path.dirname(HERE), "include", "foort_host.h")).read()
-    names = set(re.findall(r"\b(foort_[a-z_0-9]+)\s*\(", decl)) - {"foort_trace_fn"}
+    names = set(re.findall(r"\b(foort_[a-z_0-9]+)\s*\(", decl)) - {"foort_trace_fn", "foort_positions_fn"}
     assert names == set(P.EXPORTS)
     L = P.lib()
     for n in names:
