@@ -16,22 +16,29 @@
 #ifndef FOORT_WPE_PERSIST
 #define FOORT_WPE_PERSIST 2
 #endif
-// Manko-Novikov and Rasheed-Larsen: their potentials with derivatives need more
-// registers than two waves per SIMD leave, and spills to scratch memory stall a
-// lone wave (the time-outs that end an image) on every step.  Measured on MI355X:
-// one wave per SIMD halves the time of Manko-Novikov steps and takes about a sixth
-// off Rasheed-Larsen's; Johannsen does as well with two.
+// Manko-Novikov, Rasheed-Larsen and (except its finite-difference one-shot kernel) Johannsen:
+// their potentials with derivatives need more registers than two waves per SIMD leave, and
+// spills to scratch memory stall every step, worst for a lone wave (the time-outs that end an
+// image).  Measured on MI355X: one wave per SIMD halves the time of Manko-Novikov steps and
+// takes about a sixth off Rasheed-Larsen's.  With one wave per SIMD the accumulation registers
+// take the spills instead, and no kernel uses scratch memory.
 #ifndef FOORT_WPE_HEAVY
 #define FOORT_WPE_HEAVY 1
 #endif
 
 namespace foort {
 
+// occupancy target of a kernel: `resident` for the persistent and queue kernels (their lanes
+// keep a pixel index, launch counters and the wall clock next to the geodesic state) and the
+// GeodesicPosition kernel (save counters and the decimation's slot arithmetic)
 template <class RHS>
-constexpr int waves_per_eu(int wpe)
+constexpr int waves_per_eu(bool resident)
 {
-    return (RHS::metric == FOORT_METRIC_MANKONOVIKOV || RHS::metric == FOORT_METRIC_RASHEEDLARSEN) ? FOORT_WPE_HEAVY
-                                                                                                  : wpe;
+    if (RHS::metric == FOORT_METRIC_MANKONOVIKOV || RHS::metric == FOORT_METRIC_RASHEEDLARSEN)
+        return FOORT_WPE_HEAVY;
+    if (RHS::metric == FOORT_METRIC_JOHANNSEN)
+        return (RHS::exact && !resident) ? FOORT_WPE_ONESHOT : FOORT_WPE_HEAVY;
+    return resident ? FOORT_WPE_PERSIST : FOORT_WPE_ONESHOT;
 }
 
 struct DevResults {
@@ -134,7 +141,7 @@ __device__ __forceinline__ void add_steps(unsigned long long *total, unsigned lo
 // One lane per pixel, integrated to termination: the drop-in for the
 // per-thread loop of FOORT/Main.cpp:373-400.
 template <class RHS, int INTEG, bool EVERY>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(waves_per_eu<RHS>(FOORT_WPE_ONESHOT), waves_per_eu<RHS>(FOORT_WPE_ONESHOT))))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(waves_per_eu<RHS>(false), waves_per_eu<RHS>(false))))
 trace_oneshot(const RunConst R, uint64_t side, uint64_t first, uint64_t count,
                                                      const uint64_t *__restrict__ si, DevResults out,
                                                      unsigned long long *__restrict__ total_steps, LaneSlots,
@@ -214,7 +221,7 @@ __device__ __forceinline__ void slot_save(const LaneSlots &S, uint32_t l, const 
 // leaves as soon as none of its lanes has work.  The host relaunches until `done` reaches the
 // batch size.  The steps counted are those of the geodesics that finish.
 template <class RHS, int INTEG, bool EVERY>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(waves_per_eu<RHS>(FOORT_WPE_PERSIST), waves_per_eu<RHS>(FOORT_WPE_PERSIST))))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(waves_per_eu<RHS>(true), waves_per_eu<RHS>(true))))
 trace_persistent(const RunConst R, uint64_t side, uint64_t first,
                                                         uint64_t count, const uint64_t *__restrict__ si,
                                                         DevResults out, unsigned long long *__restrict__ total_steps,
@@ -283,7 +290,7 @@ trace_persistent(const RunConst R, uint64_t side, uint64_t first,
 // the metric parameters on every step -- and counting finished pixels per request.
 // Results go to storage index k & mask of the ring.
 template <class RHS, int INTEG, bool EVERY>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(waves_per_eu<RHS>(FOORT_WPE_PERSIST), waves_per_eu<RHS>(FOORT_WPE_PERSIST))))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(waves_per_eu<RHS>(true), waves_per_eu<RHS>(true))))
 trace_queue(const RunConst R, const QueueArgs Q, DevResults out, unsigned long long *__restrict__ total_steps,
             LaneSlots S, uint64_t budget)
 {
@@ -375,7 +382,7 @@ struct GPArgs {
 };
 
 template <class RHS, int INTEG>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(waves_per_eu<RHS>(FOORT_WPE_ONESHOT), waves_per_eu<RHS>(FOORT_WPE_ONESHOT))))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(waves_per_eu<RHS>(true), waves_per_eu<RHS>(true))))
 trace_gp(const RunConst R, const GPArgs A, uint64_t side, uint64_t first, uint64_t count,
          const uint64_t *__restrict__ si, DevResults out, unsigned long long *__restrict__ total_steps)
 {

@@ -667,6 +667,9 @@ void get_gpu_options(const Setting &root, RunConfig &rc)
     n = d.refill_lanes;
     if (g->lookup_counter("RefillLanes", n))
         d.refill_lanes = static_cast<uint32_t>(n);
+    n = d.refill_interval;
+    if (g->lookup_counter("RefillInterval", n))
+        d.refill_interval = static_cast<uint32_t>(n);
     g->lookup("MixedSwitchRadius", d.mixed_switch_radius);
     g->lookup("MixedPoleMargin", d.mixed_pole_margin);
     n = d.launch_us;

@@ -45,6 +45,21 @@ def _cfg(text):
 # 1. configuration reader
 # ---------------------------------------------------------------------------
 
+def test_gpu_group_sets_the_execution_options():
+    d = _cfg("""GPU = { Christoffel = "Analytic"; Precision = "Mixed"; Schedule = "Persistent"; StepCap = 5000;
+                        BlockSize = 128; StepsPerLaunch = 64; LanesPerCU = 512; RefillLanes = 16;
+                        RefillInterval = 8; MixedSwitchRadius = 20.0; MixedPoleMargin = 0.2;
+                        LaunchMicroseconds = 300; UseGraph = true; };""").run_desc()
+    assert (d.christoffel, d.precision, d.schedule) == (A.CHRISTOFFEL_ANALYTIC, A.PRECISION_MIXED,
+                                                        A.SCHEDULE_PERSISTENT)
+    assert (d.step_cap, d.block_size, d.steps_per_launch, d.lanes_per_cu) == (5000, 128, 64, 512)
+    assert (d.refill_lanes, d.refill_interval, d.launch_us, d.use_graph) == (16, 8, 300, 1)
+    assert (d.mixed_switch_radius, d.mixed_pole_margin) == (20.0, 0.2)
+    # unknown values fall back to the reference arithmetic and the one-shot schedule
+    d = _cfg('GPU = { Christoffel = "Exact"; Precision = "half"; Schedule = "x"; };').run_desc()
+    assert (d.christoffel, d.precision, d.schedule) == (A.CHRISTOFFEL_FD, A.PRECISION_FP64, A.SCHEDULE_ONESHOT)
+
+
 def test_defaults_without_any_setting():
     d = _cfg("").run_desc()
     # Kerr a = 0.5, FourColorScreen, BoundarySphere(1000) + TimeOut(10000), camera (0, 1000, pi/2, 0)
