@@ -54,7 +54,37 @@ FM_HD_INLINE float sexp(float x)
     return __builtin_expf(x);
 #endif
 }
-FM_HD_INLINE double sexp(double x) { return fm_exp(x); }
+// binary64 exponential of the analytic formulations (not the reference's arithmetic, so not
+// bound to fm_exp's bits): Cody and Waite's reduction x = k ln2 + r, |r| <= ln2/2, with ln2 in
+// two parts applied by fused multiply-adds, then exp(r) by its Taylor polynomial to r^13
+// (truncation below 5e-18 relative) and the scaling by 2^k with ldexp.  No table, so no
+// memory load: a lone wave does not wait on one.  Error within 2 ulp (tested against
+// numpy's exp over the path's argument range).
+FM_HD_INLINE double sexp(double x)
+{
+    if (!(x < 709.782712893384)) // overflow, +inf or NaN
+        return x + __builtin_huge_val();
+    if (x < -745.2)
+        return 0.0;
+    const double k = __builtin_rint(x * 1.4426950408889634);
+    double r = __builtin_fma(-k, 0x1.62e42fefa39efp-1, x);
+    r = __builtin_fma(-k, 0x1.abc9e3b39803fp-56, r);
+    double p = 1.0 / 6227020800.0;            // 1/13!
+    p = __builtin_fma(p, r, 1.0 / 479001600.0); // 1/12!
+    p = __builtin_fma(p, r, 1.0 / 39916800.0);
+    p = __builtin_fma(p, r, 1.0 / 3628800.0);
+    p = __builtin_fma(p, r, 1.0 / 362880.0);
+    p = __builtin_fma(p, r, 1.0 / 40320.0);
+    p = __builtin_fma(p, r, 1.0 / 5040.0);
+    p = __builtin_fma(p, r, 1.0 / 720.0);
+    p = __builtin_fma(p, r, 1.0 / 120.0);
+    p = __builtin_fma(p, r, 1.0 / 24.0);
+    p = __builtin_fma(p, r, 1.0 / 6.0);
+    p = __builtin_fma(p, r, 0.5);
+    p = __builtin_fma(p, r, 1.0);
+    p = __builtin_fma(p, r, 1.0);
+    return __builtin_ldexp(p, static_cast<int>(k));
+}
 FM_HD_INLINE float ssqrt(float x) { return __builtin_sqrtf(x); }
 FM_HD_INLINE double ssqrt(double x) { return __builtin_sqrt(x); }
 FM_HD_INLINE void ssincos(double x, double *s, double *c) { fm_sincos(x, s, c); }

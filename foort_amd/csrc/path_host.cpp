@@ -22,6 +22,8 @@ void params(const foort_run_desc *d, foort::RunConst &R)
         M.m = P[0] / ((P[2] + P[3]) / 4.0);
         M.p = P[2] / ((P[2] + P[3]) / 4.0);
         M.q = P[3] / ((P[2] + P[3]) / 4.0);
+        M.rl_root_pq = std::sqrt(M.p * M.q);
+        M.rl_root_pm = std::sqrt((M.p * M.p - 4. * M.m * M.m) * (M.q * M.q - 4. * M.m * M.m));
         break;
     case FOORT_METRIC_JOHANNSEN: M.a = P[0]; M.a13 = P[1]; M.a22 = P[2]; M.a52 = P[3]; M.eps3 = P[4]; break;
     default:
@@ -57,4 +59,11 @@ extern "C" void path_rhs(const foort_run_desc *d, int kind, const double *p, con
     case FOORT_METRIC_JOHANNSEN: dispatch<FOORT_METRIC_JOHANNSEN>(kind, R, p, v, out); break;
     default: dispatch<FOORT_METRIC_MANKONOVIKOV>(kind, R, p, v, out); break;
     }
+}
+
+// the binary64 exponential of the analytic formulations, for its accuracy test
+extern "C" void path_exp(const double *x, double *out, uint64_t n)
+{
+    for (uint64_t i = 0; i < n; ++i)
+        out[i] = foort::sexp(x[i]);
 }

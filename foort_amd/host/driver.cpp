@@ -247,11 +247,6 @@ int run(const char *cfg_path, const foort_program_options &o)
 
     foort_gpu_ctx *gpu = nullptr;
     if (!o.tracer) {
-        if (rc.autotune && tune_lookup(tune_key(rc.desc, rc.autotune), rc.desc)) {
-            screen_output("Autotuner: options of an earlier run of this configuration (" + tune_cache_path() + ").",
-                          Level::Subproc);
-            rc.autotune = 0;
-        }
         int rcode = foort_gpu_open(&rc.desc, &gpu);
         if (rcode != FOORT_OK) {
             set_error(std::string("GPU path: ") + foort_gpu_last_error(nullptr));
@@ -265,7 +260,7 @@ int run(const char *cfg_path, const foort_program_options &o)
     } closer{gpu};
 
     const bool grid = rc.mesh.type == FOORT_MESH_SIMPLESQUARE;
-    bool tuned = false;
+    int tuned_class = -1; // the largest batch size (log2) tuned for so far
     uint64_t iteration = 0;
     foort_stats total{};
     // wall time of the phases of the loop: screen indices, integration, results into the mesh and
@@ -331,37 +326,48 @@ int run(const char *cfg_path, const foort_program_options &o)
                 if (rcode)
                     set_error("batch tracer failed with code " + std::to_string(rcode));
             } else {
-                // tune on (a sample of at most 65536 pixels of) a representative batch: the
-                // only iteration of a single-pass mesh, or the first refinement iteration of an
-                // adaptive mesh with at least 65536 pixels (its pixels follow the image's edges,
-                // unlike the uniform initial grid)
+                // Tuning by batch size: geodesic lengths grow more uneven as an adaptive mesh
+                // refines (the later, larger iterations hug the image's edges), so the options
+                // are chosen again whenever a batch of at least 65536 pixels reaches a larger
+                // power of two than any before (the uniform initial grid of an adaptive mesh
+                // excepted).  The choice per size is kept in the tune cache; a new one is timed
+                // on the whole batch (at most its middle 2^21 pixels): a part of a batch does not
+                // show its tail of long geodesics, which decides between the schedules.
                 const bool adaptive = rc.mesh.type == FOORT_MESH_SQUARESUBDIVISION ||
                                       rc.mesh.type == FOORT_MESH_SQUARESUBDIVISIONV2;
-                if (rc.autotune && !tuned && (!adaptive || (iteration > 0 && n >= 65536))) {
-                    tuned = true;
-                    const uint64_t stride = std::max<uint64_t>(1, n / 65536);
-                    std::vector<uint64_t> sample;
-                    for (uint64_t i = 0; i < n; i += stride)
-                        sample.insert(sample.end(), {idx[2 * i], idx[2 * i + 1]});
-                    foort_run_desc chosen;
-                    foort_stats tst{};
-                    const std::string key = tune_key(rc.desc, rc.autotune);
-                    if (foort_gpu_autotune(gpu, side, sample.size() / 2, sample.data(),
-                                           rc.autotune == 2 ? FOORT_TUNE_SCHEDULE : 0u, &chosen, &tst) == FOORT_OK) {
-                        tune_store(key, chosen);
-                        screen_output("Autotuner: " + std::string(chosen.schedule == FOORT_SCHEDULE_ONESHOT
-                                                                      ? "one-shot schedule, block size " +
-                                                                            std::to_string(chosen.block_size)
-                                                                      : "persistent schedule, refill every " +
-                                                                            std::to_string(chosen.refill_interval) +
-                                                                            " steps once " +
-                                                                            std::to_string(chosen.refill_lanes) +
-                                                                            " lanes are idle") +
-                                          " (" + std::to_string(tst.device_ms) + " ms on " +
-                                          std::to_string(sample.size() / 2) + " pixels).",
-                                      Level::Subproc);
-                    } else
-                        screen_output(std::string("Autotuner failed: ") + foort_gpu_last_error(gpu), Level::Warning);
+                const int size_class = n >= 65536 ? 63 - __builtin_clzll(n) : -1;
+                if (rc.autotune && size_class > tuned_class && (!adaptive || iteration > 0)) {
+                    tuned_class = size_class;
+                    const std::string key =
+                        tune_key(rc.desc, rc.autotune) + " size=2^" + std::to_string(size_class);
+                    foort_run_desc chosen = rc.desc;
+                    if (tune_lookup(key, chosen)) {
+                        if (foort_gpu_configure(gpu, &chosen) == FOORT_OK)
+                            screen_output("Autotuner: options of an earlier run for batches of 2^" +
+                                              std::to_string(size_class) + " pixels (" + tune_cache_path() + ").",
+                                          Level::Subproc);
+                    } else {
+                        const uint64_t w = std::min<uint64_t>(n, uint64_t(1) << 21), a = (n - w) / 2;
+                        foort_stats tst{};
+                        if (foort_gpu_autotune(gpu, side, w, idx.data() + 2 * a,
+                                               rc.autotune == 2 ? FOORT_TUNE_SCHEDULE : 0u, &chosen, &tst) ==
+                            FOORT_OK) {
+                            tune_store(key, chosen);
+                            screen_output(
+                                "Autotuner: " +
+                                    std::string(chosen.schedule == FOORT_SCHEDULE_ONESHOT
+                                                    ? "one-shot schedule, block size " +
+                                                          std::to_string(chosen.block_size)
+                                                    : "persistent schedule, refill every " +
+                                                          std::to_string(chosen.refill_interval) + " steps once " +
+                                                          std::to_string(chosen.refill_lanes) + " lanes are idle") +
+                                    " (" + std::to_string(tst.device_ms) + " ms on " + std::to_string(w) +
+                                    " pixels of a batch of " + std::to_string(n) + ").",
+                                Level::Subproc);
+                        } else
+                            screen_output(std::string("Autotuner failed: ") + foort_gpu_last_error(gpu),
+                                          Level::Warning);
+                    }
                 }
                 foort_stats st{};
                 rcode = grid ? foort_gpu_trace_grid(gpu, side, 0, n, &view, &st)

@@ -68,3 +68,23 @@ def test_schwarzschild_radial_infall_known_answer():
         out = _rhs(L, d, 2, [0.0, r, 1.0, 0.0], v)
         assert abs(out[1]) < 1e-15
         assert out[0] == pytest.approx(-2 * v[0] * v[1] / (r * r * f), rel=1e-14)
+
+
+def test_exp_of_the_analytic_path_within_two_ulp():
+    """The table-free binary64 exponential of the analytic formulations (Cody-Waite reduction,
+    Taylor polynomial to r^13) against numpy's exp: at most 2 ulp over the arguments the paths
+    use (log r of the camera radius and below, the Manko-Novikov potentials' exponents), and
+    the special values."""
+    L = C.CDLL(LIB)
+    rng = np.random.default_rng(3)
+    x = np.concatenate([rng.uniform(-40.0, 12.0, 200000), rng.uniform(-700.0, 700.0, 20000),
+                        np.log(np.array([1.0, 2.0, 1000.0, 1.0 + 1e-12])), [0.0, -0.0, 1e-300, -1e-300]])
+    out = np.zeros_like(x)
+    L.path_exp(x.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p), C.c_uint64(len(x)))
+    ref = np.exp(x)
+    ulp = np.abs(out - ref) / np.spacing(ref)
+    assert ulp.max() <= 2.0, (ulp.max(), x[ulp.argmax()])
+    sp = np.array([800.0, np.inf, -800.0, -np.inf, np.nan])
+    o = np.zeros_like(sp)
+    L.path_exp(sp.ctypes.data_as(C.c_void_p), o.ctypes.data_as(C.c_void_p), C.c_uint64(len(sp)))
+    assert o[0] == np.inf and o[1] == np.inf and o[2] == 0.0 and o[3] == 0.0 and np.isnan(o[4])

@@ -133,3 +133,36 @@ def test_server_requests_match_direct_runs():
         assert (r.pixels, r.steps) == (pixels, steps), (k, info)
         assert r.end_s >= r.start_s >= r.submit_s
         assert r.kernel_ms > 0
+
+
+TUNED = """
+Metric = {{ Name = "Kerr"; a = 0.9; RLogScale = true; }};
+Diagnostics = {{ FourColorScreen = {{ On = true; UseForMesh = true; }}; EquatorialPasses = {{ On = true; }}; }};
+Terminations = {{ Horizon = {{ On = true; }}; BoundarySphere = {{ On = true; SphereRadius = 1000.0; }};
+                 TimeOut = {{ On = true; MaxSteps = 100000; }}; }};
+ViewScreen = {{ Position = {{ r = 1000.0; theta = 1.2; }}; ScreenSize = {{ x = 24.0; y = 24.0; }};
+               Mesh = {{ Type = "SimpleSquareMesh"; TotalPixels = 90000; }}; }};
+Integrator = {{ Type = "RK4"; }};
+Output = {{ FilePrefix = "unused"; FileExtension = "dat"; TimeStamp = false; ScreenOutputLevel = 0; }};
+GPU = {{ Autotune = "{tune}"; UseGraph = true; }};
+"""
+
+
+def test_autotuned_program_writes_the_same_files(tmp_path, monkeypatch):
+    """Tuning by batch size (batches of at least 65536 pixels; here one 300 x 300 grid) changes only
+    how the batches run: the tuned run, the run that reuses the cached choice and an untuned run
+    write the same files; the cache holds one entry per batch size tuned."""
+    print("[program] autotuned run", flush=True)
+    cache = tmp_path / "tune.txt"
+    monkeypatch.setenv("FOORT_TUNE_CACHE", str(cache))
+    runs = {}
+    for name, tune in (("tuned", "Schedule"), ("cached", "Schedule"), ("plain", "None")):
+        cfg = tmp_path / f"{name}.cfg"
+        cfg.write_text(TUNED.format(tune=tune))
+        st = P.run(str(cfg), output_prefix=str(tmp_path / name / "run"))
+        runs[name] = _tree(tmp_path / name)
+        print(f"[program] {name}: {st.geodesics} pixels, {st.device_ms:.0f} ms", flush=True)
+    assert runs["tuned"] == runs["cached"] == runs["plain"]
+    entries = cache.read_text().splitlines()
+    assert entries and all("size=2^16" in e for e in entries)
+    assert len(entries) == len({e.split(" | ")[0] for e in entries})  # the second run only read the cache
