@@ -382,6 +382,56 @@ FM_HD_INLINE void mn_dd_from(const MetricParams &M, const MNCommon<T> &c, T sint
     g[0] = g00; g[1] = g03; g[2] = g11; g[3] = g22; g[4] = g33;
 }
 
+// The same potentials, restructured for the analytic formulations (not the reference's
+// arithmetic): one reciprocal of R for all its powers, the Legendre polynomials of
+// w = xy/R from w and w^2, the two series in 1/R in Horner form, exp(-2 psi) as
+// 1 / exp(2 psi).  Algebraically identical to mn_common.
+template <class T>
+FM_HD_INLINE void mn_common_an(const MetricParams &M, T u, T sint, T cost, MNCommon<T> &c)
+{
+    const double al = M.mn_alpha, kk = M.mn_k, a3 = M.alpha3;
+    T r = M.rlog ? fexp(u) : u;
+    T xx = (r - 1.) * (1. / kk);
+    T yy = cost;
+    T x2 = xx * xx, y2 = yy * yy;
+    T R = fsqrt(x2 + y2 - 1.);
+    T iR = 1. / R;
+    T w = xx * yy * iR;
+    T w2 = w * w;
+    T P1 = w;
+    T P2 = 0.5 * (3. * w2 - 1.);
+    T P3 = 0.5 * w * (5. * w2 - 3.);
+    T P4 = 0.125 * ((35. * w2 - 30.) * w2 + 3.);
+    // 1/R + P1/R^2 + P2/R^3 + P3/R^4 and 1/R - P1/R^2 + P2/R^3 - P3/R^4, i.e. (even +- odd) / R
+    // with even = 1 + P2/R^2 and odd = (P1 + P3/R^2)/R
+    T iR2 = iR * iR;
+    T even = P2 * iR2 + 1.;
+    T odd = (P3 * iR2 + P1) * iR;
+    T s_m = (even + odd) * iR;
+    T s_p = (even - odd) * iR;
+    T aa = -al * fexp(2. * a3 * (1. - (xx - yy) * s_m));
+    T bb = al * fexp(2. * a3 * (-1. + (xx + yy) * s_p));
+    T ab = aa * bb;
+    T x2m1 = x2 - 1., omy2 = 1. - y2;
+    T bma = bb - aa;
+    T AA = x2m1 * fpow2(1. + ab) - omy2 * fpow2(bma);
+    T BB = fpow2(xx + 1. + (xx - 1.) * ab) + fpow2((1. + yy) * aa + (1. - yy) * bb);
+    T CC = x2m1 * (1. + ab) * (bma - yy * (aa + bb)) + omy2 * bma * (1. + ab + xx * (1. - ab));
+    T iR4 = iR2 * iR2;
+    T psi = a3 * P3 * iR4;
+    T gamma_prime = 2. * a3 * a3 * (P4 * P4 - P3 * P3) * (iR4 * iR4) +
+                    2. * a3 * (-yy + (xx * P1 + (-yy * P2 + xx * P3 * iR) * iR) * iR) * iR;
+    T e2psi = fexp(2. * psi);
+    const double ial2 = 1. / ((1. - al * al) * (1. - al * al));
+    c.f = e2psi * AA / BB;
+    c.exp_2gamma = fexp(2. * gamma_prime) * AA * ial2 / (x2 - y2);
+    c.omega = 2. * kk * CC / (e2psi * AA) - 4. * kk * al / (1. - al * al);
+    c.rho_sq = (r - 1.) * (r - 1.) - kk * kk * cost * cost;
+    c.delta = (r - 1.) * (r - 1.) - kk * kk;
+    c.r = r;
+    c.sint = sint;
+}
+
 template <class T>
 FM_HD_INLINE void mn_dd(const MetricParams &M, T u, T sint, T cost, T *g)
 {
@@ -605,7 +655,7 @@ FM_HD_INLINE void rhs_an(const RunConst &R, const T *p, const T *v, T *out)
         // g_dd and g^uu share one evaluation of the Manko-Novikov potentials
         // (their most expensive part); g^uu needs only its values
         MNCommon<D2<S>> cd;
-        mn_common(R.M, U, ST, CT, cd);
+        mn_common_an(R.M, U, ST, CT, cd);
         mn_dd_from(R.M, cd, ST, g);
         const MNCommon<S> cv{cd.r.v, cd.sint.v, cd.f.v, cd.exp_2gamma.v, cd.omega.v, cd.rho_sq.v, cd.delta.v};
         mn_uu_from(R.M, cv, s, uu);
