@@ -323,7 +323,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
-              f"the sharding design is described in DESIGN.md", file=sys.stderr)
+              f"pixels are independent, so the path would shard by interleaved pixel index, one process per GPU",
+              file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "c2":

@@ -421,6 +421,8 @@ int run(const char *cfg_path, const foort_program_options &o)
         t = std::chrono::steady_clock::now();
         mesh->EndCurrentLoop();
         t_refine += lap(t);
+        if (!mesh->IsFinished())
+            out.FormatAhead(); // this iteration's lines, formatted in the background while the next one runs
         ++iteration;
     }
     const double tt = seconds_since(t_total);

@@ -6,6 +6,7 @@
 #pragma once
 
 #include <cstdint>
+#include <future>
 #include <memory>
 #include <string>
 #include <unordered_map>
@@ -117,6 +118,9 @@ public:
     enum Kind : char { kInt = 1, kFixed = 2 };
     void setValueKinds(std::vector<char> kinds) { m_kinds = std::move(kinds); }
     void NewGeodesicValues(uint64_t index, uint64_t row, uint64_t col, const double *values);
+    // Start formatting the lines of the entries given since the last call, in the background
+    // (the entries are copied); WriteCached writes what is ready.  Values entries only.
+    void FormatAhead();
     void OutputFinished();
     std::string getFullDescriptionStr() const;
     bool writesToConsole() const { return m_console; }
@@ -147,6 +151,13 @@ private:
     std::vector<Entry> m_cache;
     std::vector<std::vector<std::string>> m_texts; // texts given through NewGeodesicOutput
     std::vector<char> m_kinds;
+    // lines formatted ahead: cache entries [lo, hi), one text per diagnostic
+    struct Ahead {
+        uint64_t lo, hi;
+        std::future<std::vector<std::string>> text;
+    };
+    std::vector<Ahead> m_ahead;
+    uint64_t m_ahead_end = 0;
 };
 
 std::string set_error(const std::string &msg); // records the thread's last host error

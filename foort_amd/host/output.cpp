@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <charconv>
 #include <cstdio>
+#include <memory>
 #include <filesystem>
 #include <iostream>
 #include <limits>
@@ -74,6 +75,56 @@ void OutputHandler::NewGeodesicValues(uint64_t index, uint64_t row, uint64_t col
     e.text = kNoText;
 }
 
+namespace {
+// "row col  value\n" for entries of values: std::to_string's text of an int or of a double ("%f")
+void append_line(std::string &b, uint64_t row, uint64_t col, double v, char kind)
+{
+    char num[512];
+    b.append(num, std::to_chars(num, num + 48, row).ptr);
+    b += ' ';
+    b.append(num, std::to_chars(num, num + 48, col).ptr);
+    b += "  ";
+    if (kind == 1)
+        b.append(num, std::to_chars(num, num + sizeof num, static_cast<int>(v)).ptr);
+    else
+        b.append(num, std::to_chars(num, num + sizeof num, v, std::chars_format::fixed, 6).ptr);
+    b += '\n';
+}
+} // namespace
+
+void OutputHandler::FormatAhead()
+{
+    const uint64_t lo = m_ahead_end, hi = m_cache.size();
+    if (m_console || hi <= lo || m_kinds.empty())
+        return;
+    for (uint64_t j = lo; j < hi; ++j)
+        if (m_cache[j].text != kNoText)
+            return; // texts: written by WriteCached itself
+    struct Line {
+        uint64_t row, col;
+        double values[kMaxValues];
+    };
+    auto lines = std::make_shared<std::vector<Line>>(hi - lo);
+    for (uint64_t j = lo; j < hi; ++j) {
+        const Entry &e = m_cache[j];
+        Line &l = (*lines)[j - lo];
+        l.row = e.row;
+        l.col = e.col;
+        std::copy(e.values, e.values + kMaxValues, l.values);
+    }
+    const std::vector<char> kinds = m_kinds;
+    m_ahead.push_back(Ahead{lo, hi, std::async(std::launch::async, [lines, kinds] {
+                                std::vector<std::string> out(kinds.size());
+                                for (size_t d = 0; d < kinds.size(); ++d) {
+                                    out[d].reserve(lines->size() * 24);
+                                    for (const Line &l : *lines)
+                                        append_line(out[d], l.row, l.col, l.values[d], kinds[d]);
+                                }
+                                return out;
+                            })});
+    m_ahead_end = hi;
+}
+
 void OutputHandler::AppendText(std::string &b, const Entry &e, int d) const
 {
     if (e.text != kNoText) {
@@ -129,6 +180,75 @@ void OutputHandler::OpenForFirstTime(const std::string &name)
 // WriteCachedOutputToFile, FOORT/InputOutput.cpp:129-267
 void OutputHandler::WriteCached()
 {
+    // Lines formatted ahead cover a prefix of the cache.  They are used when the rest can be
+    // formatted the same way (values, no texts) and every line of this write goes to the
+    // current files (no split into a next file); otherwise the lines are formatted below.
+    bool values_only = true;
+    for (uint64_t j = m_ahead_end; j < m_cache.size() && values_only; ++j)
+        values_only = m_cache[j].text == kNoText;
+    if (!m_cache.empty() && !m_console && !m_ahead.empty() && values_only && m_in_file + m_cache.size() <= m_per_file) {
+        {
+            screen_output("Writing cached geodesic output to file(s)...", Level::Subproc);
+            const int ndiag = static_cast<int>(m_kinds.size());
+            std::vector<std::vector<std::string>> texts;
+            for (Ahead &a : m_ahead)
+                texts.push_back(a.text.get());
+            // the entries not formatted ahead, in parallel chunks
+            const uint64_t lo = m_ahead_end, n = m_cache.size() - lo, kChunk = 1u << 16;
+            const long long chunks = static_cast<long long>((n + kChunk - 1) / kChunk);
+            std::vector<std::vector<std::string>> rest(static_cast<size_t>(chunks),
+                                                       std::vector<std::string>(static_cast<size_t>(ndiag)));
+#pragma omp parallel for schedule(dynamic, 1)
+            for (long long c = 0; c < chunks; ++c) {
+                const uint64_t a = lo + static_cast<uint64_t>(c) * kChunk, b = std::min<uint64_t>(a + kChunk, lo + n);
+                for (int d = 0; d < ndiag; ++d) {
+                    std::string &t = rest[static_cast<size_t>(c)][static_cast<size_t>(d)];
+                    t.reserve((b - a) * 24);
+                    for (uint64_t j = a; j < b; ++j)
+                        append_line(t, m_cache[j].row, m_cache[j].col, m_cache[j].values[d], m_kinds[d]);
+                }
+            }
+            for (auto &r : rest)
+                texts.push_back(std::move(r));
+            m_ahead.clear();
+            m_ahead_end = 0;
+            if (m_in_file == 0)
+                for (int i = 0; i < ndiag && !m_console; ++i)
+                    OpenForFirstTime(FileName(i, static_cast<unsigned short>(m_full_files + 1)));
+            for (int d = 0; d < ndiag && !m_console; ++d) {
+                const std::string name = FileName(d, static_cast<unsigned short>(m_full_files + 1));
+                std::FILE *f = std::fopen(name.c_str(), "ab");
+                if (!f) {
+                    screen_output("Output file error! Could not open " + name +
+                                      ". Will write rest of output to console.",
+                                  Level::Warning);
+                    m_console = true;
+                    break;
+                }
+                for (const auto &t : texts)
+                    std::fwrite(t[static_cast<size_t>(d)].data(), 1, t[static_cast<size_t>(d)].size(), f);
+                std::fclose(f);
+            }
+            if (!m_console) {
+                // the bookkeeping of the general path below for a write into one file
+                unsigned long last_count = m_cache.size();
+                if (last_count == m_per_file) {
+                    ++m_full_files;
+                    last_count = 0;
+                }
+                m_in_file += last_count;
+                screen_output("Done writing cached geodesic output to file(s).", Level::Subproc);
+                m_cache.clear();
+                m_texts.clear();
+                return;
+            }
+            // a file could not be opened: everything goes to the console below
+        }
+    }
+    for (Ahead &a : m_ahead)
+        a.text.wait(); // not used: finish reading the copies before they go
+    m_ahead.clear();
+    m_ahead_end = 0;
     if (m_cache.empty())
         return;
     if (!m_console) {
