@@ -1,8 +1,9 @@
 // foort_gpu_an.hip -- kernels with analytic Christoffel symbols (exact
 // derivatives of the reference metrics by forward-mode dual numbers), in
 // binary64 or mixed binary32/binary64.  These are not bit-compatible with the
-// reference's finite differences, so this unit is compiled with contraction
-// allowed (fused multiply-add).
+// reference's finite differences, so this unit is compiled with the multiply-adds
+// of each expression fused (-ffp-contract=on: the same source rounds the same way
+// in every kernel).
 #include "foort_kernels.hpp"
 
 namespace foort {

@@ -753,10 +753,12 @@ FM_HD_INLINE double adaptive_step(const RunConst &R, const double *pos, const do
 // formulations that are not bit-compatible with the reference anyway.
 FM_HD_INLINE double adaptive_step_fast(const RunConst &R, const double *pos, const double *vel)
 {
+    // eps / (|v1|+d + (|v2|+d)/m + |v3+d|), multiplied through by m: one division (at the
+    // poles, m = 0, both forms give 0 and so the smallest step)
     const double m = ref_min(pos[2], kPi - pos[2]);
-    const double den = (__builtin_fabs(vel[1]) + 1e-20) + (__builtin_fabs(vel[2]) + 1e-20) / m +
-                       __builtin_fabs(vel[3] + 1e-20);
-    return ref_max(R.eps / den, R.minstep);
+    const double den = ((__builtin_fabs(vel[1]) + 1e-20) + __builtin_fabs(vel[3] + 1e-20)) * m +
+                       (__builtin_fabs(vel[2]) + 1e-20);
+    return ref_max(R.eps * m / den, R.minstep);
 }
 
 template <class RHS>
@@ -794,7 +796,9 @@ FM_HD_INLINE void rk4_step(const RunConst &R, const double *x, const double *v, 
         l4[i] = v[i] + k3[i] * h;
     }
     RHS::eval(R, tp, l4, k4);
-    const double h6 = h / 6.0;
+    // the reference divides (FOORT/Integrators.cpp:91-92); the analytic formulations multiply by
+    // the reciprocal, one instruction instead of a correctly rounded division
+    const double h6 = RHS::exact ? h / 6.0 : h * (1.0 / 6.0);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         nv[i] = v[i] + (((k1[i] + k2[i] * 2) + k3[i] * 2) + k4[i]) * h6;
@@ -845,7 +849,7 @@ FM_HD_INLINE void rk4_step_mixed(const RunConst &R, const double *x, const doubl
         l2[i] = vf[i] + k3[i] * hf; // l4
     }
     rhs_analytic<METRIC, float, float>(R, tp, l2, k4);
-    const float h6 = static_cast<float>(h / 6.0), hsq6 = static_cast<float>(h * h / 6.0);
+    const float h6 = static_cast<float>(h * (1.0 / 6.0)), hsq6 = static_cast<float>(h * h * (1.0 / 6.0));
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         nv[i] = v[i] + static_cast<double>((((k1[i] + k2[i] * 2.0f) + k3[i] * 2.0f) + k4[i]) * h6);

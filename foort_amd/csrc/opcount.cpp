@@ -53,7 +53,8 @@ static void reset() { cnt::add = cnt::mul = cnt::div = cnt::sqrt_ = cnt::trans =
 // adaptive step: the reference form (FOORT/Integrators.cpp:34-47: 3 divisions,
 // 3 reciprocals, 3 adds inside the fabs terms, 1 product, 1 sub, 2 adds and a
 // final reciprocal = 14) for the finite-difference path, the equivalent
-// two-division form (1 sub, 3 adds, 1 division, 2 adds, 1 division = 8) for
+// one-division form (1 sub, 3 adds inside the fabs terms and the sum, 2 products, 1 add,
+// 1 division = 8) for
 // the analytic ones.
 constexpr long long kRK4Book = 48 + 48 + 1;
 constexpr long long kStepRef = 14, kStepFast = 8;

@@ -560,3 +560,25 @@ def test_program_writes_value_texts_like_to_string(tmp_path):
     assert cr == [f"{r} {c}  {v:f}" for (r, c), v in zip(rows, vals)]
     ep = (tmp_path / "v_EquatPasses.dat").read_text().splitlines()
     assert ep == [f"{r} {c}  {i - 7}" for i, (r, c) in enumerate(rows)]
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is present: the requests would run")
+def test_server_without_gpu_returns_failed_requests():
+    """Failure handling on a machine without a GPU: every request comes back with an error
+    status (its group's device queue cannot be opened), drain and shutdown do not hang, and a
+    request for GeodesicPosition or a pixel-list mesh is refused at submission."""
+    reqs = W.c4_requests(seed=5, n=6, rate=1000.0)
+    with P.Server(workers=2, hbm_budget_bytes=16 << 20) as srv:
+        for k, (_, desc, mesh, _) in enumerate(reqs):
+            srv.submit(desc, mesh, k)
+        srv.drain()
+        res = srv.results()
+        assert sorted(r.tag for r in res) == list(range(6))
+        assert all(r.status != 0 for r in res)
+        d = reqs[0][1]
+        d.diag_flags |= A.DIAGFLAG_GEODESICPOSITION
+        with pytest.raises(P.HostError):
+            srv.submit(d, reqs[0][2], 99)
+        m = P.mesh_desc(P.MESH_INPUTCERTAINPIXELS, A.DIAGFLAG_FOURCOLORSCREEN, total_pixels=100)
+        with pytest.raises(P.HostError):
+            srv.submit(reqs[1][1], m, 98)
