@@ -466,26 +466,28 @@ class SquareSubdivisionMeshV2 final : public Mesh {
 public:
     explicit SquareSubdivisionMeshV2(const foort_mesh_desc &d) : Mesh(d.value_diag), S(d), m_left(d.max_pixels)
     {
+        if (S.side >= (uint64_t(1) << 32))
+            throw std::runtime_error("SquareSubdivisionMeshV2: row/column size above 2^32");
         // InitializeFirstGrid, FOORT/Mesh.cpp:726-790
         const uint64_t n = isqrt_trunc(S.initial);
         const uint64_t step = exp2i(S.max_sub - 1);
-        m_px.reserve(S.infinite ? S.initial : std::min<uint64_t>(std::max(S.max_pixels, S.initial) + 5 * S.iteration,
-                                                                 uint64_t(1) << 26));
+        reserve(S.infinite ? S.initial : std::min<uint64_t>(std::max(S.max_pixels, S.initial) + 5 * S.iteration,
+                                                            uint64_t(1) << 26));
         for (uint64_t row = 0; row < n; ++row)
             for (uint64_t col = 0; col < n; ++col)
-                m_px.push_back(Pixel{step * row, step * col, (row == n - 1 || col == n - 1) ? S.max_sub : 1});
+                push(step * row, step * col, (row == n - 1 || col == n - 1) ? S.max_sub : 1);
         for (uint64_t row = 0; row + 1 < n; ++row)
             for (uint64_t col = 0; col + 1 < n; ++col) {
-                const int64_t p = static_cast<int64_t>(row * n + col);
+                const int32_t p = static_cast<int32_t>(row * n + col);
                 link_right(p, p + 1);
-                link_down(p, p + static_cast<int64_t>(n));
-                m_px[p].se = p + static_cast<int64_t>(n) + 1;
+                link_down(p, p + static_cast<int32_t>(n));
+                m_nd[p].se = p + static_cast<int32_t>(n) + 1;
             }
         m_queue.reserve(S.initial);
         for (uint64_t i = 0; i < n * n; ++i) {
-            m_queue.push_back(static_cast<int64_t>(i));
-            if (m_px[i].sub < S.max_sub)
-                m_updating.push_back(static_cast<int64_t>(i));
+            m_queue.push_back(static_cast<int32_t>(i));
+            if (m_nd[i].sub < S.max_sub)
+                m_updating.push_back(static_cast<int32_t>(i));
         }
         if (!S.infinite)
             m_left -= m_queue.size();
@@ -495,12 +497,12 @@ public:
     uint64_t getCurNrGeodesics() const override { return m_queue.size(); }
     void getNewInitConds(uint64_t index, uint64_t &row, uint64_t &col) const override
     {
-        row = m_px[m_queue[index]].row;
-        col = m_px[m_queue[index]].col;
+        row = m_rc[m_queue[index]].row;
+        col = m_rc[m_queue[index]].col;
     }
     void GeodesicFinished(uint64_t index, MeshValue value) override
     {
-        m_px[m_queue[index]].value = value;
+        m_val[m_queue[index]] = value;
         m_done[index] = 1;
     }
     bool IsFinished() const override { return m_queue.empty(); }
@@ -525,11 +527,22 @@ public:
         screen_output("Calculating pixels to subdivide next...", Level::Subproc);
         if (S.infinite || m_left > 0) {
             update_weights();
-            std::sort(m_active.begin(), m_active.end(), [this](int64_t a, int64_t b) {
-                if (m_px[a].weight > m_px[b].weight)
+            // sorted as (weight, level, pixel) records with the reference's comparison: std::sort
+            // makes the same comparisons as on the pixel indices, so the order is the same
+            struct Key {
+                double weight;
+                int32_t sub, px;
+            };
+            std::vector<Key> keys(m_active.size());
+            for (size_t i = 0; i < keys.size(); ++i)
+                keys[i] = Key{m_w[m_active[i]], m_nd[m_active[i]].sub, m_active[i]};
+            std::sort(keys.begin(), keys.end(), [](const Key &a, const Key &b) {
+                if (a.weight > b.weight)
                     return true;
-                return m_px[a].weight == m_px[b].weight && m_px[a].sub < m_px[b].sub;
+                return a.weight == b.weight && a.sub < b.sub;
             });
+            for (size_t i = 0; i < keys.size(); ++i)
+                m_active[i] = keys[i].px;
             m_queue.reserve(5 * S.iteration);
             const uint64_t lim = std::min<uint64_t>(m_active.size(), S.iteration);
             for (uint64_t i = 0; i < lim; ++i)
@@ -550,28 +563,51 @@ public:
     }
 
 private:
-    struct Pixel {
-        uint64_t row, col;
-        int sub;
-        double weight = -1;
-        MeshValue value{};
-        int64_t left = -1, right = -1, up = -1, down = -1, se = -1;
+    // a pixel in four arrays: its neighbour links and subdivision level (walked by the
+    // neighbour searches), its screen index, its weight and its value
+    struct Node {
+        int32_t left = -1, right = -1, up = -1, down = -1, se = -1;
+        int32_t sub;
+    };
+    struct RC {
+        uint32_t row, col;
     };
 
     SubdivisionShape S;
     uint64_t m_left;
     uint64_t m_integrated = 0;
-    std::vector<Pixel> m_px;                       // every pixel ever created
-    std::vector<int64_t> m_active, m_queue, m_updating;
+    std::vector<Node> m_nd;                        // every pixel ever created
+    std::vector<RC> m_rc;
+    std::vector<double> m_w;
+    std::vector<MeshValue> m_val;
+    std::vector<int32_t> m_active, m_queue, m_updating;
     std::vector<char> m_done;
 
-    void link_right(int64_t a, int64_t b) { m_px[a].right = b, m_px[b].left = a; }
-    void link_down(int64_t a, int64_t b) { m_px[a].down = b, m_px[b].up = a; }
-
-    int64_t add(uint64_t row, uint64_t col, int sub)
+    void reserve(uint64_t n)
     {
-        m_px.push_back(Pixel{row, col, sub});
-        const int64_t p = static_cast<int64_t>(m_px.size()) - 1;
+        m_nd.reserve(n);
+        m_rc.reserve(n);
+        m_w.reserve(n);
+        m_val.reserve(n);
+    }
+    int32_t push(uint64_t row, uint64_t col, int sub)
+    {
+        if (m_nd.size() >= static_cast<size_t>(INT32_MAX))
+            throw std::runtime_error("SquareSubdivisionMeshV2: more than 2^31 - 1 pixels");
+        Node nd;
+        nd.sub = sub;
+        m_nd.push_back(nd);
+        m_rc.push_back(RC{static_cast<uint32_t>(row), static_cast<uint32_t>(col)});
+        m_w.push_back(-1);
+        m_val.push_back(MeshValue{});
+        return static_cast<int32_t>(m_nd.size()) - 1;
+    }
+    void link_right(int32_t a, int32_t b) { m_nd[a].right = b, m_nd[b].left = a; }
+    void link_down(int32_t a, int32_t b) { m_nd[a].down = b, m_nd[b].up = a; }
+
+    int32_t add(uint64_t row, uint64_t col, int sub)
+    {
+        const int32_t p = push(row, col, sub);
         m_queue.push_back(p);
         return p;
     }
@@ -582,12 +618,13 @@ private:
         screen_output("Updating pixel weights for " + std::to_string(static_cast<unsigned long>(m_updating.size())) +
                           " pixels...",
                       Level::AllDetail);
-        for (int64_t i : m_updating) {
-            Pixel &p = m_px[i];
-            std::array<double, 3> dist{distance(p.value, m_px[p.right].value), distance(p.value, m_px[p.down].value),
-                                       distance(p.value, m_px[p.se].value)};
-            p.weight = *std::max_element(dist.begin(), dist.end());
-            if (p.weight > 0.0)
+        for (int32_t i : m_updating) {
+            const Node &p = m_nd[i];
+            const MeshValue &v = m_val[i];
+            std::array<double, 3> dist{distance(v, m_val[p.right]), distance(v, m_val[p.down]),
+                                       distance(v, m_val[p.se])};
+            m_w[i] = *std::max_element(dist.begin(), dist.end());
+            if (m_w[i] > 0.0)
                 m_active.push_back(i);
         }
         m_updating.clear();
@@ -597,36 +634,36 @@ private:
     // GetUp / GetDown / GetLeft / GetRight, FOORT/Mesh.cpp:831-935: the neighbour at
     // subdivision level `sub`, walking a finer grid where the pixels are subdivided further.
     // Up and left test the neighbour's level, down and right the pixel's own.
-    int64_t get_up(int64_t p, int sub) const
+    int32_t get_up(int32_t p, int sub) const
     {
-        if (p < 0 || m_px[p].up < 0 || m_px[m_px[p].up].sub < sub)
+        if (p < 0 || m_nd[p].up < 0 || m_nd[m_nd[p].up].sub < sub)
             return -1;
-        if (m_px[m_px[p].up].sub == sub)
-            return m_px[p].up;
+        if (m_nd[m_nd[p].up].sub == sub)
+            return m_nd[p].up;
         return get_up(get_up(p, sub + 1), sub + 1);
     }
-    int64_t get_down(int64_t p, int sub) const
+    int32_t get_down(int32_t p, int sub) const
     {
-        if (p < 0 || m_px[p].down < 0 || m_px[p].sub < sub)
+        if (p < 0 || m_nd[p].down < 0 || m_nd[p].sub < sub)
             return -1;
-        if (m_px[p].sub == sub)
-            return m_px[p].down;
+        if (m_nd[p].sub == sub)
+            return m_nd[p].down;
         return get_down(get_down(p, sub + 1), sub + 1);
     }
-    int64_t get_left(int64_t p, int sub) const
+    int32_t get_left(int32_t p, int sub) const
     {
-        if (p < 0 || m_px[p].left < 0 || m_px[m_px[p].left].sub < sub)
+        if (p < 0 || m_nd[p].left < 0 || m_nd[m_nd[p].left].sub < sub)
             return -1;
-        if (m_px[m_px[p].left].sub == sub)
-            return m_px[p].left;
+        if (m_nd[m_nd[p].left].sub == sub)
+            return m_nd[p].left;
         return get_left(get_left(p, sub + 1), sub + 1);
     }
-    int64_t get_right(int64_t p, int sub) const
+    int32_t get_right(int32_t p, int sub) const
     {
-        if (p < 0 || m_px[p].right < 0 || m_px[p].sub < sub)
+        if (p < 0 || m_nd[p].right < 0 || m_nd[p].sub < sub)
             return -1;
-        if (m_px[p].sub == sub)
-            return m_px[p].right;
+        if (m_nd[p].sub == sub)
+            return m_nd[p].right;
         return get_right(get_right(p, sub + 1), sub + 1);
     }
 
@@ -635,57 +672,57 @@ private:
     //   4 5 6
     //   7 8 9
     // with 1 the subdivided pixel and 3, 7, 9 its right, down and diagonal neighbours.
-    void subdivide(int64_t p1)
+    void subdivide(int32_t p1)
     {
-        const int64_t p3 = m_px[p1].right, p7 = m_px[p1].down, p9 = m_px[p1].se;
-        const int sub = m_px[p1].sub + 1;
+        const int32_t p3 = m_nd[p1].right, p7 = m_nd[p1].down, p9 = m_nd[p1].se;
+        const int sub = m_nd[p1].sub + 1;
         const uint64_t d = exp2i(S.max_sub - sub);
-        const uint64_t row = m_px[p1].row, col = m_px[p1].col;
+        const uint64_t row = m_rc[p1].row, col = m_rc[p1].col;
         const bool more = sub < S.max_sub;
 
-        m_px[p1].sub = sub;
+        m_nd[p1].sub = sub;
         if (more)
             m_updating.push_back(p1);
 
-        int64_t p2 = get_down(get_right(get_up(p1, sub), sub), sub);
+        int32_t p2 = get_down(get_right(get_up(p1, sub), sub), sub);
         if (p2 < 0)
             p2 = add(row, col + d, sub);
         else
-            m_px[p2].sub = sub;
+            m_nd[p2].sub = sub;
         if (more)
             m_updating.push_back(p2);
 
-        int64_t p4 = get_right(get_down(get_left(p1, sub), sub), sub);
+        int32_t p4 = get_right(get_down(get_left(p1, sub), sub), sub);
         if (p4 < 0)
             p4 = add(row + d, col, sub);
         else
-            m_px[p4].sub = sub;
+            m_nd[p4].sub = sub;
         if (more)
             m_updating.push_back(p4);
 
-        const int64_t p5 = add(row + d, col + d, sub);
+        const int32_t p5 = add(row + d, col + d, sub);
         if (more)
             m_updating.push_back(p5);
 
-        int64_t p6 = get_down(p3, sub);
+        int32_t p6 = get_down(p3, sub);
         if (p6 < 0)
             p6 = add(row + d, col + 2 * d, 0);
-        int64_t p8 = get_right(p7, sub);
+        int32_t p8 = get_right(p7, sub);
         if (p8 < 0)
             p8 = add(row + 2 * d, col + d, 0);
 
         link_right(p1, p2);
         link_down(p1, p4);
-        m_px[p1].se = p5;
+        m_nd[p1].se = p5;
         link_right(p2, p3);
         link_down(p2, p5);
-        m_px[p2].se = p6;
+        m_nd[p2].se = p6;
         link_right(p4, p5);
         link_down(p4, p7);
-        m_px[p4].se = p8;
+        m_nd[p4].se = p8;
         link_right(p5, p6);
         link_down(p5, p8);
-        m_px[p5].se = p9;
+        m_nd[p5].se = p9;
     }
 };
 
