@@ -46,6 +46,7 @@ class RunDesc(C.Structure):
         ("mixed_switch_radius", C.c_double), ("mixed_pole_margin", C.c_double),
         ("launch_us", C.c_uint32), ("use_graph", C.c_uint32),
         ("gp_update", UpdateFrequency), ("gp_output_steps", C.c_uint64),
+        ("tail_geodesics", C.c_uint32), ("reserved", C.c_uint32),
     ]
 
 

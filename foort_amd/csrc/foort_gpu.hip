@@ -312,6 +312,16 @@ void apply_lane_options(foort_gpu_ctx *ctx)
     ctx->slots.ticks = static_cast<uint64_t>(ctx->desc.launch_us) * static_cast<uint64_t>(ctx->clock_khz) / 1000u;
     if (ctx->desc.launch_us && ctx->slots.ticks == 0)
         ctx->slots.ticks = 1;
+    // the four-lane tail: finite differences, a batch that is one launch (set in launch())
+    ctx->slots.tail = 0;
+}
+
+// geodesics left to the tail kernel (0: none)
+uint64_t tail_threshold(const foort_gpu_ctx *ctx)
+{
+    if (!ctx->kernels.tail || ctx->desc.tail_geodesics == UINT32_MAX)
+        return 0;
+    return ctx->desc.tail_geodesics ? ctx->desc.tail_geodesics : std::max<uint32_t>(1, ctx->slots.L / 8);
 }
 
 int setup_persistent(foort_gpu_ctx *ctx)
@@ -383,11 +393,20 @@ int launch(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, uint64_t count, co
     if (count == 0)
         return FOORT_OK;
     if (single_launch(ctx)) {
-        // no step or time limit: the launch runs until every pixel is done
+        // no step or time limit: the launch runs until every pixel is done, or until the last
+        // few are left to the four-lane tail kernel
+        LaneSlots St = S;
+        St.tail = tail_threshold(ctx);
         hipLaunchKernelGGL(ctx->kernels.persistent, dim3(S.L / 256), dim3(256), 0, stream, ctx->rc, side, first,
-                           count, si, out, ctx->d_steps, S, ctx->budget);
+                           count, si, out, ctx->d_steps, St, ctx->budget);
         HIP_TRY(ctx, hipGetLastError());
         ctx->last_launches = 1;
+        if (St.tail) {
+            hipLaunchKernelGGL(ctx->kernels.tail, dim3(S.L / 64), dim3(256), 0, stream, ctx->rc, side, first, count,
+                               si, out, ctx->d_steps, St, ctx->budget);
+            HIP_TRY(ctx, hipGetLastError());
+            ctx->last_launches = 2;
+        }
         return FOORT_OK;
     }
     // every launch advances each busy lane by at least min(budget, interval) steps (or to the end

@@ -6,6 +6,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "foort_path.hpp"
 
 // Occupancy targets (waves per SIMD) of the two schedules; they bound the
@@ -66,6 +68,8 @@ struct LaneSlots {
     uint32_t refill;           // refill idle lanes once at least this many of a wavefront are idle
     uint32_t interval;         // steps between refills (the length of the inner step loop)
     uint64_t ticks;            // wall-clock ticks after which a launch ends (0 = no time limit)
+    uint64_t tail;             // leave once every pixel is claimed and at most this many are unfinished,
+                               // for the four-lane tail kernel (0 = never)
 };
 
 // A ring of pixels from many requests that share one kernel and every run
@@ -263,6 +267,11 @@ trace_persistent(const RunConst R, uint64_t side, uint64_t first,
         if (!__any(pix >= 0) || it >= budget ||
             (S.ticks && static_cast<uint64_t>(wall_clock64()) - t0 >= S.ticks))
             break;
+        // the batch's last geodesics go to the tail kernel: once every pixel is claimed and at
+        // most `tail` are unfinished, every wavefront parks its lanes
+        if (S.tail && __atomic_load_n(S.next, __ATOMIC_RELAXED) >= count &&
+            __atomic_load_n(S.done, __ATOMIC_RELAXED) + S.tail >= count)
+            break;
         // up to `interval` steps: the step loop of the one-shot kernel
         const uint32_t n = budget - it < S.interval ? static_cast<uint32_t>(budget - it) : S.interval;
         for (uint32_t k = 0; k < n && g.term == FOORT_TERM_CONTINUE; ++k)
@@ -437,6 +446,92 @@ trace_gp(const RunConst R, const GPArgs A, uint64_t side, uint64_t first, uint64
     add_steps(total_steps, mine);
 }
 
+// ---------------------------------------------------------------------------
+// Four lanes per geodesic for the end of a batch.  When few geodesics are left
+// they run alone in their wavefronts, each step as long as its instruction stream.
+// A quad of lanes holds one geodesic (every lane the same state, so every
+// decision is the same); in the right-hand side each lane evaluates the metric at
+// one of the four finite-difference points and the quad exchanges the values
+// (DPP quad broadcasts).  Every arithmetic operation is the one rhs_fd performs,
+// so the results are bit for bit those of one lane.
+// ---------------------------------------------------------------------------
+template <int J>
+__device__ __forceinline__ double quad_bcast(double x)
+{
+    const long long b = __builtin_bit_cast(long long, x);
+    int lo = static_cast<int>(b), hi = static_cast<int>(b >> 32);
+    lo = __builtin_amdgcn_mov_dpp(lo, J * 0x55, 0xF, 0xF, false); // quad_perm [J, J, J, J]
+    hi = __builtin_amdgcn_mov_dpp(hi, J * 0x55, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, (static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+
+// rhs_fd with lane q of the quad at (u + h, th), (u - h, th), (u, th + h), (u, th - h)
+template <int METRIC>
+__device__ __forceinline__ void rhs_fd_quad(const RunConst &R, const double *p, const double *v, double *out)
+{
+    const int q = static_cast<int>(threadIdx.x & 3);
+    const double u = p[1], th = p[2], h = R.h_fd;
+    const double uq = q == 0 ? u + h : (q == 1 ? u - h : u);
+    const double tq = q == 2 ? th + h : (q == 3 ? th - h : th);
+    double s, c, g[5];
+    fm_sincos(tq, &s, &c);
+    metric_dd<METRIC, double>(R.M, uq, s, c, g);
+    double d1[5], d2[5], uu[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        d1[k] = (quad_bcast<0>(g[k]) - quad_bcast<1>(g[k])) * R.inv2h;
+        d2[k] = (quad_bcast<2>(g[k]) - quad_bcast<3>(g[k])) * R.inv2h;
+    }
+    metric_uu<METRIC, double>(R.M, u, quad_bcast<0>(s), quad_bcast<0>(c), uu);
+    geo_contract<double>(d1, d2, uu, v, out);
+}
+
+template <int METRIC>
+struct RhsFDQuad {
+    static constexpr int metric = METRIC;
+    static constexpr bool exact = true;
+    static constexpr bool split_rk4 = false;
+    __device__ __forceinline__ static void eval(const RunConst &R, const double *p, const double *v, double *out)
+    {
+        rhs_fd_quad<METRIC>(R, p, v, out);
+    }
+};
+
+// The quad version of a policy, where there is one
+template <class RHS>
+struct QuadOf {
+    using type = void;
+};
+template <int METRIC>
+struct QuadOf<RhsFD<METRIC>> {
+    using type = RhsFDQuad<METRIC>;
+};
+
+// Finish the geodesics parked in the lane slots, one quad of lanes each (grid: 4 L threads).
+template <class RHSQ, int INTEG, bool EVERY>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(waves_per_eu<RHSQ>(true), waves_per_eu<RHSQ>(true))))
+trace_tail(const RunConst R, uint64_t, uint64_t, uint64_t, const uint64_t *, DevResults out,
+           unsigned long long *__restrict__ total_steps, LaneSlots S, uint64_t)
+{
+    const uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const uint64_t l = t >> 2; // a quad per lane slot: the quad's lanes leave together
+    if (l >= S.L)
+        return;
+    const long long pix = S.pix[l];
+    if (pix < 0)
+        return;
+    GeoState g;
+    slot_load(S, static_cast<uint32_t>(l), g);
+    while (g.term == FOORT_TERM_CONTINUE)
+        geo_update<RHSQ, INTEG, EVERY>(R, g);
+    if ((t & 3) == 0) {
+        store_result(out, static_cast<uint64_t>(pix), g);
+        S.pix[l] = -1;
+        atomicAdd(total_steps, static_cast<unsigned long long>(g.steps));
+        atomicAdd(S.done, 1ull);
+    }
+}
+
 using GPKernelFn = void (*)(const RunConst, const GPArgs, uint64_t, uint64_t, uint64_t, const uint64_t *, DevResults,
                             unsigned long long *);
 
@@ -450,7 +545,17 @@ struct KernelPair {
     KernelFn persistent;
     QueueKernelFn queue;
     GPKernelFn gp;             // GeodesicPosition (all update frequencies general)
+    KernelFn tail;             // four lanes per geodesic (finite differences), or nullptr
 };
+
+template <class RHS, int INTEG, bool EVERY>
+constexpr KernelFn tail_kernel()
+{
+    if constexpr (std::is_void_v<typename QuadOf<RHS>::type>)
+        return nullptr;
+    else
+        return trace_tail<typename QuadOf<RHS>::type, INTEG, EVERY>;
+}
 
 // `every`: every termination and diagnostic updates on every step (the kernels
 // without update counters)
@@ -458,9 +563,10 @@ template <class RHS, int INTEG>
 KernelPair kernel_pair(bool every)
 {
     return every ? KernelPair{trace_oneshot<RHS, INTEG, true>, trace_persistent<RHS, INTEG, true>,
-                              trace_queue<RHS, INTEG, true>, trace_gp<RHS, INTEG>}
+                              trace_queue<RHS, INTEG, true>, trace_gp<RHS, INTEG>, tail_kernel<RHS, INTEG, true>()}
                  : KernelPair{trace_oneshot<RHS, INTEG, false>, trace_persistent<RHS, INTEG, false>,
-                              trace_queue<RHS, INTEG, false>, trace_gp<RHS, INTEG>};
+                              trace_queue<RHS, INTEG, false>, trace_gp<RHS, INTEG>,
+                              tail_kernel<RHS, INTEG, false>()};
 }
 
 template <template <int> class POLICY, int M>

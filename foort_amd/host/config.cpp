@@ -675,6 +675,9 @@ void get_gpu_options(const Setting &root, RunConfig &rc)
     n = d.launch_us;
     if (g->lookup_counter("LaunchMicroseconds", n))
         d.launch_us = static_cast<uint32_t>(n);
+    n = d.tail_geodesics;
+    if (g->lookup_counter("TailGeodesics", n))
+        d.tail_geodesics = static_cast<uint32_t>(std::min<unsigned long>(n, UINT32_MAX));
     bool b = d.use_graph != 0;
     if (g->lookup("UseGraph", b))
         d.use_graph = b;

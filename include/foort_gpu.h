@@ -142,6 +142,11 @@ typedef struct foort_run_desc {
     /* GeodesicPosition (FOORT/Diagnostics.h:331-337): its points come from foort_gpu_trace_positions */
     foort_update_frequency gp_update;  /* (1, false, true) */
     uint64_t gp_output_steps;    /* OutputSteps: points kept per geodesic (0 = all) */
+    uint32_t tail_geodesics;     /* persistent schedule, finite differences, a batch that is one launch:
+                                    once every pixel is started and at most this many are unfinished,
+                                    finish them with four lanes each (0 = an eighth of the resident
+                                    lanes; UINT32_MAX = never) */
+    uint32_t reserved;
 } foort_run_desc;
 
 /*

@@ -76,8 +76,13 @@ FIELDS = ("term", "quadrant", "equat_passes", "closest_radius", "final_pos", "fi
 
 
 def bitwise_equal(a, b):
+    """Equal bit for bit; two NaNs count as equal whatever their bits (an invalid operation
+    gives a NaN whose sign and payload IEEE 754 leaves open: x86 makes 0xfff8..., the GPU
+    0x7ff8..., so a geodesic that turns NaN -- past a pole with ThetaSingularity off, say --
+    does so on both sides with different bits)."""
     a = np.ascontiguousarray(a)
     b = np.ascontiguousarray(b)
     if a.dtype.kind == "f":
-        return np.array_equal(a.view(np.uint64), b.view(np.uint64))
+        same = a.view(np.uint64) == b.view(np.uint64)
+        return a.shape == b.shape and bool(np.all(same | (np.isnan(a) & np.isnan(b))))
     return np.array_equal(a.astype(np.int64), b.astype(np.int64))

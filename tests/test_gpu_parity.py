@@ -12,7 +12,7 @@ import pytest
 
 import oracle as O
 from foort_amd import abi as A
-from tests.cases import FIELDS, PARITY_CASES, bitwise_equal, c1
+from tests.cases import FIELDS, PARITY_CASES, bitwise_equal, c0, c1, johannsen_cfg
 
 pytestmark = pytest.mark.gpu
 
@@ -134,8 +134,39 @@ def test_persistent_schedule_bitexact(budget, interval, launch_us):
     bounded = budget == 37 or launch_us > 0
     assert g["stats"].launches > (1 if bounded else 0)
     if budget == 0 and launch_us == 0:
-        assert g["stats"].launches == 1
+        assert g["stats"].launches in (1, 2)  # one launch, and the four-lane tail kernel
     assert g["stats"].steps == o["steps"].sum()
+
+
+TAIL_CASES = {
+    "kerr_rlog_rk4": lambda **k: c1(rlog=True, max_steps=10000, **k),
+    "johannsen_verlet": lambda **k: johannsen_cfg(**k),
+    "rasheed_larsen": lambda **k: c0(metric=A.METRIC_RASHEEDLARSEN, params=(1.0, 0.5, 2.0, 3.0), rlog=True,
+                                     inclination_deg=60.0, max_steps=20000, **k),
+    "manko_novikov": lambda **k: c0(metric=A.METRIC_MANKONOVIKOV, params=(0.5, 2.5), rlog=True,
+                                    inclination_deg=70.0, max_steps=3000, **k),
+}
+
+
+@pytest.mark.parametrize("tail", [1, 40, 1 << 30, 2 ** 32 - 1])
+@pytest.mark.parametrize("name", sorted(TAIL_CASES))
+def test_persistent_tail_bitexact(name, tail):
+    """The batch's last geodesics are finished by quads of lanes (four lanes per geodesic, the
+    four finite-difference metric points evaluated one per lane and exchanged within the quad):
+    bit for bit the oracle, whether the tail kernel takes one geodesic, some, nearly all (the
+    threshold above the batch: it starts from the geodesics' first steps) or none."""
+    mk = TAIL_CASES[name]
+    desc = mk(schedule=A.SCHEDULE_PERSISTENT, tail_geodesics=tail)
+    side = 40
+    print(f"[tail] {name} tail={tail}", flush=True)
+    F = _gpu()
+    with F.Context(desc) as ctx:
+        g = ctx.trace_grid(side)
+    o = O.trace("fm", mk(), side)
+    for k in FIELDS:
+        assert bitwise_equal(g[k], o[k]), k
+    assert g["stats"].steps == o["steps"].sum()
+    assert g["stats"].launches == (1 if tail == 2 ** 32 - 1 else 2)
 
 
 def test_persistent_pixel_list_verlet():
