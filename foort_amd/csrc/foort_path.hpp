@@ -1075,6 +1075,36 @@ FM_HD_INLINE void geo_init(const RunConst &R, double ux, double uy, GeoState &g)
     diag_update<EVERY>(R, g);
 }
 
+// A geodesic whose position and velocity are all NaN (one that went through a pole with
+// ThetaSingularity off, say) stays NaN: every later step computes NaN from NaN, and no
+// termination or diagnostic condition holds for NaN (each is a comparison, false for NaN), so
+// the steps change nothing but the step counters until TimeOut counts out or the step cap is
+// reached (the reference takes those steps: such a geodesic ends by TimeOut).  When every
+// termination and diagnostic updates on every step, the counters' values are known in closed
+// form: jump to the step before the one that ends the geodesic, which then runs as usual.
+// Returns the steps skipped; the result is the one of stepping through.  (ClosestRadius: a
+// NaN radius is taken only while the kept value is negative, and then once is as many times.)
+FM_HD_INLINE uint32_t skip_nan_steps(const RunConst &R, GeoState &g)
+{
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        if (!(g.pos[i] != g.pos[i]) || !(g.vel[i] != g.vel[i]))
+            return 0;
+    uint64_t last = g.steps + 1ull > R.step_cap ? g.steps + 1ull : R.step_cap; // STEPCAP: steps >= step_cap
+    const bool timeout = (R.terms & FOORT_TERMFLAG_TIMEOUT) != 0;
+    if (timeout) {
+        // TimeOut ends the step on which tsteps >= max_steps holds on entry; each step before
+        // it adds one to tsteps
+        const uint64_t to = g.steps + 1ull + (g.tsteps >= R.max_steps ? 0ull : uint64_t(R.max_steps - g.tsteps));
+        last = to < last ? to : last;
+    }
+    const uint32_t skip = static_cast<uint32_t>(last - 1ull - g.steps);
+    g.steps += skip;
+    if (timeout)
+        g.tsteps += skip;
+    return skip;
+}
+
 // Geodesic::Update, FOORT/Geodesic.cpp:61-88
 template <class RHS, int INTEG, bool EVERY>
 FM_HD_INLINE void geo_update(const RunConst &R, GeoState &g)
@@ -1098,6 +1128,10 @@ FM_HD_INLINE void geo_update(const RunConst &R, GeoState &g)
         t = FOORT_TERM_STEPCAP;
     g.term = t;
     diag_update<EVERY>(R, g);
+    // lambda is NaN from the first NaN step size on, so one comparison a step covers the test
+    if constexpr (EVERY)
+        if (t == FOORT_TERM_CONTINUE && g.lambda != g.lambda)
+            skip_nan_steps(R, g);
 }
 
 } // namespace foort

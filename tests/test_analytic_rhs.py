@@ -57,6 +57,18 @@ def test_analytic_formulations_agree(name):
         assert np.abs(f32 - dual).max() <= 1e-5 * scale
 
 
+@pytest.mark.parametrize("name", sorted(DESCS))
+def test_nan_state_gives_nan_derivatives(name):
+    """The kernels jump a geodesic whose position and velocity are all NaN to its last step,
+    which is exact only if every later step stays all NaN: each right-hand side (finite
+    differences, dual numbers, closed form, binary32) maps an all-NaN state to all-NaN."""
+    L = C.CDLL(LIB)
+    d = DESCS[name]
+    nan = np.full(4, np.nan)
+    for kind in range(4):
+        assert np.isnan(_rhs(L, d, kind, nan, nan)).all(), kind
+
+
 def test_schwarzschild_radial_infall_known_answer():
     """Radial null geodesic in Schwarzschild: d2r/dlambda2 = 0 for (v^t, v^r) = (1/(1-2/r), -1)
     (E = 1, so dr/dlambda is constant), and d2t/dlambda2 = -2 v^t v^r / (r^2 (1-2/r))."""

@@ -169,6 +169,43 @@ def test_persistent_tail_bitexact(name, tail):
     assert g["stats"].launches == (1 if tail == 2 ** 32 - 1 else 2)
 
 
+# Manko-Novikov a=0.5, alpha3=2.5 at 70 deg on a 40x40 grid: pixel 698 goes through the pole
+# (ThetaSingularity off), its velocity turns NaN on step 2344 and its position on step 2345
+NAN_CASE = lambda **k: c0(metric=A.METRIC_MANKONOVIKOV, params=(0.5, 2.5), rlog=True,  # noqa: E731
+                          inclination_deg=70.0, **k)
+NAN_LIMITS = {
+    "timeout_3000": dict(max_steps=3000),
+    "step_cap_first": dict(max_steps=3000, step_cap=2600),
+    "step_cap_only": dict(terms=A.TERMFLAG_HORIZON | A.TERMFLAG_BOUNDARYSPHERE, step_cap=2600),
+    "timeout_next_step": dict(max_steps=2345),
+    "step_cap_next_step": dict(max_steps=100000, step_cap=2346),
+    "gated_counters": dict(max_steps=3000, ep_update=A.UpdateFrequency(2, 0, 0)),
+}
+
+
+@pytest.mark.parametrize("schedule", ["oneshot", "persistent", "persistent_no_tail"])
+@pytest.mark.parametrize("limits", sorted(NAN_LIMITS))
+def test_nan_geodesic_skips_to_its_last_step(limits, schedule):
+    """A geodesic whose state has turned NaN changes nothing but its step counters until
+    TimeOut or the step cap ends it; the kernels jump to its last step.  Every field, the step
+    count included, is the oracle's (which takes every step), at a time-out or step cap well
+    after the NaN, right after it, and with a gated counter (no jump)."""
+    kw = dict(NAN_LIMITS[limits])
+    sched = {"oneshot": dict(schedule=A.SCHEDULE_ONESHOT),
+             "persistent": dict(schedule=A.SCHEDULE_PERSISTENT),
+             "persistent_no_tail": dict(schedule=A.SCHEDULE_PERSISTENT, tail_geodesics=2 ** 32 - 1)}[schedule]
+    side = 40
+    print(f"[nan] {limits} {schedule}", flush=True)
+    F = _gpu()
+    with F.Context(NAN_CASE(**kw, **sched)) as ctx:
+        g = ctx.trace_grid(side)
+    o = O.trace("fm", NAN_CASE(**kw), side)
+    assert np.isnan(o["final_pos"][698]).all() and np.isnan(o["final_vel"][698]).all()
+    for k in FIELDS:
+        assert bitwise_equal(g[k], o[k]), k
+    assert g["stats"].steps == o["steps"].sum()
+
+
 def test_persistent_pixel_list_verlet():
     from tests.cases import johannsen_cfg
     desc = johannsen_cfg(
