@@ -774,35 +774,44 @@ FM_HD_INLINE double step_size(const RunConst &R, const double *pos, const double
 template <class RHS>
 FM_HD_INLINE void rk4_step(const RunConst &R, const double *x, const double *v, double *nx, double *nv, double &hout)
 {
+    // The sums ((k1 + 2 k2) + 2 k3) + k4 and ((v + 2 l2) + 2 l3) + l4 are accumulated as the
+    // stages come (the same additions in the same order, so the same bits): k1..k3 and l2, l3
+    // need not stay live across the later right-hand sides.
     const double h = step_size<RHS>(R, x, v);
     const double hh = 0.5 * h;
-    double k1[4], k2[4], k3[4], k4[4], l2[4], l3[4], l4[4], tp[4];
-    RHS::eval(R, x, v, k1);
+    double k[4], sv[4], sx[4], l[4], tp[4];
+    RHS::eval(R, x, v, k);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         tp[i] = x[i] + v[i] * hh;
-        l2[i] = v[i] + k1[i] * hh;
+        l[i] = v[i] + k[i] * hh; // l2
+        sv[i] = k[i];
+        sx[i] = v[i] + l[i] * 2;
     }
-    RHS::eval(R, tp, l2, k2);
+    RHS::eval(R, tp, l, k);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        tp[i] = x[i] + l2[i] * hh;
-        l3[i] = v[i] + k2[i] * hh;
+        tp[i] = x[i] + l[i] * hh;
+        l[i] = v[i] + k[i] * hh; // l3
+        sv[i] = sv[i] + k[i] * 2;
+        sx[i] = sx[i] + l[i] * 2;
     }
-    RHS::eval(R, tp, l3, k3);
+    RHS::eval(R, tp, l, k);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        tp[i] = x[i] + l3[i] * h;
-        l4[i] = v[i] + k3[i] * h;
+        tp[i] = x[i] + l[i] * h;
+        l[i] = v[i] + k[i] * h; // l4
+        sv[i] = sv[i] + k[i] * 2;
+        sx[i] = sx[i] + l[i];
     }
-    RHS::eval(R, tp, l4, k4);
+    RHS::eval(R, tp, l, k);
     // the reference divides (FOORT/Integrators.cpp:91-92); the analytic formulations multiply by
     // the reciprocal, one instruction instead of a correctly rounded division
     const double h6 = RHS::exact ? h / 6.0 : h * (1.0 / 6.0);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        nv[i] = v[i] + (((k1[i] + k2[i] * 2) + k3[i] * 2) + k4[i]) * h6;
-        nx[i] = x[i] + (((v[i] + l2[i] * 2) + l3[i] * 2) + l4[i]) * h6;
+        nv[i] = v[i] + (sv[i] + k[i]) * h6;
+        nx[i] = x[i] + sx[i] * h6;
     }
     hout = h;
 }
@@ -824,36 +833,43 @@ FM_HD_INLINE void rk4_step_mixed(const RunConst &R, const double *x, const doubl
     }
     const double h = adaptive_step_fast(R, x, v);
     const float hf = static_cast<float>(h), hh = 0.5f * hf;
-    float xf[4], vf[4], k1[4], k2[4], k3[4], k4[4], tp[4], l2[4], l3[4];
+    // sums accumulated as the stages come, in the order of the formula (as in rk4_step)
+    float xf[4], vf[4], k[4], l[4], tp[4], sv[4], sk[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         xf[i] = static_cast<float>(x[i]);
         vf[i] = static_cast<float>(v[i]);
     }
-    rhs_analytic<METRIC, float, float>(R, xf, vf, k1);
+    rhs_analytic<METRIC, float, float>(R, xf, vf, k);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         tp[i] = xf[i] + vf[i] * hh;
-        l2[i] = vf[i] + k1[i] * hh;
+        l[i] = vf[i] + k[i] * hh; // l2
+        sv[i] = k[i];
+        sk[i] = k[i];
     }
-    rhs_analytic<METRIC, float, float>(R, tp, l2, k2);
+    rhs_analytic<METRIC, float, float>(R, tp, l, k);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        tp[i] = xf[i] + l2[i] * hh;
-        l3[i] = vf[i] + k2[i] * hh;
+        tp[i] = xf[i] + l[i] * hh;
+        l[i] = vf[i] + k[i] * hh; // l3
+        sv[i] = sv[i] + k[i] * 2.0f;
+        sk[i] = sk[i] + k[i];
     }
-    rhs_analytic<METRIC, float, float>(R, tp, l3, k3);
+    rhs_analytic<METRIC, float, float>(R, tp, l, k);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        tp[i] = xf[i] + l3[i] * hf;
-        l2[i] = vf[i] + k3[i] * hf; // l4
+        tp[i] = xf[i] + l[i] * hf;
+        l[i] = vf[i] + k[i] * hf; // l4
+        sv[i] = sv[i] + k[i] * 2.0f;
+        sk[i] = sk[i] + k[i];
     }
-    rhs_analytic<METRIC, float, float>(R, tp, l2, k4);
+    rhs_analytic<METRIC, float, float>(R, tp, l, k);
     const float h6 = static_cast<float>(h * (1.0 / 6.0)), hsq6 = static_cast<float>(h * h * (1.0 / 6.0));
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        nv[i] = v[i] + static_cast<double>((((k1[i] + k2[i] * 2.0f) + k3[i] * 2.0f) + k4[i]) * h6);
-        nx[i] = (x[i] + v[i] * h) + static_cast<double>(((k1[i] + k2[i]) + k3[i]) * hsq6);
+        nv[i] = v[i] + static_cast<double>((sv[i] + k[i]) * h6);
+        nx[i] = (x[i] + v[i] * h) + static_cast<double>(sk[i] * hsq6);
     }
     hout = h;
 }
