@@ -134,7 +134,11 @@ def run_c2(args):
                 ("analytic_fp64_tuned_graph", V2, "Analytic", "FP64", "Persistent", "Schedule", True),
                 ("mixed_tuned_graph", V2, "Analytic", "Mixed", "Persistent", "Schedule", True),
                 ("fd_tuned_graph_mesh_v1", V1, "FD", "FP64", "Persistent", "Schedule", True)]
-    if args.no_variants:
+    if args.c2_variant:
+        variants = [v for v in variants if v[0] == args.c2_variant]
+        if not variants:
+            raise SystemExit(f"bench.py: no C2 variant named {args.c2_variant!r}")
+    elif args.no_variants:
         variants = variants[:1]
     results = {}
     tmp = tempfile.mkdtemp(prefix="foort_c2_")
@@ -148,7 +152,8 @@ def run_c2(args):
             cfg = os.path.join(tmp, f"{name}.cfg")
             with open(cfg, "w") as f:
                 f.write(W.c2_config(mesh=mesh, christoffel=chris, precision=prec, schedule=sched,
-                                    prefix=os.path.join(tmp, name), autotune=tune, use_graph=graph))
+                                    prefix=os.path.join(tmp, name), autotune=tune, use_graph=graph,
+                                    tail_geodesics=args.tail_geodesics))
             walls, gpu_ms = [], []
             for i in range(args.warmup + args.steps):
                 t = time.perf_counter()
@@ -311,6 +316,11 @@ def main():
     ap.add_argument("--cpu-stride", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--tail-geodesics", type=int, default=None,
+                    help="c2: the GPU group's TailGeodesics (unfinished geodesics handed to the "
+                         "four-lanes-per-geodesic kernel; 4294967295 = never)")
+    ap.add_argument("--c2-variant", default=None,
+                    help="c2: run only the named variant (fd_tuned_graph, fd_oneshot, fd_persistent, ...)")
     ap.add_argument("--christoffel", choices=["fd", "analytic"], default="fd",
                     help="timed path: the reference's finite differences (default) or analytic")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")

@@ -51,8 +51,10 @@ C2_TEXT = ("C2: SquareSubdivisionMeshV2 on Kerr a=0.94, 60 deg, 30Mx30M screen: 
 
 def c2_config(mesh="SquareSubdivisionMeshV2", christoffel="FD", precision="FP64", schedule="OneShot",
               prefix="c2", initial=65536, max_pixels=4000000, iteration=1000000, max_subdivide=9,
-              autotune="None", use_graph=False):
-    """The C2 run as a configuration file text for the foort program."""
+              autotune="None", use_graph=False, tail_geodesics=None):
+    """The C2 run as a configuration file text for the foort program (tail_geodesics: the GPU
+    group's TailGeodesics, None for the default)."""
+    tail = "" if tail_geodesics is None else f" TailGeodesics = {int(tail_geodesics)};"
     return f"""
 Metric = {{ Name = "Kerr"; a = 0.94; RLogScale = true; }};
 Diagnostics = {{
@@ -74,7 +76,7 @@ ViewScreen = {{
 Source = {{ Name = "NoSource"; }};
 Integrator = {{ Type = "RK4"; StepSize = 0.03; }};
 GPU = {{ Christoffel = "{christoffel}"; Precision = "{precision}"; Schedule = "{schedule}";
-        Autotune = "{autotune}"; UseGraph = {"true" if use_graph else "false"}; }};
+        Autotune = "{autotune}"; UseGraph = {"true" if use_graph else "false"};{tail} }};
 Output = {{ FilePrefix = "{prefix}"; FileExtension = "dat"; TimeStamp = false; ScreenOutputLevel = 2; }};
 """
 
