@@ -405,10 +405,17 @@ FM_HD_INLINE void mn_common_an(const MetricParams &M, T u, T sint, T cost, MNCom
     // 1/R + P1/R^2 + P2/R^3 + P3/R^4 and 1/R - P1/R^2 + P2/R^3 - P3/R^4, i.e. (even +- odd) / R
     // with even = 1 + P2/R^2 and odd = (P1 + P3/R^2)/R
     T iR2 = iR * iR;
+    // the Legendre terms die here: psi and gamma' (and their exponentials) before aa, bb
+    T iR4 = iR2 * iR2;
+    T psi = a3 * P3 * iR4;
+    T gamma_prime = 2. * a3 * a3 * (P4 * P4 - P3 * P3) * (iR4 * iR4) +
+                    2. * a3 * (-yy + (xx * P1 + (-yy * P2 + xx * P3 * iR) * iR) * iR) * iR;
     T even = P2 * iR2 + 1.;
     T odd = (P3 * iR2 + P1) * iR;
     T s_m = (even + odd) * iR;
     T s_p = (even - odd) * iR;
+    T e2psi = fexp(2. * psi);
+    T e2gp = fexp(2. * gamma_prime);
     T aa = -al * fexp(2. * a3 * (1. - (xx - yy) * s_m));
     T bb = al * fexp(2. * a3 * (-1. + (xx + yy) * s_p));
     T ab = aa * bb;
@@ -417,14 +424,9 @@ FM_HD_INLINE void mn_common_an(const MetricParams &M, T u, T sint, T cost, MNCom
     T AA = x2m1 * fpow2(1. + ab) - omy2 * fpow2(bma);
     T BB = fpow2(xx + 1. + (xx - 1.) * ab) + fpow2((1. + yy) * aa + (1. - yy) * bb);
     T CC = x2m1 * (1. + ab) * (bma - yy * (aa + bb)) + omy2 * bma * (1. + ab + xx * (1. - ab));
-    T iR4 = iR2 * iR2;
-    T psi = a3 * P3 * iR4;
-    T gamma_prime = 2. * a3 * a3 * (P4 * P4 - P3 * P3) * (iR4 * iR4) +
-                    2. * a3 * (-yy + (xx * P1 + (-yy * P2 + xx * P3 * iR) * iR) * iR) * iR;
-    T e2psi = fexp(2. * psi);
     const double ial2 = 1. / ((1. - al * al) * (1. - al * al));
     c.f = e2psi * AA / BB;
-    c.exp_2gamma = fexp(2. * gamma_prime) * AA * ial2 / (x2 - y2);
+    c.exp_2gamma = e2gp * AA * ial2 / (x2 - y2);
     c.omega = 2. * kk * CC / (e2psi * AA) - 4. * kk * al / (1. - al * al);
     c.rho_sq = (r - 1.) * (r - 1.) - kk * kk * cost * cost;
     c.delta = (r - 1.) * (r - 1.) - kk * kk;
