@@ -166,6 +166,9 @@ trace_oneshot(const RunConst R, uint64_t side, uint64_t first, uint64_t count,
     add_steps(total_steps, mine);
 }
 
+// The update-frequency counters are kept only where some update frequency is not 1 (EVERY
+// false): an EVERY kernel never reads them, so they take no registers across its step loop.
+template <bool EVERY>
 __device__ __forceinline__ void slot_load(const LaneSlots &S, uint32_t l, GeoState &g)
 {
     const uint32_t L = S.L;
@@ -179,17 +182,20 @@ __device__ __forceinline__ void slot_load(const LaneSlots &S, uint32_t l, GeoSta
     g.closest = S.d[10 * L + l];
     g.term = static_cast<int32_t>(S.u[0 * L + l]);
     g.steps = S.u[1 * L + l];
-    g.hc = S.u[2 * L + l];
-    g.bc = S.u[3 * L + l];
-    g.tc = S.u[4 * L + l];
-    g.thc = S.u[5 * L + l];
     g.tsteps = S.u[6 * L + l];
-    g.epc = S.u[7 * L + l];
-    g.crc = S.u[8 * L + l];
+    if constexpr (!EVERY) {
+        g.hc = S.u[2 * L + l];
+        g.bc = S.u[3 * L + l];
+        g.tc = S.u[4 * L + l];
+        g.thc = S.u[5 * L + l];
+        g.epc = S.u[7 * L + l];
+        g.crc = S.u[8 * L + l];
+    }
     g.quadrant = static_cast<int32_t>(S.u[9 * L + l]);
     g.passes = static_cast<int32_t>(S.u[10 * L + l]);
 }
 
+template <bool EVERY>
 __device__ __forceinline__ void slot_save(const LaneSlots &S, uint32_t l, const GeoState &g)
 {
     const uint32_t L = S.L;
@@ -203,13 +209,15 @@ __device__ __forceinline__ void slot_save(const LaneSlots &S, uint32_t l, const 
     S.d[10 * L + l] = g.closest;
     S.u[0 * L + l] = static_cast<uint32_t>(g.term);
     S.u[1 * L + l] = g.steps;
-    S.u[2 * L + l] = g.hc;
-    S.u[3 * L + l] = g.bc;
-    S.u[4 * L + l] = g.tc;
-    S.u[5 * L + l] = g.thc;
     S.u[6 * L + l] = g.tsteps;
-    S.u[7 * L + l] = g.epc;
-    S.u[8 * L + l] = g.crc;
+    if constexpr (!EVERY) {
+        S.u[2 * L + l] = g.hc;
+        S.u[3 * L + l] = g.bc;
+        S.u[4 * L + l] = g.tc;
+        S.u[5 * L + l] = g.thc;
+        S.u[7 * L + l] = g.epc;
+        S.u[8 * L + l] = g.crc;
+    }
     S.u[9 * L + l] = static_cast<uint32_t>(g.quadrant);
     S.u[10 * L + l] = static_cast<uint32_t>(g.passes);
 }
@@ -236,11 +244,10 @@ trace_persistent(const RunConst R, uint64_t side, uint64_t first,
     GeoState g;
     long long pix = S.pix[l];
     if (pix >= 0)
-        slot_load(S, l, g);
+        slot_load<EVERY>(S, l, g);
     else
         g.term = -1; // no geodesic: the step loop below skips the lane
     bool exhausted = false;
-    unsigned long long mine = 0;
     uint64_t it = 0;
     const uint64_t t0 = S.ticks ? wall_clock64() : 0;
     for (;;) {
@@ -278,20 +285,23 @@ trace_persistent(const RunConst R, uint64_t side, uint64_t first,
             geo_update<RHS, INTEG, EVERY>(R, g);
         it += n;
         // lanes whose geodesic ended write it out and become idle
+        // (their steps are summed over the wavefront here rather than kept per lane: one
+        // register pair less across the step loop)
         const bool finished = pix >= 0 && g.term != FOORT_TERM_CONTINUE;
         if (finished) {
             store_result(out, static_cast<uint64_t>(pix), g);
-            mine += g.steps;
             pix = -1;
         }
         const unsigned long long fin = __ballot(finished);
-        if (fin && static_cast<int>(lane_id) == __ffsll(static_cast<long long>(fin)) - 1)
-            atomicAdd(S.done, static_cast<unsigned long long>(__popcll(fin)));
+        if (fin) {
+            add_steps(total_steps, finished ? g.steps : 0ull);
+            if (static_cast<int>(lane_id) == __ffsll(static_cast<long long>(fin)) - 1)
+                atomicAdd(S.done, static_cast<unsigned long long>(__popcll(fin)));
+        }
     }
     S.pix[l] = pix;
     if (pix >= 0)
-        slot_save(S, l, g);
-    add_steps(total_steps, mine);
+        slot_save<EVERY>(S, l, g);
 }
 
 // Persistent schedule over a request queue (QueueArgs): as trace_persistent, with
@@ -310,14 +320,13 @@ trace_queue(const RunConst R, const QueueArgs Q, DevResults out, unsigned long l
     uint32_t req = 0;
     long long pix = S.pix[l];
     if (pix >= 0) {
-        slot_load(S, l, g);
+        slot_load<EVERY>(S, l, g);
         req = S.u[11 * S.L + l];
         M = Q.reqc[req].M;
     } else {
         g.term = -1; // no geodesic: the step loop below skips the lane
     }
     bool exhausted = false;
-    unsigned long long mine = 0;
     uint64_t it = 0;
     const uint64_t t0 = S.ticks ? wall_clock64() : 0;
     for (;;) {
@@ -358,19 +367,20 @@ trace_queue(const RunConst R, const QueueArgs Q, DevResults out, unsigned long l
             store_result(out, static_cast<uint64_t>(pix) & Q.mask, g);
             atomicAdd(&Q.req_done[2 * req], 1ull);
             atomicAdd(&Q.req_done[2 * req + 1], static_cast<unsigned long long>(g.steps));
-            mine += g.steps;
             pix = -1;
         }
         const unsigned long long fin = __ballot(finished);
-        if (fin && static_cast<int>(lane_id) == __ffsll(static_cast<long long>(fin)) - 1)
-            atomicAdd(S.done, static_cast<unsigned long long>(__popcll(fin)));
+        if (fin) {
+            add_steps(total_steps, finished ? g.steps : 0ull);
+            if (static_cast<int>(lane_id) == __ffsll(static_cast<long long>(fin)) - 1)
+                atomicAdd(S.done, static_cast<unsigned long long>(__popcll(fin)));
+        }
     }
     S.pix[l] = pix;
     if (pix >= 0) {
-        slot_save(S, l, g);
+        slot_save<EVERY>(S, l, g);
         S.u[11 * S.L + l] = req;
     }
-    add_steps(total_steps, mine);
 }
 
 // GeodesicPosition (FOORT/Diagnostics.cpp:239-284).  The saved positions of a geodesic are
@@ -521,7 +531,7 @@ trace_tail(const RunConst R, uint64_t, uint64_t, uint64_t, const uint64_t *, Dev
     if (pix < 0)
         return;
     GeoState g;
-    slot_load(S, static_cast<uint32_t>(l), g);
+    slot_load<EVERY>(S, static_cast<uint32_t>(l), g);
     while (g.term == FOORT_TERM_CONTINUE)
         geo_update<RHSQ, INTEG, EVERY>(R, g);
     if ((t & 3) == 0) {
