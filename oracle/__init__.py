@@ -1,7 +1,8 @@
 """TEST INFRASTRUCTURE ONLY: ctypes loaders for the CPU oracle
 (liboracle_libm.so, liboracle_fm.so) and for the reference's own path built
 into oracle/_ref.  Imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker -- never by the product.
+cpu_baseline leg of bench.py, as the reference results to compare with -- never by
+the product.
 """
 import ctypes as C
 import os
@@ -27,7 +28,7 @@ def _load(name):
 
 def lib(kind):
     """kind: 'libm' (C library math, pinned to the reference), 'fm' (portable math,
-    the GPU checker), 'ref' (the reference sources, IEEE flags), 'ref_ofast'
+    what the GPU path is compared with bit for bit), 'ref' (the reference sources, IEEE flags), 'ref_ofast'
     (the reference sources with their own -Ofast flags)."""
     return _load({"libm": "liboracle_libm.so", "fm": "liboracle_fm.so",
                   "ref": "_ref/libfoort_ref.so", "ref_ofast": "_ref/libfoort_ref_ofast.so"}[kind])

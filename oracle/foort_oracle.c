@@ -3,7 +3,7 @@
  * C, of the reference's per-pixel null-geodesic integration (the OpenMP loop
  * of FOORT/Main.cpp:313-419 and everything it calls per pixel).  Only tests/,
  * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and
- * only as the checker; the product never links or calls it.
+ * only as the results to compare with; the product never links or calls it.
  *
  * It follows the reference's arithmetic operation by operation: dense 4x4
  * tensors, the full 256-term Christoffel contraction (FOORT/Metric.cpp:14-55),
@@ -18,7 +18,7 @@
  *                        against the reference's output files Output/Test_*.dat.
  *   liboracle_fm.so    - the same code with the portable functions of
  *                        foort_amd/csrc/fm_math.h (ORACLE_FM defined); it is the
- *                        bit-exact checker of the GPU path, whose kernels use the
+ *                        bit-exact reference of the GPU path, whose kernels use the
  *                        same functions.
  * The camera's log(r) (RLogScale, FOORT/ViewScreen.cpp:14) always uses the C
  * library (the portable set has no logarithm); it is evaluated once per run on
