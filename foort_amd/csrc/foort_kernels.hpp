@@ -501,6 +501,7 @@ struct RhsFDQuad {
     static constexpr int metric = METRIC;
     static constexpr bool exact = true;
     static constexpr bool split_rk4 = false;
+    static constexpr bool stage_sc = false;
     __device__ __forceinline__ static void eval(const RunConst &R, const double *p, const double *v, double *out)
     {
         rhs_fd_quad<METRIC>(R, p, v, out);

@@ -583,10 +583,8 @@ FM_HD_INLINE double srcp(double x) { return 1.0 / x; }
 // q = Sigma - 2 r^2; then the Euler-Lagrange contraction.  With RLogScale the
 // radial coordinate is u = log r: v^r = r v^u and a^u = a^r / r - (v^u)^2.
 template <class S, class T>
-FM_HD_INLINE void rhs_kerr_closed(const RunConst &R, const T *p, const T *v, T *out)
+FM_HD_INLINE void rhs_kerr_closed_sc(const RunConst &R, const T *p, const T *v, S s, S c, T *out)
 {
-    S s, c;
-    ssincos(p[2], &s, &c);
     const S a = static_cast<S>(R.M.a);
     const S a2 = a * a;
     S r, vr;
@@ -642,13 +640,20 @@ FM_HD_INLINE void rhs_kerr_closed(const RunConst &R, const T *p, const T *v, T *
     }
 }
 
-// Analytic right-hand side: the metric and its exact derivatives in one
-// dual-number pass (scalar type S), then the same sparse contraction.
-template <int METRIC, class S, class T>
-FM_HD_INLINE void rhs_an(const RunConst &R, const T *p, const T *v, T *out)
+template <class S, class T>
+FM_HD_INLINE void rhs_kerr_closed(const RunConst &R, const T *p, const T *v, T *out)
 {
     S s, c;
     ssincos(p[2], &s, &c);
+    rhs_kerr_closed_sc<S, T>(R, p, v, s, c, out);
+}
+
+// Analytic right-hand side: the metric and its exact derivatives in one
+// dual-number pass (scalar type S), then the same sparse contraction.  s, c: sin and cos
+// of p[2].
+template <int METRIC, class S, class T>
+FM_HD_INLINE void rhs_an_sc(const RunConst &R, const T *p, const T *v, S s, S c, T *out)
+{
     const D2<S> U(static_cast<S>(p[1]), S(1), S(0));
     const D2<S> ST(s, S(0), c), CT(c, S(0), -s);
     D2<S> g[5];
@@ -683,6 +688,14 @@ FM_HD_INLINE void rhs_an(const RunConst &R, const T *p, const T *v, T *out)
         out[k] = static_cast<T>(o[k]);
 }
 
+template <int METRIC, class S, class T>
+FM_HD_INLINE void rhs_an(const RunConst &R, const T *p, const T *v, T *out)
+{
+    S s, c;
+    ssincos(p[2], &s, &c);
+    rhs_an_sc<METRIC, S, T>(R, p, v, s, c, out);
+}
+
 // Right-hand-side policies: FD = the reference's finite differences (exact
 // parity), AN64 = analytic binary64, MIXED = analytic, binary32 away from the
 // strong-field region (u > mixed_u) and the poles, binary64 elsewhere.
@@ -691,6 +704,7 @@ struct RhsFD {
     static constexpr int metric = METRIC;
     static constexpr bool exact = true;
     static constexpr bool split_rk4 = false;
+    static constexpr bool stage_sc = false;
     FM_HD_INLINE static void eval(const RunConst &R, const double *p, const double *v, double *out)
     {
         rhs_fd<METRIC>(R, p, v, out);
@@ -706,14 +720,53 @@ FM_HD_INLINE void rhs_analytic(const RunConst &R, const T *p, const T *v, T *out
         rhs_an<METRIC, S, T>(R, p, v, out);
 }
 
+template <int METRIC, class S, class T = S>
+FM_HD_INLINE void rhs_analytic_sc(const RunConst &R, const T *p, const T *v, S s, S c, T *out)
+{
+    if constexpr (METRIC == FOORT_METRIC_KERR)
+        rhs_kerr_closed_sc<S, T>(R, p, v, s, c, out);
+    else
+        rhs_an_sc<METRIC, S, T>(R, p, v, s, c, out);
+}
+
+// sin and cos of th0 + d from s0 = sin th0, c0 = cos th0, for |d| <= 0.1: Taylor
+// polynomials of sin d (to d^9) and cos d - 1 (to d^10), truncation below 3e-18 relative,
+// joined by the addition formulas.  Within a few ulp of sin and cos of th0 + d.
+FM_HD_INLINE void sincos_shift(double s0, double c0, double d, double *s, double *c)
+{
+    const double z = d * d;
+    const double sd = d + (d * z) * (-1.0 / 6.0 + z * (1.0 / 120.0 + z * (-1.0 / 5040.0 + z * (1.0 / 362880.0))));
+    const double cm1 = z * (-0.5 + z * (1.0 / 24.0 + z * (-1.0 / 720.0 + z * (1.0 / 40320.0 + z * (-1.0 / 3628800.0)))));
+    *s = s0 + (s0 * cm1 + c0 * sd);
+    *c = c0 + (c0 * cm1 - s0 * sd);
+}
+
+// sin and cos of a stage's theta th from those of the step's starting theta th0: the
+// shift where the stage is close (the adaptive step keeps |th - th0| near eps min(th0,
+// pi - th0)), the full evaluation otherwise
+FM_HD_INLINE void stage_sincos(double s0, double c0, double th0, double th, double *s, double *c)
+{
+    const double d = th - th0;
+    if (__builtin_fabs(d) <= 0.1)
+        sincos_shift(s0, c0, d, s, c);
+    else
+        ssincos(th, s, c);
+}
+
 template <int METRIC>
 struct RhsAn64 {
     static constexpr int metric = METRIC;
     static constexpr bool exact = false;
     static constexpr bool split_rk4 = false;
+    static constexpr bool stage_sc = true; // RK4 stages take sin/cos of theta from the step's start
     FM_HD_INLINE static void eval(const RunConst &R, const double *p, const double *v, double *out)
     {
         rhs_analytic<METRIC, double>(R, p, v, out);
+    }
+    FM_HD_INLINE static void eval_sc(const RunConst &R, const double *p, const double *v, double s, double c,
+                                     double *out)
+    {
+        rhs_analytic_sc<METRIC, double>(R, p, v, s, c, out);
     }
 };
 
@@ -722,6 +775,7 @@ struct RhsMixed {
     static constexpr int metric = METRIC;
     static constexpr bool exact = false;
     static constexpr bool split_rk4 = true; // RK4 steps go through rk4_step_mixed
+    static constexpr bool stage_sc = false;
     FM_HD_INLINE static bool far(const RunConst &R, const double *p)
     {
         const double th = p[2];
@@ -782,7 +836,24 @@ FM_HD_INLINE void rk4_step(const RunConst &R, const double *x, const double *v, 
     const double h = step_size<RHS>(R, x, v);
     const double hh = 0.5 * h;
     double k[4], sv[4], sx[4], l[4], tp[4];
-    RHS::eval(R, x, v, k);
+    // the analytic policies take sin/cos of theta: evaluated once at the step's start and
+    // shifted to the later stages (stage_sincos)
+    double s0 = 0, c0 = 0;
+    auto eval = [&](const double *pp, const double *vv, double *out) {
+        if constexpr (RHS::stage_sc) {
+            double s, c;
+            stage_sincos(s0, c0, x[2], pp[2], &s, &c);
+            RHS::eval_sc(R, pp, vv, s, c, out);
+        } else {
+            RHS::eval(R, pp, vv, out);
+        }
+    };
+    if constexpr (RHS::stage_sc) {
+        ssincos(x[2], &s0, &c0);
+        RHS::eval_sc(R, x, v, s0, c0, k);
+    } else {
+        RHS::eval(R, x, v, k);
+    }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         tp[i] = x[i] + v[i] * hh;
@@ -790,7 +861,7 @@ FM_HD_INLINE void rk4_step(const RunConst &R, const double *x, const double *v, 
         sv[i] = k[i];
         sx[i] = v[i] + l[i] * 2;
     }
-    RHS::eval(R, tp, l, k);
+    eval(tp, l, k);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         tp[i] = x[i] + l[i] * hh;
@@ -798,7 +869,7 @@ FM_HD_INLINE void rk4_step(const RunConst &R, const double *x, const double *v, 
         sv[i] = sv[i] + k[i] * 2;
         sx[i] = sx[i] + l[i] * 2;
     }
-    RHS::eval(R, tp, l, k);
+    eval(tp, l, k);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         tp[i] = x[i] + l[i] * h;
@@ -806,7 +877,7 @@ FM_HD_INLINE void rk4_step(const RunConst &R, const double *x, const double *v, 
         sv[i] = sv[i] + k[i] * 2;
         sx[i] = sx[i] + l[i];
     }
-    RHS::eval(R, tp, l, k);
+    eval(tp, l, k);
     // the reference divides (FOORT/Integrators.cpp:91-92); the analytic formulations multiply by
     // the reciprocal, one instruction instead of a correctly rounded division
     const double h6 = RHS::exact ? h / 6.0 : h * (1.0 / 6.0);

@@ -61,6 +61,17 @@ extern "C" void path_rhs(const foort_run_desc *d, int kind, const double *p, con
     }
 }
 
+// sin and cos of th0 + d from those of th0 (the analytic formulations' RK4 stages), for its
+// accuracy test: s, c of th0[i] + d[i]
+extern "C" void path_sincos_shift(const double *th0, const double *d, double *s, double *c, uint64_t n)
+{
+    for (uint64_t i = 0; i < n; ++i) {
+        double s0, c0;
+        foort::ssincos(th0[i], &s0, &c0);
+        foort::stage_sincos(s0, c0, th0[i], th0[i] + d[i], &s[i], &c[i]);
+    }
+}
+
 // the binary64 exponential of the analytic formulations, for its accuracy test
 extern "C" void path_exp(const double *x, double *out, uint64_t n)
 {

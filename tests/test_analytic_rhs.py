@@ -100,3 +100,26 @@ def test_exp_of_the_analytic_path_within_two_ulp():
     o = np.zeros_like(sp)
     L.path_exp(sp.ctypes.data_as(C.c_void_p), o.ctypes.data_as(C.c_void_p), C.c_uint64(len(sp)))
     assert o[0] == np.inf and o[1] == np.inf and o[2] == 0.0 and o[3] == 0.0 and np.isnan(o[4])
+
+
+def test_stage_sincos_shift_within_a_few_ulp():
+    """The analytic RK4 stages take sin and cos of theta by shifting those of the step's start
+    (addition formulas with Taylor polynomials of the shift, |d| <= 0.1; the full evaluation
+    beyond): within 4 ulp of the larger of the start's and the stage's value, which is the size
+    of the rounding of the stage's theta itself (theta0 + d, rounded to binary64)."""
+    L = C.CDLL(LIB)
+    rng = np.random.default_rng(5)
+    n = 200000
+    th0 = np.concatenate([rng.uniform(-0.2, np.pi + 0.2, n), rng.uniform(-1e-3, 1e-3, 20000),
+                          np.pi + rng.uniform(-1e-3, 1e-3, 20000)])
+    d = np.concatenate([rng.uniform(-0.1, 0.1, n), rng.uniform(-2e-3, 2e-3, 40000)])
+    d[:2000] = rng.uniform(0.1, 1.0, 2000) * np.sign(rng.uniform(-1, 1, 2000))  # the fallback
+    s = np.zeros_like(th0)
+    c = np.zeros_like(th0)
+    L.path_sincos_shift(th0.ctypes.data_as(C.c_void_p), d.ctypes.data_as(C.c_void_p), s.ctypes.data_as(C.c_void_p),
+                        c.ctypes.data_as(C.c_void_p), C.c_uint64(len(th0)))
+    th = th0 + d
+    for got, ref, start in ((s, np.sin(th), np.sin(th0)), (c, np.cos(th), np.cos(th0))):
+        scale = np.spacing(np.maximum(np.abs(ref), np.abs(start)))
+        err = np.abs(got - ref) / scale
+        assert err.max() <= 4.0, (err.max(), th0[err.argmax()], d[err.argmax()])
