@@ -288,6 +288,18 @@ def run_c4(args):
             "offered_rate_per_s": args.rate, "completed_rate_per_s": n / span,
             "pixels_per_s": pixels / span, "pixels": int(pixels), "geodesic_steps": int(sum(r.steps for r in res)),
             "gpu_kernel_s_shared_out": kernel_s, "span_s": span}
+    # whether the stream is sustained: latency does not grow along the stream, and the requests
+    # in flight when the last one arrives drain in about one request's latency
+    end = np.array([r.end_s for r in res])
+    order = np.argsort(sub)
+    quarters = np.array_split(order, 4)
+    line["sustained"] = {
+        "arrival_span_s": float(sub.max() - sub.min()),
+        "latency_p50_by_arrival_quarter_s": [float(np.quantile(lat[q], 0.5)) for q in quarters],
+        "latency_p99_by_arrival_quarter_s": [float(np.quantile(lat[q], 0.99)) for q in quarters],
+        "in_flight_at_last_arrival": int(((sub <= sub.max()) & (end > sub.max())).sum()),
+        "drain_after_last_arrival_s": float(end.max() - sub.max()),
+    }
     by_kind = {}
     for r in res:
         by_kind.setdefault(reqs[r.tag][3]["kind"], []).append(r)

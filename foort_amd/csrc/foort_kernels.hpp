@@ -158,8 +158,12 @@ trace_oneshot(const RunConst R, uint64_t side, uint64_t first, uint64_t count,
         pixel_unit(side, first, si, i, ux, uy);
         GeoState g;
         geo_init<EVERY>(R, ux, uy, g);
-        while (g.term == FOORT_TERM_CONTINUE)
-            geo_update<RHS, INTEG, EVERY>(R, g);
+        // runs of steps with a test for NaN between them; the wavefront leaves together
+        do {
+            for (uint32_t k = 0; k < kNanTestSteps && g.term == FOORT_TERM_CONTINUE; ++k)
+                geo_update<RHS, INTEG, EVERY>(R, g);
+            geo_skip_nan<EVERY>(R, g);
+        } while (__any(g.term == FOORT_TERM_CONTINUE));
         store_result(out, i, g);
         mine = g.steps;
     }
@@ -283,6 +287,7 @@ trace_persistent(const RunConst R, uint64_t side, uint64_t first,
         const uint32_t n = budget - it < S.interval ? static_cast<uint32_t>(budget - it) : S.interval;
         for (uint32_t k = 0; k < n && g.term == FOORT_TERM_CONTINUE; ++k)
             geo_update<RHS, INTEG, EVERY>(R, g);
+        geo_skip_nan<EVERY>(R, g);
         it += n;
         // lanes whose geodesic ended write it out and become idle
         // (their steps are summed over the wavefront here rather than kept per lane: one
@@ -361,6 +366,7 @@ trace_queue(const RunConst R, const QueueArgs Q, DevResults out, unsigned long l
         Rl.M = M;
         for (uint32_t k = 0; k < n && g.term == FOORT_TERM_CONTINUE; ++k)
             geo_update<RHS, INTEG, EVERY>(Rl, g);
+        geo_skip_nan<EVERY>(Rl, g);
         it += n;
         const bool finished = pix >= 0 && g.term != FOORT_TERM_CONTINUE;
         if (finished) {
@@ -533,8 +539,11 @@ trace_tail(const RunConst R, uint64_t, uint64_t, uint64_t, const uint64_t *, Dev
         return;
     GeoState g;
     slot_load<EVERY>(S, static_cast<uint32_t>(l), g);
-    while (g.term == FOORT_TERM_CONTINUE)
-        geo_update<RHSQ, INTEG, EVERY>(R, g);
+    do {
+        for (uint32_t k = 0; k < kNanTestSteps && g.term == FOORT_TERM_CONTINUE; ++k)
+            geo_update<RHSQ, INTEG, EVERY>(R, g);
+        geo_skip_nan<EVERY>(R, g);
+    } while (__any(g.term == FOORT_TERM_CONTINUE));
     if ((t & 3) == 0) {
         store_result(out, static_cast<uint64_t>(pix), g);
         S.pix[l] = -1;

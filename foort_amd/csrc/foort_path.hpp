@@ -1217,10 +1217,20 @@ FM_HD_INLINE void geo_update(const RunConst &R, GeoState &g)
         t = FOORT_TERM_STEPCAP;
     g.term = t;
     diag_update<EVERY>(R, g);
-    // lambda is NaN from the first NaN step size on, so one comparison a step covers the test
+}
+
+// Called by the kernels between runs of steps (a test on every step costs the step loop a
+// few percent): a geodesic whose state has turned NaN jumps to its last step.  lambda is
+// NaN from the first NaN step size on, so one comparison covers the test.
+template <bool EVERY>
+FM_HD_INLINE void geo_skip_nan(const RunConst &R, GeoState &g)
+{
     if constexpr (EVERY)
-        if (t == FOORT_TERM_CONTINUE && g.lambda != g.lambda)
+        if (g.term == FOORT_TERM_CONTINUE && g.lambda != g.lambda)
             skip_nan_steps(R, g);
 }
+
+// steps between two tests for NaN in the kernels that run a geodesic to its end
+constexpr uint32_t kNanTestSteps = 64;
 
 } // namespace foort
