@@ -33,9 +33,9 @@ if REPO not in sys.path:
 METRIC = "pixels/s & geodesic steps/s, Kerr a=0.99 1024² (FP64 vs mixed); p50/p99 latency"
 PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector peak
 # HBM bytes per pixel of the finite-difference one-shot kernel: FETCH_SIZE (x2.00) + WRITE_SIZE (x1.00),
-# each scaled by what the counter reads for a plain copy of known size; 685.8 + 6176.0 KiB per
+# each scaled by what the counter reads for a plain copy of known size; 688.8 + 6176.0 KiB per
 # 65,536-pixel dispatch (profiles/trace_oneshot_fd_kerr_rk4_c1.md)
-FD_ONESHOT_HBM_BYTES_PER_PIXEL = (685.8 + 6176.0) * 1024 / 65536
+FD_ONESHOT_HBM_BYTES_PER_PIXEL = (688.8 + 6176.0) * 1024 / 65536
 
 
 def log(msg):
