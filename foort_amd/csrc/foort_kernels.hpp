@@ -234,8 +234,11 @@ __device__ __forceinline__ void slot_save(const LaneSlots &S, uint32_t l, const 
 // A launch advances every busy lane by at most `budget` steps, or stops once
 // `ticks` of the wall clock have passed (checked between step runs, so all
 // wavefronts stop together), and parks the lane state in HBM; a wavefront
-// leaves as soon as none of its lanes has work.  The host relaunches until `done` reaches the
-// batch size.  The steps counted are those of the geodesics that finish.
+// leaves as soon as none of its lanes has work, or, with a tail threshold, once
+// every pixel is claimed and at most `tail` geodesics are unfinished (trace_tail
+// then finishes them, four lanes each).  The host relaunches until `done`
+// reaches the batch size.  The steps counted are those of the geodesics that
+// finish, summed per wavefront as they finish.
 template <class RHS, int INTEG, bool EVERY>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(waves_per_eu<RHS>(true), waves_per_eu<RHS>(true))))
 trace_persistent(const RunConst R, uint64_t side, uint64_t first,
