@@ -39,7 +39,18 @@ __global__ void __launch_bounds__(256) reset_lanes(foort::LaneSlots S, unsigned 
         *S.next = 0;
         *S.done = 0;
         *total_steps = 0;
+        total_steps[foort::kCntFuture] = 0;
+        total_steps[foort::kCntOutward] = 0;
+        total_steps[foort::kCntNonfinite] = 0;
     }
+}
+
+// the warning counters of a batch (see kCntFuture) into the caller's statistics
+void set_warnings(foort_stats *stats, const unsigned long long *c)
+{
+    stats->future_pointing = c[foort::kCntFuture];
+    stats->outward_pointing = c[foort::kCntOutward];
+    stats->nonfinite = c[foort::kCntNonfinite];
 }
 
 __global__ void __launch_bounds__(64) zero_counters(unsigned long long *p, int n)
@@ -368,7 +379,7 @@ int launch(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, uint64_t count, co
 {
     ctx->last_launches = 0;
     if (ctx->desc.schedule == FOORT_SCHEDULE_ONESHOT) {
-        hipLaunchKernelGGL(zero_counters, dim3(1), dim3(64), 0, stream, ctx->d_counters, 3);
+        hipLaunchKernelGGL(zero_counters, dim3(1), dim3(64), 0, stream, ctx->d_counters, foort::kNumCounters);
         HIP_TRY(ctx, hipGetLastError());
         if (count == 0)
             return FOORT_OK;
@@ -561,15 +572,16 @@ int trace_host(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, uint64_t count
         ctx->last_launches = 1;
     }
     if (stats) {
-        unsigned long long steps = 0;
-        HIP_TRY(ctx, hipMemcpyAsync(&steps, ctx->d_steps, sizeof(steps), hipMemcpyDeviceToHost, ctx->stream));
+        unsigned long long c[foort::kNumCounters] = {};
+        HIP_TRY(ctx, hipMemcpyAsync(c, ctx->d_counters, sizeof c, hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
         float ms = 0;
         HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
         stats->geodesics = count;
-        stats->steps = steps;
+        stats->steps = c[0];
         stats->launches = ctx->last_launches;
         stats->device_ms = ms;
+        set_warnings(stats, c);
     }
     return FOORT_OK;
 }
@@ -790,7 +802,7 @@ int foort_gpu_open(const foort_run_desc *desc, foort_gpu_ctx **out)
     if (e == hipSuccess)
         e = hipDeviceGetAttribute(&ctx->clock_khz, hipDeviceAttributeWallClockRate, ctx->device);
     if (e == hipSuccess)
-        e = hipMalloc(reinterpret_cast<void **>(&ctx->d_counters), 4 * sizeof(unsigned long long));
+        e = hipMalloc(reinterpret_cast<void **>(&ctx->d_counters), 8 * sizeof(unsigned long long));
     if (e == hipSuccess)
         ctx->d_steps = ctx->d_counters;
     if (e == hipSuccess)
@@ -891,14 +903,15 @@ int foort_gpu_trace_device(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, ui
     if (stats) {
         HIP_TRY(ctx, hipEventRecord(ctx->ev1, s));
         HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
-        unsigned long long steps = 0;
-        HIP_TRY(ctx, hipMemcpy(&steps, ctx->d_steps, sizeof(steps), hipMemcpyDeviceToHost));
+        unsigned long long c[foort::kNumCounters] = {};
+        HIP_TRY(ctx, hipMemcpy(c, ctx->d_counters, sizeof c, hipMemcpyDeviceToHost));
         float ms = 0;
         HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
         stats->geodesics = count;
-        stats->steps = steps;
+        stats->steps = c[0];
         stats->launches = ctx->last_launches;
         stats->device_ms = ms;
+        set_warnings(stats, c);
     }
     return FOORT_OK;
 }
@@ -1151,6 +1164,8 @@ int foort_gpu_queue_run(foort_gpu_queue *q, uint32_t launch_us, foort_stats *sta
     }
     hipLaunchKernelGGL(zero_counters, dim3(1), dim3(64), 0, st, ctx->d_steps, 1);
     HIP_TRY(ctx, hipGetLastError());
+    hipLaunchKernelGGL(zero_counters, dim3(1), dim3(64), 0, st, ctx->d_counters + foort::kCntFuture, 3);
+    HIP_TRY(ctx, hipGetLastError());
     foort::QueueArgs Q{q->d_reqc, q->d_req_side, q->d_req_of, q->d_si, q->d_req_done, q->count, q->mask};
     DevResults d{q->d_term, q->d_quad, q->d_pass, q->d_close, nullptr, nullptr, nullptr, nullptr};
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
@@ -1160,7 +1175,7 @@ int foort_gpu_queue_run(foort_gpu_queue *q, uint32_t launch_us, foort_stats *sta
     HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
     hipLaunchKernelGGL(clamp_claims, dim3(1), dim3(64), 0, st, S.next, static_cast<unsigned long long>(q->count));
     HIP_TRY(ctx, hipGetLastError());
-    unsigned long long c[3] = {0, 0, 0};
+    unsigned long long c[foort::kNumCounters] = {};
     HIP_TRY(ctx, hipMemcpyAsync(c, ctx->d_counters, sizeof c, hipMemcpyDeviceToHost, st));
     HIP_TRY(ctx, hipStreamSynchronize(st));
     if (stats) {
@@ -1170,6 +1185,7 @@ int foort_gpu_queue_run(foort_gpu_queue *q, uint32_t launch_us, foort_stats *sta
         stats->steps = c[0];
         stats->launches = 1;
         stats->device_ms = ms;
+        set_warnings(stats, c);
     }
     return FOORT_OK;
 }
@@ -1290,7 +1306,7 @@ int foort_gpu_trace_positions(foort_gpu_ctx *ctx, uint64_t side, uint64_t count,
 
     foort::GPArgs A{f.update_n_steps, f.update_start, f.update_finish, ctx->desc.gp_output_steps,
                     d_saved, nullptr, nullptr, nullptr};
-    hipLaunchKernelGGL(zero_counters, dim3(1), dim3(64), 0, st, ctx->d_counters, 3);
+    hipLaunchKernelGGL(zero_counters, dim3(1), dim3(64), 0, st, ctx->d_counters, foort::kNumCounters);
     HIP_TRY(ctx, hipGetLastError());
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
     if (count)
@@ -1298,8 +1314,9 @@ int foort_gpu_trace_positions(foort_gpu_ctx *ctx, uint64_t side, uint64_t count,
                            ctx->rc, A, side, uint64_t(0), count, d_si, d, ctx->d_steps);
     HIP_TRY(ctx, hipGetLastError());
     HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
-    unsigned long long steps = 0;
-    HIP_TRY(ctx, hipMemcpyAsync(&steps, ctx->d_steps, 8, hipMemcpyDeviceToHost, st));
+    unsigned long long cnt[foort::kNumCounters] = {}; // of the counting pass
+    HIP_TRY(ctx, hipMemcpyAsync(cnt, ctx->d_counters, sizeof cnt, hipMemcpyDeviceToHost, st));
+    unsigned long long &steps = cnt[0];
     std::vector<uint64_t> saved(count), seen(count), offs(count);
     std::vector<uint32_t> steps1(count), steps2(count);
     HIP_TRY(ctx, hipMemcpyAsync(saved.data(), d_saved, count * 8, hipMemcpyDeviceToHost, st));
@@ -1386,6 +1403,7 @@ int foort_gpu_trace_positions(foort_gpu_ctx *ctx, uint64_t side, uint64_t count,
         stats->steps = steps;
         stats->launches = count ? launches : 0;
         stats->device_ms = ms;
+        set_warnings(stats, cnt);
     }
     return FOORT_OK;
 }

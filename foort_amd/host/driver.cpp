@@ -30,6 +30,24 @@ double seconds_since(std::chrono::steady_clock::time_point t0)
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// Add a batch's statistics to the run's, and print the warnings the reference prints per
+// pixel in ViewScreen::SetNewInitialConditions (FOORT/ViewScreen.cpp:112-116), once per
+// pixel as it does (after the batch rather than during it).
+void add_stats(foort_stats &total, const foort_stats &st)
+{
+    total.geodesics += st.geodesics;
+    total.steps += st.steps;
+    total.launches += st.launches;
+    total.device_ms += st.device_ms;
+    total.future_pointing += st.future_pointing;
+    total.outward_pointing += st.outward_pointing;
+    total.nonfinite += st.nonfinite;
+    for (uint64_t k = 0; k < st.future_pointing; ++k)
+        screen_output("Initial velocity of geodesic is future-pointing (should be past-pointing)!", Level::Warning);
+    for (uint64_t k = 0; k < st.outward_pointing; ++k)
+        screen_output("Initial velocity of geodesic is outward-pointing (should be inward-pointing)!", Level::Warning);
+}
+
 // the value and text of one diagnostic for geodesic i
 struct BatchResults {
     std::vector<int32_t> term, quadrant, passes;
@@ -315,10 +333,7 @@ int run(const char *cfg_path, const foort_program_options &o)
                     positions.points.assign(pos.points, pos.points + 4 * pos.total);
                 }
                 foort_positions_free(&pos);
-                total.geodesics += st.geodesics;
-                total.steps += st.steps;
-                total.launches += st.launches;
-                total.device_ms += st.device_ms;
+                add_stats(total, st);
             }
         } else if (n > 0) {
             if (o.tracer) {
@@ -374,10 +389,7 @@ int run(const char *cfg_path, const foort_program_options &o)
                              : foort_gpu_trace_pixels(gpu, side, n, idx.data(), &view, &st);
                 if (rcode)
                     set_error(std::string("GPU path: ") + foort_gpu_last_error(gpu));
-                total.geodesics += st.geodesics;
-                total.steps += st.steps;
-                total.launches += st.launches;
-                total.device_ms += st.device_ms;
+                add_stats(total, st);
             }
         }
         if (rcode != FOORT_OK) {

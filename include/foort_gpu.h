@@ -171,6 +171,11 @@ typedef struct foort_stats {
     uint64_t steps;           /* sum of their integration steps */
     uint64_t launches;        /* kernel launches */
     double device_ms;         /* GPU time of the integration kernels */
+    /* the per-pixel warnings of ViewScreen::SetNewInitialConditions (FOORT/ViewScreen.cpp:112-116),
+       counted: initial velocities pointing to the future (v^t > 0) and outward (v^r > 0) */
+    uint64_t future_pointing;
+    uint64_t outward_pointing;
+    uint64_t nonfinite;       /* geodesics that ended with a NaN or infinite position or velocity */
 } foort_stats;
 
 typedef struct foort_gpu_ctx foort_gpu_ctx;

@@ -266,3 +266,49 @@ def test_autotune_picks_options_and_keeps_results():
     o = O.trace("fm", desc, 48)
     for k in FIELDS:
         assert bitwise_equal(g[k], o[k]), k
+
+
+def _initial_velocities(desc, side):
+    """The initial velocities of a side x side grid (FOORT/ViewScreen.cpp:85-110, from the
+    oracle's camera frame), to count the warnings the reference prints for them."""
+    import math
+    V, g0 = O.camera("fm", desc)
+    r, th = desc.camera_position[1], desc.camera_position[2]
+    out = []
+    for row in range(side):
+        for col in range(side):
+            al = desc.screen_center[0] + desc.screen_size[0] * (row / (side - 1) - 0.5)
+            be = desc.screen_center[1] + desc.screen_size[1] * (col / (side - 1) - 0.5)
+            d = math.sqrt(r * r + al * al + be * be)
+            v = V.T @ np.array([-1.0, -r / d, -be / r / d, al / r / math.sin(th) / d])
+            out.append(v / (g0 @ v))
+    return np.array(out)
+
+
+# a camera at r = 1.5 in the ergoregion of Kerr a = 0.99: a third of the initial velocities
+# point to the future and outward, which the reference warns about pixel by pixel
+ERGO_CAMERA = dict(params=(0.99,), inclination_deg=90.0, r_cam=1.5, screen=(4.0, 4.0), max_steps=2000)
+
+
+@pytest.mark.parametrize("schedule", [A.SCHEDULE_ONESHOT, A.SCHEDULE_PERSISTENT])
+def test_initial_velocity_warnings_and_nonfinite_results_counted(schedule):
+    """foort_stats counts the initial velocities the reference warns about
+    (FOORT/ViewScreen.cpp:112-116) and the geodesics ending with a non-finite state."""
+    side = 8
+    print(f"[warn] ergoregion camera, schedule {schedule}", flush=True)
+    F = _gpu()
+    with F.Context(c0(schedule=schedule, **ERGO_CAMERA)) as ctx:
+        g = ctx.trace_grid(side)
+    o = O.trace("fm", c0(**ERGO_CAMERA), side)
+    for k in FIELDS:
+        assert bitwise_equal(g[k], o[k]), k
+    v = _initial_velocities(c0(**ERGO_CAMERA), side)
+    st = g["stats"]
+    assert (st.future_pointing, st.outward_pointing) == (int((v[:, 0] > 0).sum()), int((v[:, 1] > 0).sum()))
+    assert st.future_pointing > 0 and st.outward_pointing > 0
+    bad = ~np.isfinite(np.concatenate([o["final_pos"], o["final_vel"]], axis=1)).all(axis=1)
+    assert st.nonfinite == int(bad.sum())
+    # the Manko-Novikov grid with one geodesic through the pole: one non-finite result
+    with F.Context(NAN_CASE(max_steps=3000, schedule=schedule)) as ctx:
+        n = ctx.trace_grid(40)
+    assert (n["stats"].nonfinite, n["stats"].future_pointing, n["stats"].outward_pointing) == (1, 0, 0)

@@ -166,3 +166,41 @@ def test_autotuned_program_writes_the_same_files(tmp_path, monkeypatch):
     entries = cache.read_text().splitlines()
     assert entries and all("size=2^16" in e for e in entries)
     assert len(entries) == len({e.split(" | ")[0] for e in entries})  # the second run only read the cache
+
+
+ERGO_CFG = """
+Metric = { Name = "Kerr"; a = 0.99; RLogScale = false; };
+Source = { Name = "NoSource"; };
+Diagnostics = { FourColorScreen = { On = true; }; };
+Terminations = {
+  Horizon = { On = true; Epsilon_Horizon = 0.01; };
+  BoundarySphere = { On = true; SphereRadius = 1000.0; };
+  TimeOut = { On = true; MaxSteps = 2000; };
+};
+ViewScreen = {
+  Position = { t = 0.0; r = 1.5; theta = 1.5707963267948966; phi = 0.0; };
+  ScreenSize = { x = 4.0; y = 4.0; };
+  Mesh = { Type = "SimpleSquareMesh"; TotalPixels = 64; };
+};
+Integrator = { Type = "RK4"; };
+Output = { FilePrefix = "%s"; FileExtension = "dat"; TimeStamp = false; ScreenOutputLevel = 1; };
+"""
+
+
+def test_cli_prints_the_initial_velocity_warnings(tmp_path):
+    """The program prints the reference's warning for every pixel whose initial velocity points
+    to the future or outward (FOORT/ViewScreen.cpp:112-116): a camera at r = 1.5 in the
+    ergoregion of Kerr a = 0.99, 8 x 8 pixels."""
+    from tests.test_gpu_parity import _initial_velocities
+    from foort_amd.workloads import c0
+    cfg = tmp_path / "ergo.cfg"
+    cfg.write_text(ERGO_CFG % str(tmp_path / "ergo"))
+    print("[program] ergoregion camera", flush=True)
+    r = subprocess.run([P.CLI_PATH, str(cfg)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    v = _initial_velocities(c0(params=(0.99,), inclination_deg=90.0, r_cam=1.5, screen=(4.0, 4.0)), 8)
+    lines = r.stdout.splitlines()
+    fut = "WARNING: Initial velocity of geodesic is future-pointing (should be past-pointing)!"
+    out = "WARNING: Initial velocity of geodesic is outward-pointing (should be inward-pointing)!"
+    assert lines.count(fut) == int((v[:, 0] > 0).sum()) > 0
+    assert lines.count(out) == int((v[:, 1] > 0).sum()) > 0
