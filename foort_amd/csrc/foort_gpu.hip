@@ -45,6 +45,49 @@ __global__ void __launch_bounds__(256) reset_lanes(foort::LaneSlots S, unsigned 
     }
 }
 
+// The initial velocities of a batch's pixels that the reference warns about
+// (FOORT/ViewScreen.cpp:112-116), counted in a pass of their own (the integration kernels carry
+// no code for it): one ballot and one atomic per wavefront.
+__global__ void __launch_bounds__(256) count_ic_warnings(const RunConst R, uint64_t side, uint64_t first,
+                                                         uint64_t count, const uint64_t *si, unsigned long long *c)
+{
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    bool fut = false, outw = false;
+    if (i < count) {
+        double ux, uy;
+        foort::pixel_unit(side, first, si, i, ux, uy);
+        GeoState g;
+        foort::geo_init<true>(R, ux, uy, g);
+        fut = g.vel[0] > 0;
+        outw = g.vel[1] > 0;
+    }
+    const unsigned long long bf = __ballot(fut), bo = __ballot(outw);
+    if ((threadIdx.x & 63) == 0) {
+        if (bf)
+            atomicAdd(c + foort::kCntFuture, static_cast<unsigned long long>(__popcll(bf)));
+        if (bo)
+            atomicAdd(c + foort::kCntOutward, static_cast<unsigned long long>(__popcll(bo)));
+    }
+}
+
+// Geodesics whose final position or velocity is NaN or infinite (x - x is 0 only for a finite x),
+// from the result arrays.
+__global__ void __launch_bounds__(256) count_nonfinite(const double *pos, const double *vel, uint64_t count,
+                                                       unsigned long long *c)
+{
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    bool bad = false;
+    if (i < count) {
+        double z = 0;
+        for (int k = 0; k < 4; ++k)
+            z += (pos[4 * i + k] - pos[4 * i + k]) + (vel[4 * i + k] - vel[4 * i + k]);
+        bad = z != 0;
+    }
+    const unsigned long long b = __ballot(bad);
+    if ((threadIdx.x & 63) == 0 && b)
+        atomicAdd(c + foort::kCntNonfinite, static_cast<unsigned long long>(__popcll(b)));
+}
+
 // the warning counters of a batch (see kCntFuture) into the caller's statistics
 void set_warnings(foort_stats *stats, const unsigned long long *c)
 {
@@ -374,8 +417,8 @@ bool single_launch(const foort_gpu_ctx *ctx)
 
 // Queue the kernels of one batch on `stream`.  A batch that is one launch is only queued;
 // otherwise the persistent schedule relaunches until the device counts every pixel done.
-int launch(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, uint64_t count, const uint64_t *si,
-           const DevResults &out, hipStream_t stream)
+int launch_trace(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, uint64_t count, const uint64_t *si,
+                 const DevResults &out, hipStream_t stream)
 {
     ctx->last_launches = 0;
     if (ctx->desc.schedule == FOORT_SCHEDULE_ONESHOT) {
@@ -438,6 +481,26 @@ int launch(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, uint64_t count, co
         HIP_TRY(ctx, hipStreamSynchronize(stream));
     }
     ctx->last_launches = n;
+    return FOORT_OK;
+}
+
+// A batch's integration, then the passes that count its warnings (and its non-finite results
+// when the final positions and velocities are kept).
+int launch(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, uint64_t count, const uint64_t *si,
+           const DevResults &out, hipStream_t stream)
+{
+    int rc = launch_trace(ctx, side, first, count, si, out, stream);
+    if (rc || count == 0)
+        return rc;
+    const unsigned grid = static_cast<unsigned>((count + 255) / 256);
+    hipLaunchKernelGGL(count_ic_warnings, dim3(grid), dim3(256), 0, stream, ctx->rc, side, first, count, si,
+                       ctx->d_counters);
+    HIP_TRY(ctx, hipGetLastError());
+    if (out.pos && out.vel) {
+        hipLaunchKernelGGL(count_nonfinite, dim3(grid), dim3(256), 0, stream, out.pos, out.vel, count,
+                           ctx->d_counters);
+        HIP_TRY(ctx, hipGetLastError());
+    }
     return FOORT_OK;
 }
 
@@ -1313,6 +1376,16 @@ int foort_gpu_trace_positions(foort_gpu_ctx *ctx, uint64_t side, uint64_t count,
         hipLaunchKernelGGL(ctx->kernels.gp, dim3(static_cast<unsigned>((count + 255) / 256)), dim3(256), 0, st,
                            ctx->rc, A, side, uint64_t(0), count, d_si, d, ctx->d_steps);
     HIP_TRY(ctx, hipGetLastError());
+    if (count) {
+        const unsigned grid = static_cast<unsigned>((count + 255) / 256);
+        hipLaunchKernelGGL(count_ic_warnings, dim3(grid), dim3(256), 0, st, ctx->rc, side, uint64_t(0), count,
+                           d_si, ctx->d_counters);
+        HIP_TRY(ctx, hipGetLastError());
+        if (d.pos && d.vel) {
+            hipLaunchKernelGGL(count_nonfinite, dim3(grid), dim3(256), 0, st, d.pos, d.vel, count, ctx->d_counters);
+            HIP_TRY(ctx, hipGetLastError());
+        }
+    }
     HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
     unsigned long long cnt[foort::kNumCounters] = {}; // of the counting pass
     HIP_TRY(ctx, hipMemcpyAsync(cnt, ctx->d_counters, sizeof cnt, hipMemcpyDeviceToHost, st));

@@ -92,26 +92,9 @@ struct QueueArgs {
 // The batch counters (one array; the kernels take its address as total_steps): [0] steps,
 // [1] next pixel and [2] pixels done of the persistent schedule, then the warnings of
 // ViewScreen::SetNewInitialConditions (FOORT/ViewScreen.cpp:112-116) -- [3] initial velocity
-// pointing to the future, [4] pointing outward -- and [5] geodesics ending non-finite.
+// pointing to the future, [4] pointing outward -- and [5] geodesics ending non-finite, which
+// passes of their own count (the integration kernels carry no code for them).
 constexpr int kCntFuture = 3, kCntOutward = 4, kCntNonfinite = 5, kNumCounters = 6;
-
-__device__ __forceinline__ void count_ic_warnings(unsigned long long *c, const GeoState &g)
-{
-    if (g.vel[0] > 0)
-        atomicAdd(c + kCntFuture, 1ull);
-    if (g.vel[1] > 0)
-        atomicAdd(c + kCntOutward, 1ull);
-}
-
-__device__ __forceinline__ void count_nonfinite(unsigned long long *c, const GeoState &g)
-{
-    double z = 0; // x - x is 0 for a finite x and NaN otherwise
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-        z += (g.pos[k] - g.pos[k]) + (g.vel[k] - g.vel[k]);
-    if (z != 0)
-        atomicAdd(c + kCntNonfinite, 1ull);
-}
 
 __device__ __forceinline__ void store_result(const DevResults &out, uint64_t i, const GeoState &g)
 {
@@ -182,7 +165,6 @@ trace_oneshot(const RunConst R, uint64_t side, uint64_t first, uint64_t count,
         pixel_unit(side, first, si, i, ux, uy);
         GeoState g;
         geo_init<EVERY>(R, ux, uy, g);
-        count_ic_warnings(total_steps, g);
         // runs of steps with a test for NaN between them; the wavefront leaves together
         do {
             for (uint32_t k = 0; k < kNanTestSteps && g.term == FOORT_TERM_CONTINUE; ++k)
@@ -190,7 +172,6 @@ trace_oneshot(const RunConst R, uint64_t side, uint64_t first, uint64_t count,
             geo_skip_nan<EVERY>(R, g);
         } while (__any(g.term == FOORT_TERM_CONTINUE));
         store_result(out, i, g);
-        count_nonfinite(total_steps, g);
         mine = g.steps;
     }
     add_steps(total_steps, mine);
@@ -299,7 +280,6 @@ trace_persistent(const RunConst R, uint64_t side, uint64_t first,
                     double ux, uy;
                     pixel_unit(side, first, si, k, ux, uy);
                     geo_init<EVERY>(R, ux, uy, g);
-                    count_ic_warnings(total_steps, g);
                 } else {
                     exhausted = true;
                 }
@@ -325,7 +305,6 @@ trace_persistent(const RunConst R, uint64_t side, uint64_t first,
         const bool finished = pix >= 0 && g.term != FOORT_TERM_CONTINUE;
         if (finished) {
             store_result(out, static_cast<uint64_t>(pix), g);
-            count_nonfinite(total_steps, g);
             pix = -1;
         }
         const unsigned long long fin = __ballot(finished);
@@ -384,7 +363,6 @@ trace_queue(const RunConst R, const QueueArgs Q, DevResults out, unsigned long l
                     double ux, uy;
                     pixel_unit(Q.req_side[req], 0, Q.si, at, ux, uy);
                     geo_init<EVERY>(Rq, ux, uy, g);
-                    count_ic_warnings(total_steps, g);
                 } else {
                     exhausted = true;
                 }
@@ -403,7 +381,6 @@ trace_queue(const RunConst R, const QueueArgs Q, DevResults out, unsigned long l
         const bool finished = pix >= 0 && g.term != FOORT_TERM_CONTINUE;
         if (finished) {
             store_result(out, static_cast<uint64_t>(pix) & Q.mask, g);
-            count_nonfinite(total_steps, g);
             atomicAdd(&Q.req_done[2 * req], 1ull);
             atomicAdd(&Q.req_done[2 * req + 1], static_cast<unsigned long long>(g.steps));
             pix = -1;
@@ -451,7 +428,6 @@ trace_gp(const RunConst R, const GPArgs A, uint64_t side, uint64_t first, uint64
         pixel_unit(side, first, si, i, ux, uy);
         GeoState g;
         geo_init<false>(R, ux, uy, g);
-        count_ic_warnings(total_steps, g);
         const bool write = A.points != nullptr;
         const uint64_t S = write ? A.saved[i] : 0;
         uint64_t J = 0, K = S;
@@ -487,7 +463,6 @@ trace_gp(const RunConst R, const GPArgs A, uint64_t side, uint64_t first, uint64
             geo_update<RHS, INTEG, false>(R, g);
         }
         store_result(out, i, g);
-        count_nonfinite(total_steps, g);
         if (write)
             A.seen[i] = s;
         else
@@ -581,7 +556,6 @@ trace_tail(const RunConst R, uint64_t, uint64_t, uint64_t, const uint64_t *, Dev
     } while (__any(g.term == FOORT_TERM_CONTINUE));
     if ((t & 3) == 0) {
         store_result(out, static_cast<uint64_t>(pix), g);
-        count_nonfinite(total_steps, g);
         S.pix[l] = -1;
         atomicAdd(total_steps, static_cast<unsigned long long>(g.steps));
         atomicAdd(S.done, 1ull);

@@ -172,10 +172,12 @@ typedef struct foort_stats {
     uint64_t launches;        /* kernel launches */
     double device_ms;         /* GPU time of the integration kernels */
     /* the per-pixel warnings of ViewScreen::SetNewInitialConditions (FOORT/ViewScreen.cpp:112-116),
-       counted: initial velocities pointing to the future (v^t > 0) and outward (v^r > 0) */
+       counted: initial velocities pointing to the future (v^t > 0) and outward (v^r > 0)
+       (the request queue's runs count none) */
     uint64_t future_pointing;
     uint64_t outward_pointing;
-    uint64_t nonfinite;       /* geodesics that ended with a NaN or infinite position or velocity */
+    uint64_t nonfinite;       /* geodesics that ended with a NaN or infinite position or velocity,
+                                 counted when the call returns final positions and velocities */
 } foort_stats;
 
 typedef struct foort_gpu_ctx foort_gpu_ctx;
