@@ -291,6 +291,7 @@ struct foort_gpu_ctx {
     size_t h_cap = 0;
     hipGraphExec_t graph = nullptr;
     std::string graph_key;
+    std::string graph_shape, seen_shape; // batch shapes: the captured one, the last one run directly
 };
 
 namespace {
@@ -555,7 +556,22 @@ int trace_host(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, uint64_t count
     d.vel = static_cast<double *>(dptr(f[7]));
     const uint64_t *d_si = si ? reinterpret_cast<const uint64_t *>(base + o_si) : nullptr;
 
-    const bool graph = ctx->desc.use_graph && single_launch(ctx) && count > 0;
+    bool graph = ctx->desc.use_graph && single_launch(ctx) && count > 0;
+    std::string shape;
+    if (graph) {
+        // a batch shape is captured when it comes a second time (the first batch of a shape
+        // runs directly): a run of batches that never repeat, such as the iterations of an
+        // adaptive mesh, pays for no capture
+        shape = std::to_string(side) + "/" + std::to_string(first) + "/" + std::to_string(count) + "/" +
+                (si ? "s" : "g") + "/" + std::to_string(ctx->desc.schedule) + "/" +
+                std::to_string(ctx->desc.block_size);
+        for (int k = 0; k < 8; ++k)
+            shape += f[k].host ? "1" : "0";
+        if (shape != ctx->graph_shape && shape != ctx->seen_shape) {
+            ctx->seen_shape = shape;
+            graph = false;
+        }
+    }
     if (!graph) {
         // everything on the context's own stream, so contexts on other host threads run concurrently
         if (si)
@@ -622,6 +638,7 @@ int trace_host(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, uint64_t count
                 return hip_fail(ctx, e, "hipGraphInstantiate");
             }
             ctx->graph_key = key;
+            ctx->graph_shape = shape;
         }
         if (si)
             std::memcpy(hb + o_si, si, count * 16);

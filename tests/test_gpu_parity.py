@@ -222,9 +222,10 @@ def test_persistent_pixel_list_verlet():
 
 @pytest.mark.parametrize("schedule", [A.SCHEDULE_ONESHOT, A.SCHEDULE_PERSISTENT])
 def test_graph_batches_replay_bitexact(schedule):
-    """use_graph: each batch runs as a captured HIP graph (index upload, kernels, result
-    copies).  The same batch replayed three times, with eager work of another context between
-    the replays, then a batch of another shape (captured again), all equal the oracle."""
+    """use_graph: a batch shape that comes again runs as a captured HIP graph (index upload,
+    kernels, result copies).  The same batch three times -- run directly, captured, replayed --
+    with eager work of another context between them, then batches of other shapes, all equal
+    the oracle."""
     from tests.cases import sample_pixels
     desc = c1(rlog=True, max_steps=10000, schedule=schedule, use_graph=1)
     eager = c1(rlog=True, max_steps=10000)
