@@ -9,6 +9,8 @@ the reference tree; the outputs are committed, the reference never travels).
    (oracle/_ref/libfoort_ref.so, IEEE flags) for the parity cases in
    tests/cases.py: termination, diagnostics, final position, velocity,
    affine parameter and step count per pixel.
+3. sample_outputs.npz -- the pixels and integer values of the reference's
+   sample runs (FOORT/Output/sampleoutput_*).
 
 Run: python tests/golden/make_golden.py
 """
@@ -58,7 +60,44 @@ def ref_states():
     np.savez_compressed(os.path.join(HERE, "ref_states.npz"), **out)
 
 
+# the reference's sample runs (FOORT/Output/sampleoutput_*): Kerr a = 0.5, r coordinate,
+# 15 x 15 screen, camera at 90 deg (simple and adaptive mesh) and at 0.296697 rad (17 deg,
+# adaptive, "rings"); name -> (file stem, diagnostics, side of the finest grid)
+SAMPLES = {
+    "simplemesh": ("sampleoutput_simplemesh_221202-125440", ("FourColorScreen",), 100),
+    "adaptivemesh": ("sampleoutput_adaptivemesh_221202-125808", ("FourColorScreen",), (100 - 1) * 8 + 1),
+    "rings": ("sampleoutput_adaptivemesh_rings_221202-140701", ("FourColorScreen", "EquatPasses"), (141 - 1) * 8 + 1),
+}
+
+
+def sample_outputs():
+    """sample_outputs.npz -- the pixels (row, column on the finest grid) and integer values of
+    the reference's sample output files, in file order, with their first lines."""
+    out = {}
+    for name, (stem, diags, side) in SAMPLES.items():
+        rows = cols = None
+        for diag in diags:
+            with open(os.path.join(REF_OUT, f"{stem}_{diag}.dat")) as f:
+                header = f.readline().rstrip("\n")
+                rc, vals = [], []
+                for line in f:
+                    r, c, v = line.split()
+                    rc.append((int(r), int(c)))
+                    vals.append(int(v))
+            rc = np.array(rc, np.uint16)
+            assert rc.max() < side, name
+            if rows is None:
+                rows, cols = rc[:, 0], rc[:, 1]
+            assert (rows == rc[:, 0]).all() and (cols == rc[:, 1]).all(), name
+            out[f"{name}_{diag}"] = np.array(vals, np.int8)
+            out[f"{name}_{diag}_header"] = np.array(header)
+        out[f"{name}_rows"], out[f"{name}_cols"] = rows, cols
+        out[f"{name}_side"] = np.array(side)
+    np.savez_compressed(os.path.join(HERE, "sample_outputs.npz"), **out)
+
+
 if __name__ == "__main__":
     reference_outputs()
     ref_states()
+    sample_outputs()
     print("written", os.listdir(HERE))
