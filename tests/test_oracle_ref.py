@@ -8,6 +8,8 @@
    of the parity cases recorded in tests/golden/ref_states.npz, bit for bit,
    and -- when oracle/_ref is built -- metric, Christoffel symbols, single
    integration steps and initial conditions at random points, bit for bit.
+3. Against the reference's sample run FOORT/Output/sampleoutput_simplemesh_*
+   (an older version with a mirrored screen), on pixels away from boundaries.
 """
 import numpy as np
 import pytest
@@ -96,3 +98,32 @@ def test_initial_conditions_bitexact(mname):
     for row, col in [(0, 0), (3, 17), (40, 2), (63, 63), (31, 32)]:
         a, b = O.initial("libm", d, 64, row, col), O.initial("ref", d, 64, row, col)
         assert bitwise_equal(a[0], b[0]) and bitwise_equal(a[1], b[1])
+
+
+def test_oracle_on_the_reference_sample_run():
+    """The oracle against FOORT/Output/sampleoutput_simplemesh_* (Kerr a = 0.5, 90 deg, 100 x
+    100), written by an older version of the reference whose screen is mirrored (column c of
+    the file is column 99 - c here, FourColorScreen's phi halves swapped): 48 pixels away from
+    the file's own boundaries agree exactly (the GPU test of the same files covers every pixel)."""
+    import math
+    g = _gold("sample_outputs.npz")
+    img = np.zeros((100, 100), np.int32)
+    img[g["simplemesh_rows"].astype(int), g["simplemesh_cols"].astype(int)] = g["simplemesh_FourColorScreen"]
+    edge = np.zeros(img.shape, bool)
+    for ax in (0, 1):
+        d = np.diff(img, axis=ax) != 0
+        if ax == 0:
+            edge[1:] |= d
+            edge[:-1] |= d
+        else:
+            edge[:, 1:] |= d
+            edge[:, :-1] |= d
+    rng = np.random.default_rng(17)
+    cand = np.argwhere(~edge)
+    pick = cand[rng.choice(len(cand), 48, replace=False)]
+    desc = A.make_desc(A.METRIC_KERR, (0.5,), rlog=False, theta=math.pi / 2, screen=(15.0, 15.0), max_steps=1000000,
+                       horizon_eps=0.01, diags=A.DIAGFLAG_FOURCOLORSCREEN)
+    px = np.stack([pick[:, 0], 99 - pick[:, 1]], 1).astype(np.uint64)
+    o = O.trace("fm", desc, 100, screen_index=px)
+    swap = np.array([0, 2, 1, 4, 3])
+    assert (swap[o["quadrant"]] == img[pick[:, 0], pick[:, 1]]).all()
