@@ -72,3 +72,36 @@ def test_analytic_persistent_equals_oneshot():
     for k in ("term", "quadrant", "equat_passes", "steps"):
         assert np.array_equal(a[k], b[k]), k
     assert np.array_equal(a["final_pos"].view(np.uint64), b["final_pos"].view(np.uint64))
+
+
+@pytest.mark.parametrize("path", ["fd", "analytic", "mixed"])
+def test_kerr_constants_of_motion_on_every_path(path):
+    """E = -p_t, L = p_phi, Carter's Q and the null norm at the end of escaping Kerr geodesics
+    equal their initial values to each path's accuracy: the finite-difference path as the
+    oracle (it is the oracle bit for bit), the closed-form FP64 path better, and the mixed
+    path within the rounding of binary32 far-field steps."""
+    import math
+    import foort_amd
+    import oracle as O
+    a, side = 0.9, 64
+    kw = {"fd": {}, "analytic": dict(christoffel=A.CHRISTOFFEL_ANALYTIC),
+          "mixed": dict(christoffel=A.CHRISTOFFEL_ANALYTIC, precision=A.PRECISION_MIXED)}[path]
+    d = A.make_desc(A.METRIC_KERR, (a,), rlog=False, inclination_deg=60.0, screen=(20.0, 20.0), max_steps=200000, **kw)
+    with foort_amd.Context(d) as ctx:
+        g = ctx.trace_grid(side)
+    ref = A.make_desc(A.METRIC_KERR, (a,), rlog=False, inclination_deg=60.0, screen=(20.0, 20.0))
+
+    def invariants(p, v):
+        m = O.metric("fm", ref, p)[0]
+        E, L, pth = -(m[0] @ v), m[3] @ v, m[2, 2] * v[2]
+        Q = pth ** 2 + math.cos(p[2]) ** 2 * (-a * a * E * E + L * L / math.sin(p[2]) ** 2)
+        return np.array([E, L, Q]), (v @ m @ v) / np.abs(np.outer(v, v) * m).max()
+
+    worst = np.zeros(4)
+    for i in np.nonzero(g["term"] == 2)[0][::16]:
+        c0, _ = invariants(*O.initial("fm", ref, side, int(i // side), int(i % side)))
+        c1, n1 = invariants(g["final_pos"][i], g["final_vel"][i])
+        rel = np.abs(c1 - c0) / np.maximum(np.abs(c0), 1e-3)
+        worst = np.maximum(worst, np.append(rel, abs(n1)))
+    print(f"[invariants] {path}: dE {worst[0]:.2e} dL {worst[1]:.2e} dQ {worst[2]:.2e} null {worst[3]:.2e}", flush=True)
+    assert (worst < {"fd": 1e-5, "analytic": 1e-5, "mixed": 1e-4}[path]).all()
