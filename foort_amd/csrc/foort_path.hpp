@@ -957,14 +957,36 @@ FM_HD_INLINE void verlet_step(const RunConst &R, const double *x, const double *
 {
     const double h = step_size<RHS>(R, x, v);
     double ac[4], as[4], vi[4];
-    RHS::eval(R, x, v, ac);
-    const double hh2 = h * h / 2.0;
+    // the analytic policies take sin/cos of theta: evaluated at x, shifted to nx (stage_sincos),
+    // and kept for the velocity iterations at nx
+    double s1 = 0, c1 = 0;
+    auto eval_nx = [&](const double *vv, double *out) {
+        if constexpr (RHS::stage_sc)
+            RHS::eval_sc(R, nx, vv, s1, c1, out);
+        else
+            RHS::eval(R, nx, vv, out);
+    };
+    if constexpr (RHS::stage_sc) {
+        double s0, c0;
+        ssincos(x[2], &s0, &c0);
+        RHS::eval_sc(R, x, v, s0, c0, ac);
+        const double hh2 = h * h / 2.0;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        nx[i] = (x[i] + v[i] * h) + ac[i] * hh2;
-        vi[i] = v[i] + ac[i] * h;
+        for (int i = 0; i < 4; ++i) {
+            nx[i] = (x[i] + v[i] * h) + ac[i] * hh2;
+            vi[i] = v[i] + ac[i] * h;
+        }
+        stage_sincos(s0, c0, x[2], nx[2], &s1, &c1);
+    } else {
+        RHS::eval(R, x, v, ac);
+        const double hh2 = h * h / 2.0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            nx[i] = (x[i] + v[i] * h) + ac[i] * hh2;
+            vi[i] = v[i] + ac[i] * h;
+        }
     }
-    RHS::eval(R, nx, vi, as);
+    eval_nx(vi, as);
     const double h2 = h / 2.0;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
@@ -984,7 +1006,7 @@ FM_HD_INLINE void verlet_step(const RunConst &R, const double *x, const double *
 #pragma unroll
             for (int i = 0; i < 4; ++i)
                 vi[i] = nv[i];
-            RHS::eval(R, nx, vi, as);
+            eval_nx(vi, as);
 #pragma unroll
             for (int i = 0; i < 4; ++i)
                 nv[i] = v[i] + (ac[i] + as[i]) * h2;

@@ -37,6 +37,8 @@ CASES = {
                                           inclination_deg=60.0, max_steps=20000, **k), 24),
     "manko_novikov": (lambda **k: c0(metric=A.METRIC_MANKONOVIKOV, params=(0.5, 1.0), rlog=True,
                                      inclination_deg=70.0, max_steps=20000, **k), 24),
+    "kerr_verlet_tolerance": (lambda **k: c0(params=(0.9,), rlog=True, integrator=A.INTEGRATOR_VERLET,
+                                             inclination_deg=30.0, verlet_tol=1e-3, max_steps=20000, **k), 24),
     "flat": (lambda **k: c0(metric=A.METRIC_FLATSPACE, params=(), inclination_deg=45.0,
                             terms=A.TERMFLAG_BOUNDARYSPHERE | A.TERMFLAG_TIMEOUT, screen=(2000.0, 2000.0),
                             max_steps=100000, **k), 8),
