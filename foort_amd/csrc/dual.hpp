@@ -5,6 +5,8 @@
 // in (the metric parameters and literal constants) are converted to S.
 #pragma once
 
+#include "fm_div.hpp"
+
 #if defined(__HIPCC__)
 #define DUAL_HD __host__ __device__ __attribute__((always_inline)) inline
 #else
@@ -33,7 +35,7 @@ struct D2 {
     }
     DUAL_HD friend D2 operator/(const D2 &a, const D2 &b)
     {
-        const S inv = S(1) / b.v;
+        const S inv = fm_div(S(1), b.v);
         const S q = a.v * inv;
         return D2(q, (a.du - q * b.du) * inv, (a.dt - q * b.dt) * inv);
     }
@@ -50,12 +52,12 @@ struct D2 {
     DUAL_HD friend D2 operator*(double c, const D2 &a) { return a * c; }
     DUAL_HD friend D2 operator/(const D2 &a, double c)
     {
-        const S k = S(1) / static_cast<S>(c);
+        const S k = fm_div(S(1), static_cast<S>(c));
         return D2(a.v * k, a.du * k, a.dt * k);
     }
     DUAL_HD friend D2 operator/(double c, const D2 &b)
     {
-        const S inv = S(1) / b.v;
+        const S inv = fm_div(S(1), b.v);
         const S q = static_cast<S>(c) * inv;
         return D2(q, -q * b.du * inv, -q * b.dt * inv);
     }

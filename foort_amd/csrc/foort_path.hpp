@@ -19,6 +19,7 @@
 #include <stdint.h>
 
 #include "dual.hpp"
+#include "fm_div.hpp"
 #include "fm_math.h"
 #include "foort_gpu.h"
 
@@ -190,11 +191,11 @@ FM_HD_INLINE void kerr_dd(const MetricParams &M, T u, T sint, T cost, T *g)
     T sigma = r * r + a * a * cost * cost;
     T delta = r * r + a * a - 2. * r;
     T A_ = (r * r + a * a) * (r * r + a * a) - delta * a * a * sint * sint;
-    T g00 = -(1. - 2. * r / sigma);
-    T g11 = sigma / delta;
+    T g00 = -(1. - fm_div(2. * r, sigma));
+    T g11 = fm_div(sigma, delta);
     T g22 = sigma;
-    T g33 = A_ / sigma * sint * sint;
-    T g03 = -2. * a * r * sint * sint / sigma;
+    T g33 = fm_div(A_, sigma) * sint * sint;
+    T g03 = fm_div(-2. * a * r * sint * sint, sigma);
     if (M.rlog)
         g11 *= (r * r);
     g[0] = g00; g[1] = g03; g[2] = g11; g[3] = g22; g[4] = g33;
@@ -209,13 +210,13 @@ FM_HD_INLINE void kerr_uu(const MetricParams &M, T u, T sint, T cost, T *g)
     T sigma = r * r + a * a * cost * cost;
     T delta = r * r + a * a - 2. * r;
     T A_ = (r * r + a * a) * (r * r + a * a) - delta * a * a * sint * sint;
-    T g00 = -A_ / (sigma * delta);
-    T g11 = delta / sigma;
-    T g22 = 1. / sigma;
-    T g33 = (delta - a * a * sint * sint) / (sigma * delta * sint * sint);
-    T g03 = -2. * a * r / (sigma * delta);
+    T g00 = fm_div(-A_, sigma * delta);
+    T g11 = fm_div(delta, sigma);
+    T g22 = fm_div(1., sigma);
+    T g33 = fm_div(delta - a * a * sint * sint, sigma * delta * sint * sint);
+    T g03 = fm_div(-2. * a * r, sigma * delta);
     if (M.rlog)
-        g11 *= 1.0 / (r * r);
+        g11 *= fm_div(1.0, r * r);
     g[0] = g00; g[1] = g03; g[2] = g11; g[3] = g22; g[4] = g33;
 }
 
@@ -229,7 +230,7 @@ FM_HD_INLINE void flat_dd(const MetricParams &, T u, T sint, T, T *g)
 template <class T>
 FM_HD_INLINE void flat_uu(const MetricParams &, T u, T sint, T, T *g)
 {
-    g[0] = -1; g[1] = 0; g[2] = 1; g[3] = 1 / (u * u); g[4] = 1 / (u * u * sint * sint);
+    g[0] = -1; g[1] = 0; g[2] = 1; g[3] = fm_div(1., u * u); g[4] = fm_div(1., u * u * sint * sint);
 }
 
 // FOORT/Metric.cpp:276-300 / :321-345
@@ -249,11 +250,11 @@ FM_HD_INLINE void rl_common(const MetricParams &M, T u, T sint, T cost, RLCommon
     T delta = r2 + a * a - 2. * r * m;
     T H3 = r2 - 2. * r * m + a * a * cost2;
     T Bp = M.rl_root_pq * a * sint2 *
-                (((pp * qq + 4. * m * m) * r - m * (pp - 2. * m) * (qq - 2. * m)) / (2. * m * (pp + qq) * H3));
-    T H1 = r2 + a * a * cost2 + r * (pp - 2. * m) + (pp / (pp + qq)) * ((pp - 2. * m) * (qq - 2. * m) / 2.) -
-                (pp / (2. * m * (pp + qq))) * M.rl_root_pm * a * cost;
-    T H2 = r2 + a * a * cost2 + r * (qq - 2. * m) + (qq / (pp + qq)) * ((pp - 2. * m) * (qq - 2. * m) / 2.) +
-                (qq / (2. * m * (pp + qq))) * M.rl_root_pm * a * cost;
+                fm_div((pp * qq + 4. * m * m) * r - m * (pp - 2. * m) * (qq - 2. * m), 2. * m * (pp + qq) * H3);
+    T H1 = r2 + a * a * cost2 + r * (pp - 2. * m) + fm_div(pp, pp + qq) * ((pp - 2. * m) * (qq - 2. * m) / 2.) -
+                fm_div(pp, 2. * m * (pp + qq)) * M.rl_root_pm * a * cost;
+    T H2 = r2 + a * a * cost2 + r * (qq - 2. * m) + fm_div(qq, pp + qq) * ((pp - 2. * m) * (qq - 2. * m) / 2.) +
+                fm_div(qq, 2. * m * (pp + qq)) * M.rl_root_pm * a * cost;
     c.r = r; c.sint2 = sint2; c.delta = delta; c.H3 = H3; c.Bp = Bp; c.H1 = H1; c.H2 = H2;
     c.sqH1H2 = fsqrt(H1 * H2);
 }
@@ -263,11 +264,11 @@ FM_HD_INLINE void rl_dd(const MetricParams &M, T u, T sint, T cost, T *g)
 {
     RLCommon<T> c;
     rl_common(M, u, sint, cost, c);
-    T g00 = -c.H3 / c.sqH1H2;
-    T g11 = c.sqH1H2 / c.delta;
+    T g00 = fm_div(-c.H3, c.sqH1H2);
+    T g11 = fm_div(c.sqH1H2, c.delta);
     T g22 = c.sqH1H2;
-    T g33 = -(c.H3 * c.Bp * c.Bp) / c.sqH1H2 + (c.sqH1H2 * c.delta * c.sint2) / c.H3;
-    T g03 = -(c.H3 * c.Bp) / c.sqH1H2;
+    T g33 = fm_div(-(c.H3 * c.Bp * c.Bp), c.sqH1H2) + fm_div(c.sqH1H2 * c.delta * c.sint2, c.H3);
+    T g03 = fm_div(-(c.H3 * c.Bp), c.sqH1H2);
     if (M.rlog)
         g11 *= (c.r * c.r);
     g[0] = g00; g[1] = g03; g[2] = g11; g[3] = g22; g[4] = g33;
@@ -278,13 +279,13 @@ FM_HD_INLINE void rl_uu(const MetricParams &M, T u, T sint, T cost, T *g)
 {
     RLCommon<T> c;
     rl_common(M, u, sint, cost, c);
-    T g00 = ((c.H3 * c.H3 * c.Bp * c.Bp) / c.sint2 - c.H1 * c.H2 * c.delta) / (c.sqH1H2 * c.H3 * c.delta);
-    T g11 = c.delta / c.sqH1H2;
-    T g22 = 1. / c.sqH1H2;
-    T g33 = c.H3 / (c.sqH1H2 * c.delta * c.sint2);
-    T g03 = -(c.H3 * c.Bp) / (c.sqH1H2 * c.delta * c.sint2);
+    T g00 = fm_div(fm_div(c.H3 * c.H3 * c.Bp * c.Bp, c.sint2) - c.H1 * c.H2 * c.delta, c.sqH1H2 * c.H3 * c.delta);
+    T g11 = fm_div(c.delta, c.sqH1H2);
+    T g22 = fm_div(1., c.sqH1H2);
+    T g33 = fm_div(c.H3, c.sqH1H2 * c.delta * c.sint2);
+    T g03 = fm_div(-(c.H3 * c.Bp), c.sqH1H2 * c.delta * c.sint2);
     if (M.rlog)
-        g11 *= 1.0 / (c.r * c.r);
+        g11 *= fm_div(1.0, c.r * c.r);
     g[0] = g00; g[1] = g03; g[2] = g11; g[3] = g22; g[4] = g33;
 }
 
@@ -294,18 +295,18 @@ FM_HD_INLINE void joh_dd(const MetricParams &M, T u, T sint, T cost, T *g)
 {
     const double a = M.a;
     T r = M.rlog ? fexp(u) : u;
-    T A1 = 1. + M.a13 / (r * r * r);
-    T A2 = 1. + M.a22 / (r * r);
-    T A5 = 1. + M.a52 / (r * r);
-    T eps_f = M.eps3 / r;
+    T A1 = 1. + fm_div(M.a13, r * r * r);
+    T A2 = 1. + fm_div(M.a22, r * r);
+    T A5 = 1. + fm_div(M.a52, r * r);
+    T eps_f = fm_div(M.eps3, r);
     T rho2 = r * r + a * a * cost * cost + eps_f;
     T delta = r * r + a * a - 2. * r;
     T den = fpow2((r * r + a * a) * A1 - a * a * A2 * sint * sint);
-    T g00 = -rho2 * (delta - a * a * A2 * A2 * sint * sint) / den;
-    T g11 = rho2 / (delta * A5);
+    T g00 = fm_div(-rho2 * (delta - a * a * A2 * A2 * sint * sint), den);
+    T g11 = fm_div(rho2, delta * A5);
     T g22 = rho2;
-    T g33 = rho2 * sint * sint * (fpow2((r * r + a * a) * A1) - a * a * delta * sint * sint) / den;
-    T g03 = -a * ((r * r + a * a) * A1 * A2 - delta) * rho2 * sint * sint / den;
+    T g33 = fm_div(rho2 * sint * sint * (fpow2((r * r + a * a) * A1) - a * a * delta * sint * sint), den);
+    T g03 = fm_div(-a * ((r * r + a * a) * A1 * A2 - delta) * rho2 * sint * sint, den);
     if (M.rlog)
         g11 *= (r * r);
     g[0] = g00; g[1] = g03; g[2] = g11; g[3] = g22; g[4] = g33;
@@ -317,19 +318,19 @@ FM_HD_INLINE void joh_uu(const MetricParams &M, T u, T sint, T cost, T *g)
 {
     const double a = M.a;
     T r = M.rlog ? fexp(u) : u;
-    T A1 = 1. + M.a13 / (r * r * r);
-    T A2 = 1. + M.a22 / (r * r);
-    T A5 = 1. + M.a52 / (r * r);
-    T eps_f = M.eps3 / r;
+    T A1 = 1. + fm_div(M.a13, r * r * r);
+    T A2 = 1. + fm_div(M.a22, r * r);
+    T A5 = 1. + fm_div(M.a52, r * r);
+    T eps_f = fm_div(M.eps3, r);
     T rho2 = r * r + a * a * cost * cost + eps_f;
     T delta = r * r + a * a - 2. * r;
-    T g00 = (-1. * fpow2((r * r + a * a) * A1) + a * a * delta * sint * sint) / (delta * rho2);
-    T g11 = delta * A5 / rho2;
-    T g22 = 1. / rho2;
-    T g33 = (-a * a * A2 * A2 * sint * sint + delta) / (delta * rho2 * sint * sint);
-    T g03 = -a * (A2 * A1 * (r * r + a * a) - delta) / (delta * rho2);
+    T g00 = fm_div(-1. * fpow2((r * r + a * a) * A1) + a * a * delta * sint * sint, delta * rho2);
+    T g11 = fm_div(delta * A5, rho2);
+    T g22 = fm_div(1., rho2);
+    T g33 = fm_div(-a * a * A2 * A2 * sint * sint + delta, delta * rho2 * sint * sint);
+    T g03 = fm_div(-a * (A2 * A1 * (r * r + a * a) - delta), delta * rho2);
     if (M.rlog)
-        g11 *= 1.0 / (r * r);
+        g11 *= fm_div(1.0, r * r);
     g[0] = g00; g[1] = g03; g[2] = g11; g[3] = g22; g[4] = g33;
 }
 
@@ -339,6 +340,9 @@ struct MNCommon {
     T r, sint, f, exp_2gamma, omega, rho_sq, delta;
 };
 
+// The compiler's own divisions here and in mn_dd_from / mn_uu_from: with fm_div the shared
+// reciprocals of R, R^2, ... took the finite-difference kernel to 508 of its 512 registers
+// and made it 12% slower (measured), for the same bits.
 template <class T>
 FM_HD_INLINE void mn_common(const MetricParams &M, T u, T sint, T cost, MNCommon<T> &c)
 {
@@ -575,7 +579,7 @@ FM_HD_INLINE float srcp(float x)
     return 1.0f / x;
 #endif
 }
-FM_HD_INLINE double srcp(double x) { return 1.0 / x; }
+FM_HD_INLINE double srcp(double x) { return fm_div(1.0, x); }
 
 // Kerr (M = 1) in closed form: the ten non-zero first derivatives of the
 // Boyer-Lindquist g_dd of FOORT/Metric.cpp:131-159 and g^uu of :162-191,
@@ -797,10 +801,11 @@ FM_HD_INLINE double ref_max(double a, double b) { return (a < b) ? b : a; }
 // Integrators::GetAdaptiveStep, FOORT/Integrators.cpp:34-47
 FM_HD_INLINE double adaptive_step(const RunConst &R, const double *pos, const double *vel)
 {
-    double d1 = R.eps / (__builtin_fabs(vel[1]) + 1e-20);
-    double d2 = R.eps * ref_min(pos[2], kPi - pos[2]) / (__builtin_fabs(vel[2]) + 1e-20);
-    double d3 = R.eps / (__builtin_fabs(vel[3] + 1e-20));
-    double h = 1 / (1 / __builtin_fabs(d1) + 1 / __builtin_fabs(d2) + 1 / __builtin_fabs(d3));
+    double d1 = fm_div(R.eps, __builtin_fabs(vel[1]) + 1e-20);
+    double d2 = fm_div(R.eps * ref_min(pos[2], kPi - pos[2]), __builtin_fabs(vel[2]) + 1e-20);
+    double d3 = fm_div(R.eps, __builtin_fabs(vel[3] + 1e-20));
+    double h = fm_div(1.0, fm_div(1.0, __builtin_fabs(d1)) + fm_div(1.0, __builtin_fabs(d2)) +
+                               fm_div(1.0, __builtin_fabs(d3)));
     return ref_max(h, R.minstep);
 }
 
@@ -814,6 +819,9 @@ FM_HD_INLINE double adaptive_step_fast(const RunConst &R, const double *pos, con
     const double m = ref_min(pos[2], kPi - pos[2]);
     const double den = ((__builtin_fabs(vel[1]) + 1e-20) + __builtin_fabs(vel[3] + 1e-20)) * m +
                        (__builtin_fabs(vel[2]) + 1e-20);
+    // the compiler's own division here: this quotient heads every step's chain of dependent
+    // instructions, and without v_div_scale_f64 between v_rcp_f64 and its first use the
+    // mixed kernel ran 5% slower on C1 (measured), for the same bits
     return ref_max(R.eps * m / den, R.minstep);
 }
 
@@ -880,7 +888,7 @@ FM_HD_INLINE void rk4_step(const RunConst &R, const double *x, const double *v, 
     eval(tp, l, k);
     // the reference divides (FOORT/Integrators.cpp:91-92); the analytic formulations multiply by
     // the reciprocal, one instruction instead of a correctly rounded division
-    const double h6 = RHS::exact ? h / 6.0 : h * (1.0 / 6.0);
+    const double h6 = RHS::exact ? fm_div(h, 6.0) : h * (1.0 / 6.0);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         nv[i] = v[i] + (sv[i] + k[i]) * h6;
