@@ -26,6 +26,7 @@ EXPORTS = [
     "foort_gpu_queue_open", "foort_gpu_queue_close", "foort_gpu_queue_last_error", "foort_gpu_queue_add",
     "foort_gpu_queue_remove", "foort_gpu_queue_push", "foort_gpu_queue_run", "foort_gpu_queue_progress",
     "foort_gpu_queue_results", "foort_gpu_trace_positions", "foort_positions_free",
+    "foort_gpu_check_division",
 ]
 
 
@@ -52,6 +53,7 @@ def lib():
         L.foort_gpu_trace_device.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_void_p,
                                              C.POINTER(GeodesicResults), C.c_void_p, C.POINTER(Stats)]
         L.foort_gpu_camera.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.foort_gpu_check_division.argtypes = [C.c_uint64] + [C.c_void_p] * 4
         L.foort_gpu_configure.argtypes = [C.c_void_p, C.POINTER(RunDesc)]
         L.foort_gpu_autotune.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p, C.c_uint32,
                                          C.POINTER(RunDesc), C.POINTER(Stats)]

@@ -1543,6 +1543,49 @@ int foort_gpu_synchronize(void)
     return e == hipSuccess ? FOORT_OK : hip_fail(nullptr, e, "hipDeviceSynchronize");
 }
 
+} // extern "C"
+
+namespace {
+
+__global__ void check_division(const double *a, const double *b, double *q_path, double *q_div, uint64_t n)
+{
+    const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (i < n) {
+        q_path[i] = foort::fm_div(a[i], b[i]);
+        q_div[i] = a[i] / b[i];
+    }
+}
+
+} // namespace
+
+extern "C" {
+
+int foort_gpu_check_division(uint64_t n, const double *a, const double *b, double *q_path, double *q_div)
+{
+    if (n == 0 || n > (uint64_t(1) << 26) || !a || !b || !q_path || !q_div)
+        return FOORT_EINVAL;
+    const size_t bytes = n * sizeof(double);
+    double *d = nullptr;
+    hipError_t e = hipMalloc(&d, 4 * bytes);
+    if (e != hipSuccess)
+        return hip_fail(nullptr, e, "hipMalloc");
+    e = hipMemcpy(d, a, bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(d + n, b, bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        check_division<<<static_cast<unsigned>((n + 255) / 256), 256>>>(d, d + n, d + 2 * n, d + 3 * n, n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipMemcpy(q_path, d + 2 * n, bytes, hipMemcpyDeviceToHost);
+    if (e == hipSuccess)
+        e = hipMemcpy(q_div, d + 3 * n, bytes, hipMemcpyDeviceToHost);
+    const hipError_t f = hipFree(d);
+    if (e == hipSuccess)
+        e = f;
+    return e == hipSuccess ? FOORT_OK : hip_fail(nullptr, e, "foort_gpu_check_division");
+}
+
 int foort_gpu_camera(const foort_gpu_ctx *ctx, double vielbein[16], double row0[4])
 {
     if (!ctx || !vielbein || !row0)

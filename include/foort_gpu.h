@@ -312,6 +312,13 @@ int foort_gpu_synchronize(void);
  * FOORT/ViewScreen.h:88) and row 0 of g_dd at the camera (FOORT/ViewScreen.h:86). */
 int foort_gpu_camera(const foort_gpu_ctx *ctx, double vielbein[16], double metric_dd_row0[4]);
 
+/* Self-check of the path's binary64 division on the device (no counterpart in the
+ * reference; it tests this library's own arithmetic): q_path[i] = a[i] / b[i] as the
+ * integration kernels divide (fm_div.hpp), q_div[i] = the compiler's a[i] / b[i].  Host
+ * arrays of n <= 2^26 elements; the two results are bit-identical unless an operand or the
+ * quotient lies near the ends of the exponent range. */
+int foort_gpu_check_division(uint64_t n, const double *a, const double *b, double *q_path, double *q_div);
+
 #ifdef __cplusplus
 }
 #endif
