@@ -18,6 +18,16 @@
 #ifndef FOORT_WPE_PERSIST
 #define FOORT_WPE_PERSIST 2
 #endif
+// One-shot kernels of the closed-form formulations: more waves per SIMD hide the latency of
+// their dependent chains better than the spills they cause cost.  Measured on MI355X on a
+// 1024x1024 Kerr image: binary64 3.3% faster at three waves (slower at four), mixed 8.6%
+// faster at four.
+#ifndef FOORT_WPE_ONESHOT_AN64
+#define FOORT_WPE_ONESHOT_AN64 3
+#endif
+#ifndef FOORT_WPE_ONESHOT_MIXED
+#define FOORT_WPE_ONESHOT_MIXED 4
+#endif
 // Manko-Novikov, Rasheed-Larsen and (except its finite-difference one-shot kernel) Johannsen:
 // their potentials with derivatives need more registers than two waves per SIMD leave, and
 // spills to scratch memory stall every step, worst for a lone wave (the time-outs that end an
@@ -40,7 +50,11 @@ constexpr int waves_per_eu(bool resident)
         return FOORT_WPE_HEAVY;
     if (RHS::metric == FOORT_METRIC_JOHANNSEN)
         return (RHS::exact && !resident) ? FOORT_WPE_ONESHOT : FOORT_WPE_HEAVY;
-    return resident ? FOORT_WPE_PERSIST : FOORT_WPE_ONESHOT;
+    if (resident)
+        return FOORT_WPE_PERSIST;
+    if (RHS::exact)
+        return FOORT_WPE_ONESHOT;
+    return RHS::split_rk4 ? FOORT_WPE_ONESHOT_MIXED : FOORT_WPE_ONESHOT_AN64;
 }
 
 struct DevResults {
