@@ -26,7 +26,7 @@ EXPORTS = [
     "foort_gpu_queue_open", "foort_gpu_queue_close", "foort_gpu_queue_last_error", "foort_gpu_queue_add",
     "foort_gpu_queue_remove", "foort_gpu_queue_push", "foort_gpu_queue_run", "foort_gpu_queue_progress",
     "foort_gpu_queue_results", "foort_gpu_trace_positions", "foort_positions_free",
-    "foort_gpu_check_division",
+    "foort_gpu_check_division", "foort_gpu_trace_device_noalloc",
 ]
 
 
@@ -52,6 +52,7 @@ def lib():
                                              C.POINTER(GeodesicResults), C.POINTER(Stats)]
         L.foort_gpu_trace_device.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_void_p,
                                              C.POINTER(GeodesicResults), C.c_void_p, C.POINTER(Stats)]
+        L.foort_gpu_trace_device_noalloc.argtypes = L.foort_gpu_trace_device.argtypes
         L.foort_gpu_camera.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.foort_gpu_check_division.argtypes = [C.c_uint64] + [C.c_void_p] * 4
         L.foort_gpu_configure.argtypes = [C.c_void_p, C.POINTER(RunDesc)]
@@ -156,16 +157,19 @@ class Context:
         out["stats"] = st
         return out
 
-    def trace_device(self, side, first, count, results, screen_index=None, stream=None, stats=True):
+    def trace_device(self, side, first, count, results, screen_index=None, stream=None, stats=True,
+                     noalloc=False):
         """Device-resident batch: `results` maps field name -> DeviceArray (or None);
-        `screen_index` is a DeviceArray of (row, column) pairs or None for grid indices."""
+        `screen_index` is a DeviceArray of (row, column) pairs or None for grid indices.
+        noalloc: through foort_gpu_trace_device_noalloc (work memory in the library's static
+        device array; the call allocates nothing)."""
         res = _results_struct(results, lambda t: C.c_void_p(t.ptr))
         si = C.c_void_p(screen_index.ptr) if screen_index is not None else None
         st = Stats() if stats else None
-        self._check(self._L.foort_gpu_trace_device(self._h, side, first, count, si, C.byref(res),
-                                                   C.c_void_p(stream) if stream else None,
-                                                   C.byref(st) if st is not None else None),
-                    "foort_gpu_trace_device")
+        fn = self._L.foort_gpu_trace_device_noalloc if noalloc else self._L.foort_gpu_trace_device
+        self._check(fn(self._h, side, first, count, si, C.byref(res), C.c_void_p(stream) if stream else None,
+                       C.byref(st) if st is not None else None),
+                    "foort_gpu_trace_device_noalloc" if noalloc else "foort_gpu_trace_device")
         return st
 
     def trace_positions(self, side, screen_index=None, count=None):

@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -19,6 +20,20 @@
 #include "foort_kernels.hpp"
 
 #pragma clang fp contract(off)
+
+// Work memory of foort_gpu_trace_device_noalloc, for callers that cannot allocate: the batch
+// counters (the first 256 bytes) and the lane slots of up to kScratchLanes resident lanes of
+// the persistent schedule (144 bytes each), 18 MiB in all.  Every GPU the library runs on holds
+// its own copy until the process ends; host code reaches it only through the address that
+// hipGetSymbolAddress returns for the current device (scratch_address below), never through
+// the array's name.
+namespace {
+constexpr uint32_t kScratchLanes = 1u << 17;
+constexpr size_t kScratchCounterBytes = 256;
+constexpr size_t kScratchBytes =
+    kScratchCounterBytes + size_t(kScratchLanes) * (foort::kSlotD * 8 + 8 + foort::kSlotU * 4);
+} // namespace
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
 using foort::DevResults;
 using foort::GeoState;
@@ -292,6 +307,10 @@ struct foort_gpu_ctx {
     hipGraphExec_t graph = nullptr;
     std::string graph_key;
     std::string graph_shape, seen_shape; // batch shapes: the captured one, the last one run directly
+    // the device address of `scratch` on this context's device (NULL, with the reason in
+    // scratch_err, if it could not be found)
+    char *scratch = nullptr;
+    std::string scratch_err;
 };
 
 namespace {
@@ -371,19 +390,18 @@ void apply_lane_options(foort_gpu_ctx *ctx)
     ctx->slots.tail = 0;
 }
 
-// geodesics left to the tail kernel (0: none)
-uint64_t tail_threshold(const foort_gpu_ctx *ctx)
+// geodesics left to the tail kernel (0: none), for L resident lanes
+uint64_t tail_threshold(const foort_gpu_ctx *ctx, uint32_t L)
 {
     if (!ctx->kernels.tail || ctx->desc.tail_geodesics == UINT32_MAX)
         return 0;
-    return ctx->desc.tail_geodesics ? ctx->desc.tail_geodesics : std::max<uint32_t>(1, ctx->slots.L / 8);
+    return ctx->desc.tail_geodesics ? ctx->desc.tail_geodesics : std::max<uint32_t>(1, L / 8);
 }
 
-int setup_persistent(foort_gpu_ctx *ctx)
+// Resident lanes of the persistent schedule: lanes_per_cu, or as many as the persistent
+// kernel's occupancy allows, on every CU; a multiple of 256 (the workgroup size).
+int lane_count(foort_gpu_ctx *ctx, uint64_t &L)
 {
-    apply_lane_options(ctx);
-    if (ctx->d_slots)
-        return FOORT_OK;
     uint32_t lanes_per_cu = ctx->desc.lanes_per_cu;
     if (lanes_per_cu == 0) {
         int blocks = 0;
@@ -391,21 +409,63 @@ int setup_persistent(foort_gpu_ctx *ctx)
                          &blocks, reinterpret_cast<const void *>(ctx->kernels.persistent), 256, 0));
         lanes_per_cu = static_cast<uint32_t>(std::max(1, blocks)) * 256u;
     }
-    uint64_t L = static_cast<uint64_t>(ctx->num_cu) * lanes_per_cu;
+    L = static_cast<uint64_t>(ctx->num_cu) * lanes_per_cu;
     L = (L + 255) / 256 * 256;
     if (L == 0 || L > (1u << 24))
         return fail(ctx, FOORT_EINVAL, "persistent lane count out of range");
-    const size_t bytes = L * (foort::kSlotD * 8 + foort::kSlotU * 4 + 8);
-    hipError_t e = hipMalloc(&ctx->d_slots, bytes);
+    return FOORT_OK;
+}
+
+// bytes of lane state per resident lane: kSlotD doubles, the pixel index, kSlotU words
+constexpr size_t kLaneBytes = foort::kSlotD * 8 + 8 + foort::kSlotU * 4;
+
+// Point the lane slots of S at L lanes of state starting at `base` (256-byte aligned), with
+// the claim and done counters of the batch counters c.
+void place_slots(LaneSlots &S, char *base, uint64_t L, unsigned long long *c)
+{
+    S.d = reinterpret_cast<double *>(base);
+    S.pix = reinterpret_cast<long long *>(base + L * foort::kSlotD * 8);
+    S.u = reinterpret_cast<uint32_t *>(base + L * (foort::kSlotD * 8 + 8));
+    S.next = c + 1;
+    S.done = c + 2;
+    S.L = static_cast<uint32_t>(L);
+}
+
+int setup_persistent(foort_gpu_ctx *ctx)
+{
+    apply_lane_options(ctx);
+    if (ctx->d_slots)
+        return FOORT_OK;
+    uint64_t L = 0;
+    int rc = lane_count(ctx, L);
+    if (rc)
+        return rc;
+    hipError_t e = hipMalloc(&ctx->d_slots, L * kLaneBytes);
     if (e != hipSuccess)
         return fail(ctx, FOORT_ENOMEM, std::string("hipMalloc of lane slots: ") + hipGetErrorString(e));
-    char *b = static_cast<char *>(ctx->d_slots);
-    ctx->slots.d = reinterpret_cast<double *>(b);
-    ctx->slots.pix = reinterpret_cast<long long *>(b + L * foort::kSlotD * 8);
-    ctx->slots.u = reinterpret_cast<uint32_t *>(b + L * (foort::kSlotD * 8 + 8));
-    ctx->slots.next = ctx->d_counters + 1;
-    ctx->slots.done = ctx->d_counters + 2;
-    ctx->slots.L = static_cast<uint32_t>(L);
+    place_slots(ctx->slots, static_cast<char *>(ctx->d_slots), L, ctx->d_counters);
+    return FOORT_OK;
+}
+
+// The device memory a batch's kernels use besides its inputs and results: the batch
+// counters (steps, claimed and finished pixels, warnings; kNumCounters) and, for the
+// persistent schedule, the lane slots and the schedule's options.
+struct Work {
+    unsigned long long *counters;
+    LaneSlots slots;
+};
+
+// The context's own work memory (the persistent schedule's lane slots are allocated on
+// first use).
+int ctx_work(foort_gpu_ctx *ctx, Work &w)
+{
+    if (ctx->desc.schedule == FOORT_SCHEDULE_PERSISTENT) {
+        int rc = setup_persistent(ctx);
+        if (rc)
+            return rc;
+    }
+    w.counters = ctx->d_counters;
+    w.slots = ctx->slots;
     return FOORT_OK;
 }
 
@@ -416,14 +476,68 @@ bool single_launch(const foort_gpu_ctx *ctx)
            (ctx->desc.steps_per_launch == 0 && ctx->desc.launch_us == 0);
 }
 
-// Queue the kernels of one batch on `stream`.  A batch that is one launch is only queued;
-// otherwise the persistent schedule relaunches until the device counts every pixel done.
-int launch_trace(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, uint64_t count, const uint64_t *si,
-                 const DevResults &out, hipStream_t stream)
+// The address of `scratch` on each device, looked up once per device with that device current.
+std::mutex g_scratch_mu;
+std::map<int, char *> g_scratch_at;
+
+hipError_t scratch_address(int device, char **out)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_at.find(device);
+    if (it != g_scratch_at.end()) {
+        *out = it->second;
+        return hipSuccess;
+    }
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return e != hipSuccess ? e : hipErrorInvalidSymbol;
+    g_scratch_at[device] = static_cast<char *>(p);
+    *out = static_cast<char *>(p);
+    return hipSuccess;
+}
+
+// The work memory in `scratch` (no allocation): the counters, and for the persistent schedule
+// the lane slots of at most kScratchLanes lanes -- the occupancy's lane count capped there, or
+// lanes_per_cu on every CU, which must fit.
+int scratch_work(foort_gpu_ctx *ctx, Work &w)
+{
+    if (!ctx->scratch)
+        return fail(ctx, FOORT_EDEVICE, "no-allocation work memory unavailable: " + ctx->scratch_err);
+    int dev = -1;
+    HIP_TRY(ctx, hipGetDevice(&dev));
+    if (dev != ctx->device)
+        return fail(ctx, FOORT_EINVAL, "the current device is not the context's");
+    w.counters = reinterpret_cast<unsigned long long *>(ctx->scratch);
+    w.slots = ctx->slots;
+    if (ctx->desc.schedule != FOORT_SCHEDULE_PERSISTENT)
+        return FOORT_OK;
+    apply_lane_options(ctx);
+    w.slots = ctx->slots;
+    uint64_t L = 0;
+    int rc = lane_count(ctx, L);
+    if (rc)
+        return rc;
+    if (L > kScratchLanes) {
+        if (ctx->desc.lanes_per_cu)
+            return fail(ctx, FOORT_EINVAL,
+                        "lanes_per_cu gives " + std::to_string(L) + " resident lanes; the no-allocation entry point "
+                        "holds at most " + std::to_string(kScratchLanes));
+        L = kScratchLanes;
+    }
+    place_slots(w.slots, ctx->scratch + kScratchCounterBytes, L, w.counters);
+    return FOORT_OK;
+}
+
+// Queue the kernels of one batch on `stream`, with the work memory w.  A batch that is one
+// launch is only queued; otherwise the persistent schedule relaunches until the device counts
+// every pixel done.
+int launch_trace(foort_gpu_ctx *ctx, const Work &w, uint64_t side, uint64_t first, uint64_t count,
+                 const uint64_t *si, const DevResults &out, hipStream_t stream)
 {
     ctx->last_launches = 0;
     if (ctx->desc.schedule == FOORT_SCHEDULE_ONESHOT) {
-        hipLaunchKernelGGL(zero_counters, dim3(1), dim3(64), 0, stream, ctx->d_counters, foort::kNumCounters);
+        hipLaunchKernelGGL(zero_counters, dim3(1), dim3(64), 0, stream, w.counters, foort::kNumCounters);
         HIP_TRY(ctx, hipGetLastError());
         if (count == 0)
             return FOORT_OK;
@@ -434,16 +548,13 @@ int launch_trace(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, uint64_t cou
         if (grid > 0x7fffffffull)
             return fail(ctx, FOORT_EINVAL, "batch too large");
         hipLaunchKernelGGL(ctx->kernels.oneshot, dim3(static_cast<unsigned>(grid)), dim3(block), 0, stream, ctx->rc,
-                           side, first, count, si, out, ctx->d_steps, ctx->slots, uint64_t(0));
+                           side, first, count, si, out, w.counters, w.slots, uint64_t(0));
         HIP_TRY(ctx, hipGetLastError());
         ctx->last_launches = 1;
         return FOORT_OK;
     }
-    int rc = setup_persistent(ctx);
-    if (rc)
-        return rc;
-    const LaneSlots &S = ctx->slots;
-    hipLaunchKernelGGL(reset_lanes, dim3(S.L / 256), dim3(256), 0, stream, S, ctx->d_steps);
+    const LaneSlots &S = w.slots;
+    hipLaunchKernelGGL(reset_lanes, dim3(S.L / 256), dim3(256), 0, stream, S, w.counters);
     HIP_TRY(ctx, hipGetLastError());
     if (count == 0)
         return FOORT_OK;
@@ -451,14 +562,14 @@ int launch_trace(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, uint64_t cou
         // no step or time limit: the launch runs until every pixel is done, or until the last
         // few are left to the four-lane tail kernel
         LaneSlots St = S;
-        St.tail = tail_threshold(ctx);
+        St.tail = tail_threshold(ctx, S.L);
         hipLaunchKernelGGL(ctx->kernels.persistent, dim3(S.L / 256), dim3(256), 0, stream, ctx->rc, side, first,
-                           count, si, out, ctx->d_steps, St, ctx->budget);
+                           count, si, out, w.counters, St, ctx->budget);
         HIP_TRY(ctx, hipGetLastError());
         ctx->last_launches = 1;
         if (St.tail) {
             hipLaunchKernelGGL(ctx->kernels.tail, dim3(S.L / 64), dim3(256), 0, stream, ctx->rc, side, first, count,
-                               si, out, ctx->d_steps, St, ctx->budget);
+                               si, out, w.counters, St, ctx->budget);
             HIP_TRY(ctx, hipGetLastError());
             ctx->last_launches = 2;
         }
@@ -475,7 +586,7 @@ int launch_trace(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, uint64_t cou
         if (n >= max_launches)
             return fail(ctx, FOORT_EDEVICE, "persistent schedule did not finish within its launch bound");
         hipLaunchKernelGGL(ctx->kernels.persistent, dim3(S.L / 256), dim3(256), 0, stream, ctx->rc, side, first,
-                           count, si, out, ctx->d_steps, S, ctx->budget);
+                           count, si, out, w.counters, S, ctx->budget);
         HIP_TRY(ctx, hipGetLastError());
         ++n;
         HIP_TRY(ctx, hipMemcpyAsync(&done, S.done, sizeof(done), hipMemcpyDeviceToHost, stream));
@@ -487,19 +598,18 @@ int launch_trace(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, uint64_t cou
 
 // A batch's integration, then the passes that count its warnings (and its non-finite results
 // when the final positions and velocities are kept).
-int launch(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, uint64_t count, const uint64_t *si,
+int launch(foort_gpu_ctx *ctx, const Work &w, uint64_t side, uint64_t first, uint64_t count, const uint64_t *si,
            const DevResults &out, hipStream_t stream)
 {
-    int rc = launch_trace(ctx, side, first, count, si, out, stream);
+    int rc = launch_trace(ctx, w, side, first, count, si, out, stream);
     if (rc || count == 0)
         return rc;
     const unsigned grid = static_cast<unsigned>((count + 255) / 256);
     hipLaunchKernelGGL(count_ic_warnings, dim3(grid), dim3(256), 0, stream, ctx->rc, side, first, count, si,
-                       ctx->d_counters);
+                       w.counters);
     HIP_TRY(ctx, hipGetLastError());
     if (out.pos && out.vel) {
-        hipLaunchKernelGGL(count_nonfinite, dim3(grid), dim3(256), 0, stream, out.pos, out.vel, count,
-                           ctx->d_counters);
+        hipLaunchKernelGGL(count_nonfinite, dim3(grid), dim3(256), 0, stream, out.pos, out.vel, count, w.counters);
         HIP_TRY(ctx, hipGetLastError());
     }
     return FOORT_OK;
@@ -556,6 +666,11 @@ int trace_host(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, uint64_t count
     d.vel = static_cast<double *>(dptr(f[7]));
     const uint64_t *d_si = si ? reinterpret_cast<const uint64_t *>(base + o_si) : nullptr;
 
+    // the counters and lane slots (allocated here on first use: never inside a capture)
+    Work w{};
+    rc = ctx_work(ctx, w);
+    if (rc)
+        return rc;
     bool graph = ctx->desc.use_graph && single_launch(ctx) && count > 0;
     std::string shape;
     if (graph) {
@@ -577,7 +692,7 @@ int trace_host(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, uint64_t count
         if (si)
             HIP_TRY(ctx, hipMemcpyAsync(base + o_si, si, count * 16, hipMemcpyHostToDevice, ctx->stream));
         HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-        rc = launch(ctx, side, first, count, d_si, d, ctx->stream);
+        rc = launch(ctx, w, side, first, count, d_si, d, ctx->stream);
         if (rc)
             return rc;
         HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
@@ -589,11 +704,6 @@ int trace_host(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, uint64_t count
         rc = ensure_pinned(ctx, off);
         if (rc)
             return rc;
-        if (ctx->desc.schedule == FOORT_SCHEDULE_PERSISTENT) {
-            rc = setup_persistent(ctx); // allocates: not inside the capture
-            if (rc)
-                return rc;
-        }
         if (!ctx->stream)
             HIP_TRY(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
         char *hb = static_cast<char *>(ctx->h_buf);
@@ -612,7 +722,7 @@ int trace_host(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, uint64_t count
             auto capture = [&]() -> int {
                 if (si)
                     HIP_TRY(ctx, hipMemcpyAsync(base + o_si, hb + o_si, count * 16, hipMemcpyHostToDevice, ctx->stream));
-                int r = launch(ctx, side, first, count, d_si, d, ctx->stream);
+                int r = launch(ctx, w, side, first, count, d_si, d, ctx->stream);
                 if (r)
                     return r;
                 for (const Field &x : f)
@@ -653,7 +763,7 @@ int trace_host(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, uint64_t count
     }
     if (stats) {
         unsigned long long c[foort::kNumCounters] = {};
-        HIP_TRY(ctx, hipMemcpyAsync(c, ctx->d_counters, sizeof c, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(ctx, hipMemcpyAsync(c, w.counters, sizeof c, hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
         float ms = 0;
         HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
@@ -797,6 +907,46 @@ std::string group_key(const foort_run_desc &d, const RunConst &R, const KernelPa
     return key;
 }
 
+// foort_gpu_trace_device and its no-allocation variant: the same kernels, with the context's
+// work memory or with `scratch`.
+int trace_device(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, uint64_t count, const uint64_t *screen_index,
+                 const foort_geodesic_results *out, void *stream, foort_stats *stats, bool noalloc)
+{
+    if (!ctx)
+        return fail(nullptr, FOORT_EINVAL, "NULL context");
+    if (!out)
+        return fail(ctx, FOORT_EINVAL, "results pointer is NULL");
+    int rc = check_batch(ctx, side, first, count, screen_index == nullptr);
+    if (rc)
+        return rc;
+    Work w{};
+    rc = noalloc ? scratch_work(ctx, w) : ctx_work(ctx, w);
+    if (rc)
+        return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    DevResults d{out->term, out->quadrant, out->equat_passes, out->closest_radius,
+                 out->final_pos, out->final_vel, out->lambda, out->steps};
+    if (stats)
+        HIP_TRY(ctx, hipEventRecord(ctx->ev0, s));
+    rc = launch(ctx, w, side, first, count, screen_index, d, s);
+    if (rc)
+        return rc;
+    if (stats) {
+        HIP_TRY(ctx, hipEventRecord(ctx->ev1, s));
+        HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+        unsigned long long c[foort::kNumCounters] = {};
+        HIP_TRY(ctx, hipMemcpy(c, w.counters, sizeof c, hipMemcpyDeviceToHost));
+        float ms = 0;
+        HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        stats->geodesics = count;
+        stats->steps = c[0];
+        stats->launches = ctx->last_launches;
+        stats->device_ms = ms;
+        set_warnings(stats, c);
+    }
+    return FOORT_OK;
+}
+
 } // namespace
 
 struct foort_gpu_queue {
@@ -891,6 +1041,15 @@ int foort_gpu_open(const foort_run_desc *desc, foort_gpu_ctx **out)
         e = hipEventCreate(&ctx->ev1);
     if (e == hipSuccess)
         e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) {
+        // the no-allocation entry point's work memory on this device (only that entry point
+        // needs it, so a failure here is reported there)
+        const hipError_t es = scratch_address(ctx->device, &ctx->scratch);
+        if (es != hipSuccess) {
+            ctx->scratch = nullptr;
+            ctx->scratch_err = std::string("hipGetSymbolAddress: ") + hipGetErrorString(es);
+        }
+    }
     if (e != hipSuccess) {
         std::string msg = std::string("HIP initialisation: ") + hipGetErrorString(e);
         foort_gpu_close(ctx);
@@ -965,35 +1124,14 @@ int foort_gpu_trace_device(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, ui
                            const uint64_t *screen_index, const foort_geodesic_results *out, void *stream,
                            foort_stats *stats)
 {
-    if (!ctx)
-        return fail(nullptr, FOORT_EINVAL, "NULL context");
-    if (!out)
-        return fail(ctx, FOORT_EINVAL, "results pointer is NULL");
-    int rc = check_batch(ctx, side, first, count, screen_index == nullptr);
-    if (rc)
-        return rc;
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    DevResults d{out->term, out->quadrant, out->equat_passes, out->closest_radius,
-                 out->final_pos, out->final_vel, out->lambda, out->steps};
-    if (stats)
-        HIP_TRY(ctx, hipEventRecord(ctx->ev0, s));
-    rc = launch(ctx, side, first, count, screen_index, d, s);
-    if (rc)
-        return rc;
-    if (stats) {
-        HIP_TRY(ctx, hipEventRecord(ctx->ev1, s));
-        HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
-        unsigned long long c[foort::kNumCounters] = {};
-        HIP_TRY(ctx, hipMemcpy(c, ctx->d_counters, sizeof c, hipMemcpyDeviceToHost));
-        float ms = 0;
-        HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-        stats->geodesics = count;
-        stats->steps = c[0];
-        stats->launches = ctx->last_launches;
-        stats->device_ms = ms;
-        set_warnings(stats, c);
-    }
-    return FOORT_OK;
+    return trace_device(ctx, side, first, count, screen_index, out, stream, stats, false);
+}
+
+int foort_gpu_trace_device_noalloc(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, uint64_t count,
+                                   const uint64_t *screen_index, const foort_geodesic_results *out, void *stream,
+                                   foort_stats *stats)
+{
+    return trace_device(ctx, side, first, count, screen_index, out, stream, stats, true);
 }
 
 int foort_gpu_autotune(foort_gpu_ctx *ctx, uint64_t side, uint64_t count, const uint64_t *screen_index,

@@ -224,6 +224,20 @@ int foort_gpu_trace_device(foort_gpu_ctx *ctx, uint64_t row_column_size, uint64_
                            uint64_t count, const uint64_t *screen_index,
                            const foort_geodesic_results *out, void *stream, foort_stats *stats);
 
+/* No counterpart in the reference: foort_gpu_trace_device for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, kernels and results; the call allocates nothing: the batch
+ * counters and the persistent schedule's lane slots live in a static device array of 18 MiB
+ * per GPU (held until the process ends) instead of the context's allocations.  The persistent
+ * schedule runs at most 131072 resident lanes there (the occupancy's lane count is capped at
+ * that; a lanes_per_cu that asks for more fails with FOORT_EINVAL).  Call it with the
+ * context's device current.  Every call on one GPU shares that array and leaves its contents
+ * behind (each call clears what must start at zero): calls that overlap on one GPU, from
+ * other threads or on other streams, race and must not be made. */
+int foort_gpu_trace_device_noalloc(foort_gpu_ctx *ctx, uint64_t row_column_size, uint64_t first,
+                                   uint64_t count, const uint64_t *screen_index,
+                                   const foort_geodesic_results *out, void *stream, foort_stats *stats);
+
 /* Autotuner (no counterpart in the reference): time a sample batch -- screen_index as for
  * foort_gpu_trace_pixels (host memory), or NULL for grid indices 0 .. count-1 -- under
  * candidate execution options and keep the fastest in the context: the one-shot block size
