@@ -5,6 +5,7 @@ finite-difference path must round exactly like the CPU oracle.  Packed FP32
 instructions are disabled (-packed-fp32-ops).
 """
 import os
+import re
 import shutil
 import subprocess
 import sys
@@ -35,13 +36,83 @@ def hipcc():
     return exe
 
 
-def _run(cmd, verbose):
+def _run(cmd, verbose, capture=False):
     if verbose:
         print(" ".join(cmd), flush=True)
+    if capture:
+        r = subprocess.run(cmd, stderr=subprocess.PIPE, text=True)
+        if r.returncode:
+            sys.stderr.write(r.stderr)
+            raise subprocess.CalledProcessError(r.returncode, cmd)
+        return r.stderr
     subprocess.run(cmd, check=True)
+    return ""
 
 
-def build_gpu(extra=(), out=LIB, verbose=False):
+RESOURCES = os.path.join(REPO, "profiles", "kernel_resources.md")
+_REMARK = re.compile(r"remark:\s+(Function Name|TotalSGPRs|VGPRs|AGPRs|ScratchSize \[bytes/lane\]|"
+                     r"Occupancy \[waves/SIMD\]|SGPRs Spill|VGPRs Spill|LDS Size \[bytes/block\]):\s*(\S+)")
+
+
+def _census(remarks):
+    """Kernel resource usage from the compiler's kernel-resource-usage remarks: one dict per kernel."""
+    out, cur = [], None
+    for line in remarks.splitlines():
+        m = _REMARK.search(line)
+        if not m:
+            continue
+        key, val = m.groups()
+        if key == "Function Name":
+            cur = {"name": val}
+            out.append(cur)
+        elif cur is not None:
+            cur[key] = val
+    return out
+
+
+def _demangle(names):
+    try:
+        r = subprocess.run(["c++filt"], input="\n".join(names), capture_output=True, text=True, check=True)
+        return r.stdout.splitlines()
+    except (OSError, subprocess.CalledProcessError):
+        return list(names)
+
+
+def write_census(units, path=RESOURCES):
+    """profiles/kernel_resources.md: registers, scratch and occupancy of every kernel, as the
+    compiler reports them for this build (units: [(source, remarks text)])."""
+    rows = []
+    for src, text in units:
+        ks = _census(text)
+        for k, dn in zip(ks, _demangle([k["name"] for k in ks])):
+            dn = re.sub(r"\(.*$", "", dn.replace("foort::", "").replace("void ", ""))
+            dn = dn.replace("(anonymous namespace)::", "")
+            rows.append((src, dn, k))
+    spill = [r for r in rows if int(r[2].get("ScratchSize [bytes/lane]", "0")) > 0]
+    lines = ["# Kernel resources of this build (every kernel of libfoort_gpu.so)", "",
+             "Registers, scratch memory and occupancy as the compiler reports them, written by the build",
+             "(`python foort_amd/build.py`, which compiles with `-Rpass-analysis=kernel-resource-usage`).",
+             "Template arguments: right-hand side policy `<metric>` (0 Kerr, 1 FlatSpace, 2 Rasheed-Larsen,",
+             "3 Johannsen, 4 Manko-Novikov), integrator (0 RK4, 1 Verlet), and `true` when every termination",
+             "and diagnostic updates on every step (no update counters).", "",
+             f"{len(rows)} kernels; {len(spill)} use scratch memory (register spills to memory):", ""]
+    for src, dn, k in spill:
+        lines.append(f"* `{dn}`: {k.get('ScratchSize [bytes/lane]')} B/lane "
+                     f"({k.get('VGPRs Spill')} VGPR, {k.get('SGPRs Spill')} SGPR spills), "
+                     f"{k.get('Occupancy [waves/SIMD]')} waves/SIMD")
+    lines += ["", "| unit | kernel | VGPRs | AGPRs | SGPRs | scratch B/lane | waves/SIMD | VGPR spills | SGPR spills |",
+              "|---|---|---|---|---|---|---|---|---|"]
+    for src, dn, k in rows:
+        lines.append(f"| {src} | `{dn}` | {k.get('VGPRs')} | {k.get('AGPRs')} | {k.get('TotalSGPRs')} | "
+                     f"{k.get('ScratchSize [bytes/lane]')} | {k.get('Occupancy [waves/SIMD]')} | "
+                     f"{k.get('VGPRs Spill')} | {k.get('SGPRs Spill')} |")
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    return rows
+
+
+def build_gpu(extra=(), out=LIB, verbose=False, census=True):
     from concurrent.futures import ThreadPoolExecutor
     os.makedirs(BUILD_DIR, exist_ok=True)
     jobs = []
@@ -49,13 +120,14 @@ def build_gpu(extra=(), out=LIB, verbose=False):
     for src, contract in SOURCES:
         obj = os.path.join(BUILD_DIR, os.path.splitext(src)[0] + ".o")
         objs.append(obj)
-        jobs.append([hipcc()] + COMMON + ["-ffp-contract=" + contract] + list(extra)
-                    + ["-c", os.path.join(CSRC, src), "-o", obj])
+        jobs.append([hipcc()] + COMMON + ["-ffp-contract=" + contract, "-Rpass-analysis=kernel-resource-usage"]
+                    + list(extra) + ["-c", os.path.join(CSRC, src), "-o", obj])
     with ThreadPoolExecutor(len(jobs)) as ex:
-        for f in [ex.submit(_run, j, verbose) for j in jobs]:
-            f.result()
+        remarks = [f.result() for f in [ex.submit(_run, j, verbose, True) for j in jobs]]
     _run([hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC"] + objs + ["-o", out + ".tmp"], verbose)
     os.replace(out + ".tmp", out)
+    if census and out == LIB and not extra:
+        write_census([(src, r) for (src, _), r in zip(SOURCES, remarks)])
     return out
 
 
