@@ -384,12 +384,34 @@ int run(const char *cfg_path, const foort_program_options &o)
                                           Level::Warning);
                     }
                 }
-                foort_stats st{};
-                rcode = grid ? foort_gpu_trace_grid(gpu, side, 0, n, &view, &st)
-                             : foort_gpu_trace_pixels(gpu, side, n, idx.data(), &view, &st);
-                if (rcode)
-                    set_error(std::string("GPU path: ") + foort_gpu_last_error(gpu));
-                add_stats(total, st);
+                // With Output.LoopMessageFrequency set, the iteration is integrated in chunks of
+                // about that many geodesics (at least 65,536, so the GPU stays full) and the
+                // reference's progress line follows each chunk (FOORT/Main.cpp:354-371); each
+                // geodesic's result does not depend on the chunks.
+                const unsigned long freq = loop_message_frequency();
+                const uint64_t chunk =
+                    freq == ~0UL ? n : std::max<uint64_t>(uint64_t(1) << 16, (freq + 255) / 256 * 256);
+                auto t_loop = std::chrono::steady_clock::now();
+                for (uint64_t a = 0; a < n && rcode == FOORT_OK; a += chunk) {
+                    const uint64_t c = std::min(chunk, n - a);
+                    foort_geodesic_results part = view;
+                    part.term += a, part.quadrant += a, part.equat_passes += a, part.closest_radius += a;
+                    foort_stats st{};
+                    rcode = grid ? foort_gpu_trace_grid(gpu, side, a, c, &part, &st)
+                                 : foort_gpu_trace_pixels(gpu, side, c, idx.data() + 2 * a, &part, &st);
+                    if (rcode)
+                        set_error(std::string("GPU path: ") + foort_gpu_last_error(gpu));
+                    add_stats(total, st);
+                    if (!rcode && a + c < n) {
+                        const double el = seconds_since(t_loop), speed = double(a + c) / el;
+                        screen_output("Approx. at geodesic " + std::to_string(static_cast<long long>(a + c)) + " (" +
+                                          std::to_string(el) + "s elapsed; speed: " +
+                                          std::to_string(static_cast<long>(speed)) +
+                                          " geod/s; est. loop time remaining: " +
+                                          std::to_string(double(n - a - c) / speed) + "s)...",
+                                      Level::Subproc);
+                    }
+                }
             }
         }
         if (rcode != FOORT_OK) {

@@ -51,6 +51,29 @@ std::string group_key(const foort_run_desc &d)
 
 struct Group;
 
+// The output handler the program would build for this run description and mesh with
+// Output.FilePrefix = prefix, FileExtension = ext, TimeStamp = false, FirstLineInfo = true
+// (FOORT/Config.cpp:1079-1155): one file per diagnostic in the order of CreateDiagnosticVector.
+std::unique_ptr<OutputHandler> make_output(const foort_run_desc &d, const foort_mesh_desc &m, const Mesh &mesh,
+                                           const std::string &prefix, const std::string &ext)
+{
+    RunConfig rc;
+    rc.desc = d;
+    rc.mesh = m;
+    rc.integrator_name = d.integrator == FOORT_INTEGRATOR_VERLET ? "Verlet" : "RK4";
+    const std::vector<uint32_t> order = diagnostic_order(d.diag_flags, d.value_diag);
+    std::vector<std::string> names;
+    std::vector<char> kinds;
+    for (uint32_t f : order) {
+        names.push_back(diag_name(f));
+        kinds.push_back(f == FOORT_DIAGFLAG_CLOSESTRADIUS ? OutputHandler::kFixed : OutputHandler::kInt);
+    }
+    auto out = std::make_unique<OutputHandler>(prefix, "", ext, names, ~0UL - 1, ~0UL,
+                                               first_line_info(rc, mesh.getFullDescriptionStr()));
+    out->setValueKinds(kinds);
+    return out;
+}
+
 struct Req {
     foort_request req{};
     foort_request_result res{};
@@ -65,6 +88,10 @@ struct Req {
     uint64_t steps_seen = 0;       // the slot's step count at the last look
     std::vector<int32_t> term, quad, passes;
     std::vector<double> closest;
+    // output files (when the request names a prefix): the handler and the diagnostics' order
+    std::string out_prefix, out_ext;
+    std::unique_ptr<OutputHandler> out;
+    std::vector<uint32_t> order;
 };
 
 struct Group {
@@ -136,23 +163,38 @@ struct foort_server {
     void mesh_step(Req *r, bool build)
     {
         try {
+            auto value = [r](uint32_t diag, uint64_t i) -> double {
+                return diag == FOORT_DIAGFLAG_FOURCOLORSCREEN    ? r->quad[i]
+                       : diag == FOORT_DIAGFLAG_EQUATORIALPASSES ? r->passes[i]
+                                                                 : r->closest[i];
+            };
             if (build) {
                 r->res.start_s = now();
                 r->mesh = make_mesh(r->req.mesh);
-                r->vd = diagnostic_order(r->req.desc.diag_flags, r->req.desc.value_diag).front();
+                r->order = diagnostic_order(r->req.desc.diag_flags, r->req.desc.value_diag);
+                r->vd = r->order.front();
+                if (!r->out_prefix.empty())
+                    r->out = make_output(r->req.desc, r->req.mesh, *r->mesh, r->out_prefix, r->out_ext);
             } else {
-                for (uint64_t i = 0; i < r->n; ++i)
-                    r->mesh->GeodesicFinished(
-                        i, MeshValue{r->vd == FOORT_DIAGFLAG_FOURCOLORSCREEN    ? r->quad[i]
-                                     : r->vd == FOORT_DIAGFLAG_EQUATORIALPASSES ? r->passes[i]
-                                                                                : r->closest[i],
-                                     0});
+                if (r->out)
+                    r->out->PrepareForOutput(r->n);
+                for (uint64_t i = 0; i < r->n; ++i) {
+                    r->mesh->GeodesicFinished(i, MeshValue{value(r->vd, i), 0});
+                    if (r->out) {
+                        double vals[4];
+                        for (size_t k = 0; k < r->order.size(); ++k)
+                            vals[k] = value(r->order[k], i);
+                        r->out->NewGeodesicValues(i, r->idx[2 * i], r->idx[2 * i + 1], vals);
+                    }
+                }
                 r->mesh->EndCurrentLoop();
                 r->res.pixels += r->n;
                 ++r->res.iterations;
             }
             for (;;) {
                 if (r->mesh->IsFinished()) {
+                    if (r->out)
+                        r->out->OutputFinished();
                     r->finished = true;
                     break;
                 }
@@ -374,6 +416,11 @@ int foort_server_submit(foort_server *srv, const foort_request *req)
     r->req = *req;
     r->req.mesh.pixels = nullptr; // a request's mesh is adaptive or a grid; pixel lists are not kept
     r->req.mesh.pixel_count = 0;
+    if (req->output_prefix && *req->output_prefix) {
+        r->out_prefix = req->output_prefix;
+        r->out_ext = req->output_extension ? req->output_extension : "";
+    }
+    r->req.output_prefix = r->req.output_extension = nullptr;
     r->res.tag = req->tag;
     {
         std::lock_guard<std::mutex> lk(srv->mu);

@@ -65,7 +65,8 @@ class ProgramOptions(C.Structure):
 
 class Request(C.Structure):
     """foort_request (include/foort_host.h)"""
-    _fields_ = [("desc", RunDesc), ("mesh", MeshDesc), ("tag", C.c_uint64)]
+    _fields_ = [("desc", RunDesc), ("mesh", MeshDesc), ("tag", C.c_uint64), ("output_prefix", C.c_char_p),
+                ("output_extension", C.c_char_p)]
 
 
 class RequestResult(C.Structure):
@@ -250,6 +251,91 @@ class Output:
         _check(lib().foort_output_finished(self._h))
 
 
+_METRIC_CFG = {0: ("Kerr", ("a",)), 1: ("FlatSpace", ()), 2: ("RasheedLarsen", ("m", "a", "p", "q")),
+               3: ("Johannsen", ("a", "alpha13", "alpha22", "alpha52", "epsilon3")),
+               4: ("MankoNovikov", ("a", "alpha3"))}
+_MESH_CFG = {MESH_SIMPLESQUARE: "SimpleSquareMesh", MESH_INPUTCERTAINPIXELS: "InputCertainPixelsMesh",
+             MESH_SQUARESUBDIVISION: "SquareSubdivisionMesh", MESH_SQUARESUBDIVISIONV2: "SquareSubdivisionMeshV2"}
+
+
+def config_text(desc, mesh, prefix=None, extension="dat"):
+    """A configuration file (the reference's option names, FOORT/Config.cpp) that the program
+    reads back into this run description and mesh; output files `prefix`_<name>.`extension`
+    without a time stamp (console output when prefix is None)."""
+    from . import abi as A
+
+    def num(x):
+        return repr(float(x))
+
+    def cnt(n):
+        n = int(n)
+        return f"{n}L" if n >= 2 ** 31 else str(n)
+
+    def upd(u, gp=False):
+        return (f"UpdateFrequency = {cnt(u.update_n_steps)}; UpdateStart = {'true' if u.update_start else 'false'}; "
+                f"UpdateFinish = {'true' if u.update_finish else 'false'};")
+
+    name, pnames = _METRIC_CFG[desc.metric]
+    params = "".join(f" {n} = {num(desc.metric_params[i])};" for i, n in enumerate(pnames))
+    diag = []
+    for flag, dname, extra in ((A.DIAGFLAG_FOURCOLORSCREEN, "FourColorScreen", ""),
+                               (A.DIAGFLAG_GEODESICPOSITION, "GeodesicPosition",
+                                f" {upd(desc.gp_update)} OutputSteps = {cnt(desc.gp_output_steps)};"),
+                               (A.DIAGFLAG_EQUATORIALPASSES, "EquatorialPasses",
+                                f" {upd(desc.ep_update)} Threshold = {num(desc.ep_threshold)};"),
+                               (A.DIAGFLAG_CLOSESTRADIUS, "ClosestRadius", f" {upd(desc.cr_update)}")):
+        if desc.diag_flags & flag:
+            use = " UseForMesh = true;" if desc.value_diag == flag else ""
+            diag.append(f"  {dname} = {{ On = true;{use}{extra} }};")
+    term = []
+    tf = desc.term_flags
+    if tf & A.TERMFLAG_HORIZON:
+        term.append(f"  Horizon = {{ On = true; Epsilon_Horizon = {num(desc.horizon_epsilon)}; "
+                    f"UpdateFrequency = {cnt(desc.horizon_update_n)}; }};")
+    if tf & A.TERMFLAG_BOUNDARYSPHERE:
+        term.append(f"  BoundarySphere = {{ On = true; SphereRadius = {num(desc.sphere_radius)}; "
+                    f"UpdateFrequency = {cnt(desc.sphere_update_n)}; }};")
+    if tf & A.TERMFLAG_TIMEOUT:
+        term.append(f"  TimeOut = {{ On = true; MaxSteps = {cnt(desc.timeout_max_steps)}; "
+                    f"UpdateFrequency = {cnt(desc.timeout_update_n)}; }};")
+    if tf & A.TERMFLAG_THETASINGULARITY:
+        term.append(f"  ThetaSingularity = {{ On = true; Epsilon = {num(desc.theta_epsilon)}; "
+                    f"UpdateFrequency = {cnt(desc.theta_update_n)}; }};")
+    mtype = _MESH_CFG[mesh.type]
+    if mesh.type in (MESH_SIMPLESQUARE, MESH_INPUTCERTAINPIXELS):
+        mopts = f"TotalPixels = {cnt(mesh.total_pixels)};"
+    else:
+        mopts = (f"InitialPixels = {cnt(mesh.initial_pixels)}; MaxPixels = {cnt(mesh.max_pixels)}; "
+                 f"IterationPixels = {cnt(mesh.iteration_pixels)}; MaxSubdivide = {int(mesh.max_subdivide)}; "
+                 f"InitialSubdivisionToFinal = {'true' if mesh.initial_subdivision_to_final else 'false'};")
+    cam = desc.camera_position
+    integ = "Verlet" if desc.integrator == A.INTEGRATOR_VERLET else "RK4"
+    chris = "Analytic" if desc.christoffel == A.CHRISTOFFEL_ANALYTIC else "FD"
+    prec = "Mixed" if desc.precision == A.PRECISION_MIXED else "FP64"
+    out = (f'FilePrefix = "{prefix}"; FileExtension = "{extension}"; TimeStamp = false; FirstLineInfo = true;'
+           if prefix else "")
+    return f"""Metric = {{ Name = "{name}";{params} RLogScale = {'true' if desc.rlog_scale else 'false'}; }};
+Source = {{ Name = "NoSource"; }};
+Diagnostics = {{
+{chr(10).join(diag)}
+}};
+Terminations = {{
+{chr(10).join(term)}
+}};
+ViewScreen = {{
+  Position = {{ t = {num(cam[0])}; r = {num(cam[1])}; theta = {num(cam[2])}; phi = {num(cam[3])}; }};
+  ScreenSize = {{ x = {num(desc.screen_size[0])}; y = {num(desc.screen_size[1])}; }};
+  ScreenCenter = {{ x = {num(desc.screen_center[0])}; y = {num(desc.screen_center[1])}; }};
+  Mesh = {{ Type = "{mtype}"; {mopts} }};
+}};
+Integrator = {{ Type = "{integ}"; StepSize = {num(desc.step_size)}; DerivativeH = {num(desc.derivative_h)};
+  SmallestPossibleStepsize = {num(desc.smallest_step)}; VerletVelocityTolerance = {num(desc.verlet_tolerance)}; }};
+GPU = {{ Christoffel = "{chris}"; Precision = "{prec}"; StepCap = {cnt(desc.step_cap)};
+  MixedSwitchRadius = {num(desc.mixed_switch_radius)}; MixedPoleMargin = {num(desc.mixed_pole_margin)}; }};
+Output = {{ ScreenOutputLevel = 1; {out} }};
+"""
+
+
 def run(cfg_path, output_prefix=None, christoffel=-1, precision=-1, schedule=-1, screen_output_level=-1,
         tracer=None):
     """Run the program (foort_program_run) on a configuration file; returns the GPU totals
@@ -320,8 +406,11 @@ class Server:
         self._h = C.c_void_p()
         _check(lib().foort_server_create(C.byref(cfg), C.byref(self._h)))
 
-    def submit(self, desc, mesh, tag):
-        req = Request(desc=desc, mesh=mesh, tag=tag)
+    def submit(self, desc, mesh, tag, output_prefix=None, output_extension=None):
+        """Queue a request; with output_prefix it writes the reference's output files."""
+        req = Request(desc=desc, mesh=mesh, tag=tag,
+                      output_prefix=output_prefix.encode() if output_prefix else None,
+                      output_extension=output_extension.encode() if output_extension else None)
         _check(lib().foort_server_submit(self._h, C.byref(req)))
 
     def results(self, cap=4096, timeout=0.0):

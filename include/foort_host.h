@@ -155,6 +155,14 @@ typedef struct foort_request {
     foort_run_desc desc;
     foort_mesh_desc mesh;         /* pixel lists (InputCertainPixelsMesh) are not accepted */
     uint64_t tag;                 /* the caller's identifier, returned with the result */
+    /* NULL: no files.  Else the request writes the reference's output files, as the program
+     * does for a configuration with Output.FilePrefix = output_prefix, FileExtension =
+     * output_extension (NULL: none), TimeStamp = false and FirstLineInfo = true: one file per
+     * diagnostic, "<prefix>_<name>[.<extension>]", the run description on its first line and
+     * every mesh iteration's geodesics (FOORT/InputOutput.cpp:129-332).  Both strings are
+     * copied at submission. */
+    const char *output_prefix;
+    const char *output_extension;
 } foort_request;
 
 typedef struct foort_request_result {

@@ -116,13 +116,14 @@ void setup(Run &R, const foort_run_desc *d, std::unique_ptr<Mesh> mesh)
         std::move(mesh), R.metric.get()));
 }
 
-// The reference's own meshes, by FOORT_MESH_* code (InputCertainPixelsMesh, which
-// reads the console, is left out).
+// The reference's own meshes, by FOORT_MESH_* code (InputCertainPixelsMesh runs its
+// console dialogue on this process's standard input and output).
 std::unique_ptr<Mesh> make_ref_mesh(int type, uint64_t pixels, uint64_t max_pixels, uint64_t iteration, int max_sub,
                                     int to_final, DiagBitflag val)
 {
     switch (type) {
     case 0: return std::unique_ptr<Mesh>(new SimpleSquareMesh(pixels, val));
+    case 1: return std::unique_ptr<Mesh>(new InputCertainPixelsMesh(pixels, val));
     case 2:
         return std::unique_ptr<Mesh>(new SquareSubdivisionMesh(max_pixels, pixels, max_sub, iteration, to_final != 0, val));
     case 3:

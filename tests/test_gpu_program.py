@@ -12,6 +12,7 @@
 3. The persistent schedule gives the same files as the one-shot schedule.
 """
 import os
+import re
 import subprocess
 
 import numpy as np
@@ -204,3 +205,64 @@ def test_cli_prints_the_initial_velocity_warnings(tmp_path):
     out = "WARNING: Initial velocity of geodesic is outward-pointing (should be inward-pointing)!"
     assert lines.count(fut) == int((v[:, 0] > 0).sum()) > 0
     assert lines.count(out) == int((v[:, 1] > 0).sum()) > 0
+
+
+def test_cli_input_certain_pixels_from_the_console(tmp_path):
+    """`foort <cfg>` with InputCertainPixelsMesh reads its pixels from the console
+    (FOORT/Mesh.cpp:81-141): the GPU program prints the dialogue and writes the same files as
+    the program with the oracle as its batch tracer (tests/test_host.py pins that one to the
+    reference's own mesh class)."""
+    import sys
+    from tests.test_host import INPUT_PIXELS_CFG, INPUT_PIXELS_STDIN, _RUN_ORACLE_PROGRAM, _dialogue
+    print("[program] InputCertainPixelsMesh through the console", flush=True)
+    repo = os.path.dirname(HERE)
+    for side in ("gpu", "cpu"):
+        (tmp_path / side).mkdir()
+        cfg = tmp_path / f"{side}.cfg"
+        cfg.write_text(INPUT_PIXELS_CFG.format(prefix=str(tmp_path / side / "run")))
+        cmd = ([P.CLI_PATH, str(cfg)] if side == "gpu" else
+               [sys.executable, "-c", _RUN_ORACLE_PROGRAM.format(repo=repo, cfg=str(cfg))])
+        r = subprocess.run(cmd, input=INPUT_PIXELS_STDIN, capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr
+        if side == "gpu":
+            d = _dialogue(r.stdout)
+            assert sum("added." in ln for ln in d) == 2 and sum("Invalid coordinates" in ln for ln in d) == 1
+    gpu, cpu = _tree(tmp_path / "gpu"), _tree(tmp_path / "cpu")
+    assert sorted(gpu) == ["run_ClosestRadius.dat", "run_EquatPasses.dat", "run_FourColorScreen.dat"]
+    for k in gpu:
+        assert gpu[k] == cpu[k], k
+
+
+PROGRESS = """
+Metric = {{ Name = "Kerr"; a = 0.9; RLogScale = true; }};
+Diagnostics = {{ FourColorScreen = {{ On = true; UseForMesh = true; }}; EquatorialPasses = {{ On = true; }}; }};
+Terminations = {{ Horizon = {{ On = true; }}; BoundarySphere = {{ On = true; }}; TimeOut = {{ On = true; MaxSteps = 20000; }}; }};
+ViewScreen = {{ Position = {{ r = 1000.0; theta = 1.3; }}; ScreenSize = {{ x = 24.0; y = 24.0; }};
+               Mesh = {{ Type = "{mesh}"; {pixels} }}; }};
+Integrator = {{ Type = "RK4"; }};
+Output = {{ FilePrefix = "unused"; FileExtension = "dat"; TimeStamp = false; ScreenOutputLevel = 2; {freq} }};
+"""
+
+
+@pytest.mark.parametrize("mesh,pixels", [("SimpleSquareMesh", "TotalPixels = 262144;"),
+                                         ("SquareSubdivisionMeshV2", "InitialPixels = 160000; MaxPixels = 400000; "
+                                                                     "IterationPixels = 100000; MaxSubdivide = 3;")])
+def test_loop_message_frequency_prints_progress_and_keeps_results(mesh, pixels, tmp_path):
+    """Output.LoopMessageFrequency: the program prints the reference's progress line
+    (FOORT/Main.cpp:354-371) between chunks of about that many geodesics, and the files are
+    those of the run without it."""
+    print(f"[program] LoopMessageFrequency, {mesh}", flush=True)
+    outs = {}
+    for name, freq in (("plain", ""), ("progress", "LoopMessageFrequency = 70000;")):
+        cfg = tmp_path / f"{name}.cfg"
+        cfg.write_text(PROGRESS.format(mesh=mesh, pixels=pixels, freq=freq))
+        r = subprocess.run([P.CLI_PATH, str(cfg), "--output-prefix", str(tmp_path / name / "run")],
+                           capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr
+        outs[name] = r.stdout
+    lines = [ln for ln in outs["progress"].splitlines() if ln.startswith("Approx. at geodesic ")]
+    print("\n".join(lines[:4]), flush=True)
+    assert len(lines) >= 2 and not any(ln.startswith("Approx.") for ln in outs["plain"].splitlines())
+    assert all(re.match(r"Approx\. at geodesic \d+ \(\d+\.\d+s elapsed; speed: \d+ geod/s; "
+                        r"est\. loop time remaining: \d+\.\d+s\)\.\.\.$", ln) for ln in lines)
+    assert _tree(tmp_path / "plain") == _tree(tmp_path / "progress")

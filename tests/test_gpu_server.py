@@ -1,0 +1,136 @@
+"""The render server against the CPU oracle, on requests of the C4 stream (seed 2024).
+
+A subset of the stream -- every metric kind, adaptive meshes of 2 to 6 iterations, final
+sides up to 100 (what the oracle finishes in seconds) -- is submitted to the server at once,
+each request writing the reference's output files (foort_request.output_prefix).  The same
+request, written as a configuration file (program.config_text), then runs through the
+drop-in program with the CPU oracle as its batch tracer (the reference's per-pixel loop,
+FOORT/Main.cpp:313-419, mesh refinement FOORT/Mesh.cpp:1092-1187, output files
+FOORT/InputOutput.cpp:129-332).
+
+* finite differences, binary64: the server's files equal the oracle program's byte for byte
+  (first line, every iteration's pixels in order, every value);
+* mixed precision (the stream as the server benchmark runs it): on the pixels both runs
+  integrated, the integer diagnostics agree but on a few pixels of the diagnostic
+  boundaries, and the closest radii agree to the accuracy of the binary32 far field.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from foort_amd import abi as A
+from foort_amd import program as P
+from foort_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+# stream indices: Kerr, Johannsen, Rasheed-Larsen and Manko-Novikov requests of final side <= 100
+SUBSET = [7, 17, 30, 40, 51, 56, 66, 68, 83, 88, 109, 112, 23, 41, 198, 264, 285, 407, 95, 137, 331, 435, 52]
+NAMES = ("FourColorScreen", "EquatPasses", "ClosestRadius")
+
+
+def _requests():
+    reqs = W.c4_requests(seed=2024, n=max(SUBSET) + 1)
+    return [(k, reqs[k]) for k in SUBSET]
+
+
+def _oracle_tracer(desc, side, si):
+    return O.trace("fm", desc, side, screen_index=si)
+
+
+def _oracle_files(tmp, k, desc, mesh):
+    cfg = os.path.join(tmp, f"oracle_{k}.cfg")
+    with open(cfg, "w") as f:
+        f.write(P.config_text(desc, mesh, prefix=os.path.join(tmp, f"oracle_{k}")))
+    P.run(cfg, screen_output_level=0, tracer=_oracle_tracer)
+    return [os.path.join(tmp, f"oracle_{k}_{n}.dat") for n in NAMES]
+
+
+def _serve(tmp, reqs, tag):
+    with P.Server(workers=4, hbm_budget_bytes=64 << 20, launch_us=500) as srv:
+        for k, (_, desc, mesh, _) in reqs:
+            srv.submit(desc, mesh, k, output_prefix=os.path.join(tmp, f"{tag}_{k}"), output_extension="dat")
+        srv.drain()
+        res = {r.tag: r for r in srv.results(cap=len(reqs) + 4)}
+    assert sorted(res) == sorted(k for k, _ in reqs)
+    assert all(r.status == 0 for r in res.values())
+    return res
+
+
+def _values(path):
+    """(row, column) -> value text of one output file (its first line is the run description)."""
+    with open(path) as f:
+        lines = f.read().splitlines()
+    out = {}
+    for line in lines[1:]:
+        row, col, val = line.split()
+        out[(int(row), int(col))] = val
+    return lines[0], out
+
+
+def test_server_fd_requests_equal_oracle_files(tmp_path):
+    tmp = str(tmp_path)
+    reqs = []
+    for k, (t, desc, mesh, info) in _requests():
+        desc.christoffel, desc.precision = A.CHRISTOFFEL_FD, A.PRECISION_FP64
+        reqs.append((k, (t, desc, mesh, info)))
+    kinds = {info["kind"] for _, (_, _, _, info) in reqs}
+    assert kinds == {"kerr", "johannsen", "rasheed_larsen", "manko_novikov"}
+    print(f"[server-oracle] {len(reqs)} finite-difference requests", flush=True)
+    res = _serve(tmp, reqs, "server")
+    for k, (_, desc, mesh, info) in reqs:
+        want = _oracle_files(tmp, k, desc, mesh)
+        for name, w in zip(NAMES, want):
+            got = os.path.join(tmp, f"server_{k}_{name}.dat")
+            with open(got, "rb") as a, open(w, "rb") as b:
+                assert a.read() == b.read(), (k, info, name)
+        lines = sum(1 for _ in open(want[0])) - 1
+        assert res[k].pixels == lines, (k, info)
+    print("[server-oracle] every file byte for byte", flush=True)
+
+
+def test_server_mixed_requests_close_to_oracle(tmp_path):
+    """An adaptive mesh refines where neighbouring values differ, so after the uniform initial
+    grid nearly every pixel lies on a diagnostic boundary, where the formulations may differ:
+    the initial grid of each request is held to the bound of the analytic formulations on
+    uniform grids (tests/test_gpu_analytic.py), all common pixels together to 2%."""
+    tmp = str(tmp_path)
+    reqs = _requests()
+    print(f"[server-oracle] {len(reqs)} mixed-precision requests", flush=True)
+    _serve(tmp, reqs, "server")
+    total = mismatched = 0
+    for k, (_, desc, mesh, info) in reqs:
+        fd = desc.__class__.from_buffer_copy(desc)
+        fd.christoffel, fd.precision = A.CHRISTOFFEL_FD, A.PRECISION_FP64
+        want = _oracle_files(tmp, k, fd, mesh)
+        got = [_values(os.path.join(tmp, f"server_{k}_{n}.dat")) for n in NAMES]
+        ref = [_values(w) for w in want]
+        # the first initial_pixels lines of a file are the initial grid (files keep iteration order)
+        initial = list(ref[0][1])[:mesh.initial_pixels]
+        common = sorted(set(got[0][1]) & set(ref[0][1]))
+        assert set(initial) <= set(got[0][1]), (k, info)
+        # the adaptive meshes follow their own values: both runs share most later pixels
+        assert len(common) >= 0.9 * len(ref[0][1]), (k, info)
+
+        def mismatch(px):
+            bad = np.zeros(len(px), bool)
+            for g, r in zip(got[:2], ref[:2]):
+                bad |= np.array([int(g[1][p]) != int(r[1][p]) for p in px])
+            return bad
+
+        mism, mism0 = mismatch(common), mismatch(initial)
+        rg = np.array([float(got[2][1][p]) for p in common])
+        rr = np.array([float(ref[2][1][p]) for p in common])
+        rel = np.abs(rg - rr)[~mism] / np.maximum(np.abs(rr[~mism]), 1.0)
+        print(f"[server-oracle] request {k} ({info['kind']}, side {info['side']}, {info['iterations']} iterations): "
+              f"initial grid {int(mism0.sum())} of {len(initial)} differ; all {len(common)} common pixels: "
+              f"integer mismatch {mism.mean():.4f}, closest radius median rel {np.median(rel):.1e}", flush=True)
+        assert mism0.sum() <= max(1, 0.05 * len(initial)), (k, info)
+        # ClosestRadius is written with 6 decimals (std::to_string), about 1e-6 of r = 1..1000
+        assert np.median(rel) <= 1e-4, (k, info)
+        total += len(common)
+        mismatched += int(mism.sum())
+    print(f"[server-oracle] all requests: {mismatched} of {total} common pixels differ", flush=True)
+    assert mismatched <= 0.02 * total

@@ -21,6 +21,7 @@ import ctypes as C
 import os
 import re
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -305,6 +306,81 @@ def test_input_certain_pixels_mesh():
     m.finished(np.zeros(3))
     m.end_loop()
     assert m.is_finished and m.description() == "Mesh: User-input pixels"
+
+
+INPUT_PIXELS_CFG = """
+Metric = {{ Name = "Kerr"; a = 0.6; RLogScale = true; }};
+Diagnostics = {{ FourColorScreen = {{ On = true; UseForMesh = true; }}; EquatorialPasses = {{ On = true; }};
+                ClosestRadius = {{ On = true; }}; }};
+Terminations = {{ Horizon = {{ On = true; }}; BoundarySphere = {{ On = true; SphereRadius = 1000.0; }};
+                 TimeOut = {{ On = true; MaxSteps = 20000; }}; }};
+ViewScreen = {{ Position = {{ r = 1000.0; theta = 1.2; }}; ScreenSize = {{ x = 20.0; y = 20.0; }};
+               Mesh = {{ Type = "InputCertainPixelsMesh"; TotalPixels = 100; }}; }};
+Integrator = {{ Type = "RK4"; }};
+Output = {{ FilePrefix = "{prefix}"; FileExtension = "dat"; TimeStamp = false; ScreenOutputLevel = 1; }};
+"""
+# pixels (3, 4) and (0, 0), then (99, 1) outside the 10 x 10 screen, then a negative number to stop
+INPUT_PIXELS_STDIN = "3 4\n0 0\n99 1\n-1\n"
+
+_RUN_ORACLE_PROGRAM = """
+import sys
+sys.path.insert(0, {repo!r})
+import oracle as O
+from foort_amd import program as P
+P.run({cfg!r}, tracer=lambda d, s, si: O.trace("fm", d, s, screen_index=si))
+"""
+_RUN_REFERENCE_PROGRAM = """
+import sys
+sys.path.insert(0, {repo!r})
+import oracle as O
+from foort_amd import program as P
+c = P.Config(path={cfg!r})
+O.ref_program_run(c.run_desc(), P.MESH_INPUTCERTAINPIXELS, c.mesh_desc().total_pixels, {prefix!r}, ext="dat")
+"""
+
+
+def _dialogue(text):
+    return [ln for ln in text.splitlines() if "InputCertainPixelsMesh message" in ln or "No pixels added" in ln]
+
+
+@need_ref
+def test_input_certain_pixels_dialogue_matches_reference(tmp_path):
+    """InputCertainPixelsMesh's console dialogue (FOORT/Mesh.cpp:81-141) and the files of the
+    run: the program (oracle as its batch tracer) against the reference's own mesh class and
+    program loop (oracle/_ref), the same console input piped into both."""
+    repo = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    cfg = tmp_path / "pixels.cfg"
+    cfg.write_text(INPUT_PIXELS_CFG.format(prefix=str(tmp_path / "new" / "run")))
+    (tmp_path / "new").mkdir()
+    (tmp_path / "ref").mkdir()
+    new = subprocess.run([sys.executable, "-c", _RUN_ORACLE_PROGRAM.format(repo=repo, cfg=str(cfg))],
+                         input=INPUT_PIXELS_STDIN, capture_output=True, text=True, timeout=120)
+    assert new.returncode == 0, new.stderr
+    ref = subprocess.run([sys.executable, "-c", _RUN_REFERENCE_PROGRAM.format(repo=repo, cfg=str(cfg),
+                                                                                prefix=str(tmp_path / "ref" / "run"))],
+                         input=INPUT_PIXELS_STDIN, capture_output=True, text=True, timeout=120)
+    assert ref.returncode == 0, ref.stderr
+    d_new, d_ref = _dialogue(new.stdout), _dialogue(ref.stdout)
+    assert d_new == d_ref
+    assert sum("added." in ln for ln in d_new) == 2 and sum("Invalid coordinates" in ln for ln in d_new) == 1
+    assert _tree(tmp_path / "new") == _tree(tmp_path / "ref")
+    lines = (tmp_path / "new" / "run_FourColorScreen.dat").read_text().splitlines()
+    assert [ln.split()[:2] for ln in lines[1:]] == [["3", "4"], ["0", "0"]]
+
+
+def test_input_certain_pixels_dialogue_ends_at_end_of_input(tmp_path):
+    """End of input ends the dialogue (the reference, reading a failed stream as 0, would add
+    (0, 0) for ever); with no pixel the run integrates nothing and says so."""
+    repo = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    cfg = tmp_path / "pixels.cfg"
+    cfg.write_text(INPUT_PIXELS_CFG.format(prefix=str(tmp_path / "run")))
+    for stdin, added in (("5 5\n", 1), ("", 0)):
+        r = subprocess.run([sys.executable, "-c", _RUN_ORACLE_PROGRAM.format(repo=repo, cfg=str(cfg))],
+                           input=stdin, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        d = _dialogue(r.stdout)
+        assert sum("added." in ln for ln in d) == added
+        assert ("No pixels added to integration list!" in r.stdout) == (added == 0)
 
 
 # ---------------------------------------------------------------------------
