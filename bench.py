@@ -15,7 +15,10 @@ are bit-identical to the CPU oracle; the roofline of its kernel; the other
 formulations of the same image (persistent schedule, analytic Christoffels in
 FP64 and mixed precision) with their agreement with it; and the reference's
 own CPU build (oracle/_ref, its -Ofast flags) timed on a sample of the same
-image on this host's cores.
+image on this host's cores.  The same line carries the metric's latency half:
+the render server under the C4 request stream (2000 requests from seed 2024,
+Poisson arrivals at 40/s, mixed precision), its p50/p99 latency and whether the
+stream is sustained (--no-server skips it).
 
 python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -64,6 +67,19 @@ def cpu_model():
     return platform.processor()
 
 
+def host_threads():
+    """Threads for the CPU baseline: the process's CPU share -- OMP_NUM_THREADS where the
+    environment sets it (a GPU box gives each GPU's processes a share of its cores and says
+    so there), else every core this process may run on."""
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    share = int(env) if env.isdigit() and int(env) > 0 else affinity
+    return min(share, affinity), affinity
+
+
 def cpu_baseline(desc, side, stride):
     """The reference's own path (oracle/_ref, built from its sources with its
     -Ofast flags) on every `stride`-th row and column of the image."""
@@ -71,7 +87,9 @@ def cpu_baseline(desc, side, stride):
     import oracle as O
     rows = np.arange(0, side, stride, dtype=np.uint64)
     px = np.stack(np.meshgrid(rows, rows, indexing="ij"), -1).reshape(-1, 2)
-    threads = min(16, os.cpu_count() or 1)
+    threads, affinity = host_threads()
+    log(f"cpu baseline on {threads} threads (cores available to this process: {affinity}, "
+        f"os.cpu_count: {os.cpu_count()}, {cpu_model()})")
     kind = "reference" if O.have("ref_ofast") else "port"
     if kind == "reference":
         steps, sec = O.time_reference("ref_ofast", desc, side, px, threads)
@@ -85,6 +103,8 @@ def cpu_baseline(desc, side, stride):
         "sample": f"{len(px)} pixels (every {stride}th row and column of the 1024x1024 image), "
                   f"{steps} geodesic steps in {sec:.1f} s on {threads} threads of {cpu_model()}",
         "geodesic_steps_per_s": steps / sec,
+        "host": {"cpu_model": cpu_model(), "cores_available": affinity, "os_cpu_count": os.cpu_count(),
+                 "omp_num_threads": os.environ.get("OMP_NUM_THREADS")},
     }
 
 
@@ -252,9 +272,9 @@ def run_c3(args):
     print(json.dumps(line), flush=True)
 
 
-def run_c4(args):
-    """C4: the render server under a seeded open-loop request stream (Poisson arrivals):
-    latency (submission to completion) percentiles and throughput."""
+def c4_measure(args):
+    """The render server under a seeded open-loop request stream (Poisson arrivals): latency
+    (submission to completion) percentiles, throughput, and whether the stream is sustained."""
     import numpy as np
     from foort_amd import program as P
     from foort_amd import workloads as W
@@ -268,6 +288,8 @@ def run_c4(args):
             if wait > 0:
                 time.sleep(wait)
             srv.submit(desc, mesh, k)
+            if k % 250 == 249:
+                log(f"c4: {k + 1} of {n} requests submitted")
         srv.drain()
         res = srv.results(cap=n + 16)
     assert len(res) == n and all(r.status == 0 for r in res), [r.status for r in res if r.status]
@@ -277,23 +299,19 @@ def run_c4(args):
     span = max(r.end_s for r in res) - sub.min()
     pixels = sum(r.pixels for r in res)
     kernel_s = sum(r.kernel_ms for r in res) / 1e3
-    line = {"metric": "render-server request latency p99", "value": float(np.quantile(lat, 0.99)), "unit": "s",
-            "n_gpus": 1, "steps": n, "warmup": 0, "ms_per_step": float(np.quantile(lat, 0.5)) * 1e3,
-            "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
-            "data": "synthetic",
-            "config": {"workload": W.C4_TEXT.format(seed=args.seed, rate=args.rate), "workers": args.workers,
-                       "ring_bytes_per_group": args.hbm_budget << 20, "launch_us": args.launch_us or 2000},
-            "latency_p50_s": float(np.quantile(lat, 0.5)), "latency_p99_s": float(np.quantile(lat, 0.99)),
-            "latency_max_s": float(lat.max()), "queue_wait_p99_s": float(np.quantile(wait, 0.99)),
-            "offered_rate_per_s": args.rate, "completed_rate_per_s": n / span,
-            "pixels_per_s": pixels / span, "pixels": int(pixels), "geodesic_steps": int(sum(r.steps for r in res)),
-            "gpu_kernel_s_shared_out": kernel_s, "span_s": span}
+    out = {"workload": W.C4_TEXT.format(seed=args.seed, rate=args.rate), "requests": n, "workers": args.workers,
+           "ring_bytes_per_group": args.hbm_budget << 20, "launch_us": args.launch_us or 2000,
+           "latency_p50_s": float(np.quantile(lat, 0.5)), "latency_p99_s": float(np.quantile(lat, 0.99)),
+           "latency_max_s": float(lat.max()), "queue_wait_p99_s": float(np.quantile(wait, 0.99)),
+           "offered_rate_per_s": args.rate, "completed_rate_per_s": n / span,
+           "pixels_per_s": pixels / span, "pixels": int(pixels), "geodesic_steps": int(sum(r.steps for r in res)),
+           "gpu_kernel_s_shared_out": kernel_s, "span_s": span}
     # whether the stream is sustained: latency does not grow along the stream, and the requests
     # in flight when the last one arrives drain in about one request's latency
     end = np.array([r.end_s for r in res])
     order = np.argsort(sub)
     quarters = np.array_split(order, 4)
-    line["sustained"] = {
+    out["sustained"] = {
         "arrival_span_s": float(sub.max() - sub.min()),
         "latency_p50_by_arrival_quarter_s": [float(np.quantile(lat[q], 0.5)) for q in quarters],
         "latency_p99_by_arrival_quarter_s": [float(np.quantile(lat[q], 0.99)) for q in quarters],
@@ -303,11 +321,24 @@ def run_c4(args):
     by_kind = {}
     for r in res:
         by_kind.setdefault(reqs[r.tag][3]["kind"], []).append(r)
-    line["by_kind"] = {
+    out["by_kind"] = {
         k: {"requests": len(v), "latency_p50_s": float(np.quantile([r.end_s - r.submit_s for r in v], 0.5)),
             "latency_p99_s": float(np.quantile([r.end_s - r.submit_s for r in v], 0.99)),
             "steps_per_pixel": sum(r.steps for r in v) / max(1, sum(r.pixels for r in v))}
         for k, v in sorted(by_kind.items())}
+    log(f"c4: p50 {out['latency_p50_s']:.3f} s, p99 {out['latency_p99_s']:.3f} s, "
+        f"{out['completed_rate_per_s']:.1f} requests/s completed of {args.rate:g}/s offered")
+    return out
+
+
+def run_c4(args):
+    """C4 alone: the render server's line (value = latency p99)."""
+    m = c4_measure(args)
+    line = {"metric": "render-server request latency p99", "value": m["latency_p99_s"], "unit": "s",
+            "n_gpus": 1, "steps": args.requests, "warmup": 0, "ms_per_step": m["latency_p50_s"] * 1e3,
+            "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+            "data": "synthetic", "config": {"workload": m["workload"]}}
+    line.update({k: v for k, v in m.items() if k != "workload"})
     print(json.dumps(line), flush=True)
 
 
@@ -328,6 +359,7 @@ def main():
     ap.add_argument("--cpu-stride", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--no-server", action="store_true", help="c1: skip the render-server (C4) leg")
     ap.add_argument("--tail-geodesics", type=int, default=None,
                     help="c2: the GPU group's TailGeodesics (unfinished geodesics handed to the "
                          "four-lanes-per-geodesic kernel; 4294967295 = never)")
@@ -457,6 +489,18 @@ def main():
                               "vs_fd_oneshot": agreement(base, out)}
             log(f"{name}: {n / dt:.3e} px/s")
         line["variants"] = variants
+
+    if not args.no_variants:
+        # the north-star ratio: the mixed-precision persistent path over the plain FP64 path
+        line["mixed_persistent_vs_fd_fp64"] = line["variants"]["mixed_persistent"]["pixels_per_s"] / value
+
+    if not args.no_server:
+        # the headline metric's latency half: the render server under the C4 request stream
+        log("render server (C4 stream) ...")
+        srv = c4_measure(args)
+        line["latency_p50_s"] = srv["latency_p50_s"]
+        line["latency_p99_s"] = srv["latency_p99_s"]
+        line["render_server"] = srv
 
     if not args.no_cpu:
         log("cpu baseline ...")
