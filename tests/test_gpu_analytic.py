@@ -63,17 +63,23 @@ def test_analytic_close_to_fd_oracle(name, precision):
     assert np.median(rel[same]) < (1e-6 if precision == A.PRECISION_FP64 else 1e-4)
 
 
-def test_analytic_persistent_equals_oneshot():
-    mk, side = CASES["kerr099_rlog"]
+@pytest.mark.parametrize("precision", [A.PRECISION_FP64, A.PRECISION_MIXED])
+@pytest.mark.parametrize("name", ["kerr099_rlog", "test_cfg", "manko_novikov"])
+def test_analytic_persistent_equals_oneshot(name, precision):
+    """Bit for bit between the schedules.  The one-shot analytic kernels leave their step loop
+    lane by lane, the persistent ones park lanes between launches: a lane stepping past its
+    termination (the code-generation issue that term_after_run guards against) would differ."""
+    mk, side = CASES[name]
     F = _gpu()
-    with F.Context(mk(christoffel=A.CHRISTOFFEL_ANALYTIC)) as ctx:
+    kw = dict(christoffel=A.CHRISTOFFEL_ANALYTIC, precision=precision)
+    with F.Context(mk(**kw)) as ctx:
         a = ctx.trace_grid(side)
-    with F.Context(mk(christoffel=A.CHRISTOFFEL_ANALYTIC, schedule=A.SCHEDULE_PERSISTENT, lanes_per_cu=1,
-                      steps_per_launch=50)) as ctx:
+    with F.Context(mk(schedule=A.SCHEDULE_PERSISTENT, lanes_per_cu=1, steps_per_launch=50, **kw)) as ctx:
         b = ctx.trace_grid(side)
     for k in ("term", "quadrant", "equat_passes", "steps"):
         assert np.array_equal(a[k], b[k]), k
-    assert np.array_equal(a["final_pos"].view(np.uint64), b["final_pos"].view(np.uint64))
+    for k in ("final_pos", "final_vel", "lambda", "closest_radius"):
+        assert np.array_equal(a[k].view(np.uint64), b[k].view(np.uint64)), k
 
 
 @pytest.mark.parametrize("path", ["fd", "analytic", "mixed"])
