@@ -35,10 +35,16 @@ if REPO not in sys.path:
 
 METRIC = "pixels/s & geodesic steps/s, Kerr a=0.99 1024² (FP64 vs mixed); p50/p99 latency"
 PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector peak
-# HBM bytes per pixel of the finite-difference one-shot kernel: FETCH_SIZE (x2.00) + WRITE_SIZE (x1.00),
-# each scaled by what the counter reads for a plain copy of known size; 688.8 + 6176.0 KiB per
-# 65,536-pixel dispatch (profiles/trace_oneshot_fd_kerr_rk4_c1.md)
-FD_ONESHOT_HBM_BYTES_PER_PIXEL = (688.8 + 6176.0) * 1024 / 65536
+
+
+def hbm_bytes_per_pixel(christoffel, precision, schedule):
+    """HBM bytes per pixel of a path's kernels, measured with rocprofv3 FETCH_SIZE and WRITE_SIZE
+    (each calibrated on a plain device copy) on a 256x256 image: foort_amd/hbm_traffic.json,
+    written by foort_amd/counters.py; None for a path not measured."""
+    from foort_amd import counters
+    t = counters.load()
+    e = t and t["paths"].get(f"{christoffel}/{precision}/{schedule}")
+    return e["bytes_per_pixel"] if e else None
 
 
 def log(msg):
@@ -454,8 +460,10 @@ def main():
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64_TFLOPS,
-            "traffic": (FD_ONESHOT_HBM_BYTES_PER_PIXEL * n
-                        if (args.christoffel, args.precision, args.schedule) == ("fd", "fp64", "oneshot") else None),
+            "traffic": (lambda b: b * n if b is not None else None)(
+                hbm_bytes_per_pixel(args.christoffel, args.precision, args.schedule)),
+            "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE per pixel of this path (calibrated on a device "
+                              "copy, 256x256 image) x the launch's pixels: profiles/hbm_traffic_kerr_rk4_c1_256.md",
             "kernel": kernel,
             "flops_per_step": closed["rk4_step_flops"],
             "flops_per_step_formulation": closed["formulation"],
@@ -484,8 +492,12 @@ def main():
                 F.synchronize()
                 dt = time.perf_counter() - t
             out = {k: v.numpy() for k, v in res.items()}
+            c = "analytic" if kw.get("christoffel") == A.CHRISTOFFEL_ANALYTIC else "fd"
+            p = "mixed" if kw.get("precision") == A.PRECISION_MIXED else "fp64"
+            s = "persistent" if kw.get("schedule") == A.SCHEDULE_PERSISTENT else "oneshot"
             variants[name] = {"pixels_per_s": n / dt, "geodesic_steps_per_s": st.steps / dt,
                               "kernel_ms": st.device_ms, "launches": st.launches,
+                              "hbm_bytes_per_pixel": hbm_bytes_per_pixel(c, p, s),
                               "vs_fd_oneshot": agreement(base, out)}
             log(f"{name}: {n / dt:.3e} px/s")
         line["variants"] = variants
