@@ -66,7 +66,7 @@ class Positions(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("geodesics", C.c_uint64), ("steps", C.c_uint64), ("launches", C.c_uint64),
                 ("device_ms", C.c_double), ("future_pointing", C.c_uint64), ("outward_pointing", C.c_uint64),
-                ("nonfinite", C.c_uint64)]
+                ("nonfinite", C.c_uint64), ("graph_launches", C.c_uint64)]
 
 
 def default_desc():

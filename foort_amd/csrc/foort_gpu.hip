@@ -109,6 +109,7 @@ void set_warnings(foort_stats *stats, const unsigned long long *c)
     stats->future_pointing = c[foort::kCntFuture];
     stats->outward_pointing = c[foort::kCntOutward];
     stats->nonfinite = c[foort::kCntNonfinite];
+    stats->graph_launches = 0;
 }
 
 __global__ void __launch_bounds__(64) zero_counters(unsigned long long *p, int n)
@@ -300,13 +301,13 @@ struct foort_gpu_ctx {
     uint64_t last_launches = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // captured batches (use_graph): the context's own stream, pinned host staging with the
-    // layout of d_buf, and the executable graph of the last batch shape
+    // layout of d_buf, and the executable graph that every batch's capture updates
     hipStream_t stream = nullptr;
     void *h_buf = nullptr;
     size_t h_cap = 0;
     hipGraphExec_t graph = nullptr;
-    std::string graph_key;
-    std::string graph_shape, seen_shape; // batch shapes: the captured one, the last one run directly
+    uint64_t graph_updates = 0;   // batches whose capture updated the executable graph in place
+    bool last_graph = false;      // the last host-pointer batch ran as a graph launch
     // the device address of `scratch` on this context's device (NULL, with the reason in
     // scratch_err, if it could not be found)
     char *scratch = nullptr;
@@ -373,7 +374,6 @@ void drop_graph(foort_gpu_ctx *ctx)
     if (ctx->graph)
         (void)hipGraphExecDestroy(ctx->graph);
     ctx->graph = nullptr;
-    ctx->graph_key.clear();
 }
 
 // the persistent schedule's options, from the descriptor (the autotuner changes them)
@@ -626,8 +626,8 @@ int check_batch(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, uint64_t coun
 
 // Host-pointer entry points: stage through the context buffer, launch, copy back.  With
 // use_graph, a batch that is one launch goes through pinned host staging and runs as one HIP
-// graph (index upload, counter reset, kernel, result copies), replayed while the batch shape
-// (sizes, result fields, schedule, buffers) stays the same and captured again otherwise.
+// graph (index upload, counter reset, kernels, result copies): each batch is captured and
+// updates the context's executable graph in place, which is then launched.
 int trace_host(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, uint64_t count, const uint64_t *si,
                const foort_geodesic_results *out, foort_stats *stats)
 {
@@ -671,22 +671,7 @@ int trace_host(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, uint64_t count
     rc = ctx_work(ctx, w);
     if (rc)
         return rc;
-    bool graph = ctx->desc.use_graph && single_launch(ctx) && count > 0;
-    std::string shape;
-    if (graph) {
-        // a batch shape is captured when it comes a second time (the first batch of a shape
-        // runs directly): a run of batches that never repeat, such as the iterations of an
-        // adaptive mesh, pays for no capture
-        shape = std::to_string(side) + "/" + std::to_string(first) + "/" + std::to_string(count) + "/" +
-                (si ? "s" : "g") + "/" + std::to_string(ctx->desc.schedule) + "/" +
-                std::to_string(ctx->desc.block_size);
-        for (int k = 0; k < 8; ++k)
-            shape += f[k].host ? "1" : "0";
-        if (shape != ctx->graph_shape && shape != ctx->seen_shape) {
-            ctx->seen_shape = shape;
-            graph = false;
-        }
-    }
+    const bool graph = ctx->desc.use_graph && single_launch(ctx) && count > 0;
     if (!graph) {
         // everything on the context's own stream, so contexts on other host threads run concurrently
         if (si)
@@ -700,56 +685,60 @@ int trace_host(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, uint64_t count
             if (x.host)
                 HIP_TRY(ctx, hipMemcpyAsync(x.host, base + x.off, x.bytes, hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        ctx->last_graph = false;
     } else {
+        // The batch -- index upload, counter reset, kernels, warning counts, result copies -- is
+        // captured from the context's stream and run as one graph.  The executable graph of the
+        // previous batch is updated in place with this capture (hipGraphExecUpdate: the same
+        // nodes with new sizes, grids and arguments), so the batches of an adaptive mesh, whose
+        // sizes never repeat, each run as a launch of the same executable graph; it is built anew
+        // only when the nodes change (another schedule, other result fields).
         rc = ensure_pinned(ctx, off);
         if (rc)
             return rc;
-        if (!ctx->stream)
-            HIP_TRY(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
         char *hb = static_cast<char *>(ctx->h_buf);
-        unsigned mask = 0;
-        for (int k = 0; k < 8; ++k)
-            mask |= f[k].host ? 1u << k : 0u;
-        const std::string key = std::to_string(side) + "/" + std::to_string(first) + "/" + std::to_string(count) +
-                                "/" + (si ? "s" : "g") + "/" + std::to_string(mask) + "/" +
-                                std::to_string(ctx->desc.schedule) + "/" + std::to_string(ctx->desc.block_size) + "/" +
-                                std::to_string(reinterpret_cast<uintptr_t>(ctx->d_buf)) + "/" +
-                                std::to_string(reinterpret_cast<uintptr_t>(ctx->h_buf)) + "/" +
-                                std::to_string(ctx->slots.refill) + "/" + std::to_string(ctx->slots.interval);
-        if (key != ctx->graph_key) {
-            drop_graph(ctx);
-            HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-            auto capture = [&]() -> int {
-                if (si)
-                    HIP_TRY(ctx, hipMemcpyAsync(base + o_si, hb + o_si, count * 16, hipMemcpyHostToDevice, ctx->stream));
-                int r = launch(ctx, w, side, first, count, d_si, d, ctx->stream);
-                if (r)
-                    return r;
-                for (const Field &x : f)
-                    if (x.host)
-                        HIP_TRY(ctx, hipMemcpyAsync(hb + x.off, base + x.off, x.bytes, hipMemcpyDeviceToHost,
-                                                    ctx->stream));
-                return FOORT_OK;
-            };
-            rc = capture();
-            hipGraph_t g = nullptr;
-            hipError_t e = hipStreamEndCapture(ctx->stream, &g);
-            if (rc) {
-                if (g)
-                    (void)hipGraphDestroy(g);
-                return rc;
+        HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        auto capture = [&]() -> int {
+            if (si)
+                HIP_TRY(ctx, hipMemcpyAsync(base + o_si, hb + o_si, count * 16, hipMemcpyHostToDevice, ctx->stream));
+            int r = launch(ctx, w, side, first, count, d_si, d, ctx->stream);
+            if (r)
+                return r;
+            for (const Field &x : f)
+                if (x.host)
+                    HIP_TRY(ctx, hipMemcpyAsync(hb + x.off, base + x.off, x.bytes, hipMemcpyDeviceToHost, ctx->stream));
+            return FOORT_OK;
+        };
+        rc = capture();
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+        if (rc || e != hipSuccess) {
+            if (g)
+                (void)hipGraphDestroy(g);
+            return rc ? rc : hip_fail(ctx, e, "hipStreamEndCapture");
+        }
+        bool updated = false;
+        if (ctx->graph) {
+            hipGraphNode_t bad = nullptr;
+            hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+            e = hipGraphExecUpdate(ctx->graph, g, &bad, &res);
+            updated = e == hipSuccess && res == hipGraphExecUpdateSuccess;
+            if (!updated) {
+                (void)hipGetLastError(); // a refused update is not an error: the graph is built anew
+                drop_graph(ctx);
             }
-            if (e != hipSuccess)
-                return hip_fail(ctx, e, "hipStreamEndCapture");
+        }
+        if (!updated) {
             e = hipGraphInstantiate(&ctx->graph, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
             if (e != hipSuccess) {
                 ctx->graph = nullptr;
+                (void)hipGraphDestroy(g);
                 return hip_fail(ctx, e, "hipGraphInstantiate");
             }
-            ctx->graph_key = key;
-            ctx->graph_shape = shape;
+        } else {
+            ++ctx->graph_updates;
         }
+        (void)hipGraphDestroy(g);
         if (si)
             std::memcpy(hb + o_si, si, count * 16);
         HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
@@ -759,7 +748,7 @@ int trace_host(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, uint64_t count
         for (const Field &x : f)
             if (x.host)
                 std::memcpy(x.host, hb + x.off, x.bytes);
-        ctx->last_launches = 1;
+        ctx->last_graph = true;
     }
     if (stats) {
         unsigned long long c[foort::kNumCounters] = {};
@@ -772,6 +761,7 @@ int trace_host(foort_gpu_ctx *ctx, uint64_t side, uint64_t first, uint64_t count
         stats->launches = ctx->last_launches;
         stats->device_ms = ms;
         set_warnings(stats, c);
+        stats->graph_launches = ctx->last_graph ? 1 : 0;
     }
     return FOORT_OK;
 }

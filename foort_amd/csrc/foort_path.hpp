@@ -780,10 +780,16 @@ struct RhsMixed {
     static constexpr bool exact = false;
     static constexpr bool split_rk4 = true; // RK4 steps go through rk4_step_mixed
     static constexpr bool stage_sc = false;
+    using Near = RhsAn64<METRIC>;           // the binary64 step near the hole and the poles
     FM_HD_INLINE static bool far(const RunConst &R, const double *p)
     {
         const double th = p[2];
         return p[1] > R.mixed_u && th > R.mixed_pole && (kPi - th) > R.mixed_pole;
+    }
+    // the binary32 right-hand side of the far field
+    FM_HD_INLINE static void eval_far(const RunConst &R, const float *p, const float *v, float *out)
+    {
+        rhs_analytic<METRIC, float, float>(R, p, v, out);
     }
     FM_HD_INLINE static void eval(const RunConst &R, const double *p, const double *v, double *out)
     {
@@ -904,12 +910,12 @@ FM_HD_INLINE void rk4_step(const RunConst &R, const double *x, const double *v, 
 // which is the RK4 combination above (l2 - v = (h/2) k1, l3 - v = (h/2) k2,
 // l4 - v = h k3).  The step size stays binary64.  Near the hole and the poles
 // the step is the binary64 analytic one.
-template <int METRIC>
+template <class RHS>
 FM_HD_INLINE void rk4_step_mixed(const RunConst &R, const double *x, const double *v, double *nx, double *nv,
                                  double &hout)
 {
-    if (!RhsMixed<METRIC>::far(R, x)) {
-        rk4_step<RhsAn64<METRIC>>(R, x, v, nx, nv, hout);
+    if (!RHS::far(R, x)) {
+        rk4_step<typename RHS::Near>(R, x, v, nx, nv, hout);
         return;
     }
     const double h = adaptive_step_fast(R, x, v);
@@ -921,7 +927,7 @@ FM_HD_INLINE void rk4_step_mixed(const RunConst &R, const double *x, const doubl
         xf[i] = static_cast<float>(x[i]);
         vf[i] = static_cast<float>(v[i]);
     }
-    rhs_analytic<METRIC, float, float>(R, xf, vf, k);
+    RHS::eval_far(R, xf, vf, k);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         tp[i] = xf[i] + vf[i] * hh;
@@ -929,7 +935,7 @@ FM_HD_INLINE void rk4_step_mixed(const RunConst &R, const double *x, const doubl
         sv[i] = k[i];
         sk[i] = k[i];
     }
-    rhs_analytic<METRIC, float, float>(R, tp, l, k);
+    RHS::eval_far(R, tp, l, k);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         tp[i] = xf[i] + l[i] * hh;
@@ -937,7 +943,7 @@ FM_HD_INLINE void rk4_step_mixed(const RunConst &R, const double *x, const doubl
         sv[i] = sv[i] + k[i] * 2.0f;
         sk[i] = sk[i] + k[i];
     }
-    rhs_analytic<METRIC, float, float>(R, tp, l, k);
+    RHS::eval_far(R, tp, l, k);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         tp[i] = xf[i] + l[i] * hf;
@@ -945,7 +951,7 @@ FM_HD_INLINE void rk4_step_mixed(const RunConst &R, const double *x, const doubl
         sv[i] = sv[i] + k[i] * 2.0f;
         sk[i] = sk[i] + k[i];
     }
-    rhs_analytic<METRIC, float, float>(R, tp, l, k);
+    RHS::eval_far(R, tp, l, k);
     const float h6 = static_cast<float>(h * (1.0 / 6.0)), hsq6 = static_cast<float>(h * h * (1.0 / 6.0));
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -1232,7 +1238,7 @@ FM_HD_INLINE void geo_update(const RunConst &R, GeoState &g)
     if constexpr (INTEG == FOORT_INTEGRATOR_VERLET)
         verlet_step<RHS>(R, g.pos, g.vel, np, nv, h);
     else if constexpr (RHS::split_rk4)
-        rk4_step_mixed<RHS::metric>(R, g.pos, g.vel, np, nv, h);
+        rk4_step_mixed<RHS>(R, g.pos, g.vel, np, nv, h);
     else
         rk4_step<RHS>(R, g.pos, g.vel, np, nv, h);
     g.lambda += h;

@@ -42,6 +42,7 @@ void add_stats(foort_stats &total, const foort_stats &st)
     total.future_pointing += st.future_pointing;
     total.outward_pointing += st.outward_pointing;
     total.nonfinite += st.nonfinite;
+    total.graph_launches += st.graph_launches;
     for (uint64_t k = 0; k < st.future_pointing; ++k)
         screen_output("Initial velocity of geodesic is future-pointing (should be past-pointing)!", Level::Warning);
     for (uint64_t k = 0; k < st.outward_pointing; ++k)

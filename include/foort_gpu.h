@@ -137,8 +137,9 @@ typedef struct foort_run_desc {
     uint32_t launch_us;          /* persistent schedule: end each kernel launch after about this many
                                     microseconds (0 = no time limit; with both limits 0 a batch is one launch) */
     uint32_t use_graph;          /* host entry points: run each batch that is one launch (one-shot, or
-                                    persistent without limits) as a captured HIP graph -- index upload,
-                                    kernels and result copies -- replayed while the batch shape is unchanged */
+                                    persistent without limits) as one HIP graph -- index upload, kernels and
+                                    result copies -- captured per batch into the context's executable graph,
+                                    which is updated in place (hipGraphExecUpdate) and launched */
     /* GeodesicPosition (FOORT/Diagnostics.h:331-337): its points come from foort_gpu_trace_positions */
     foort_update_frequency gp_update;  /* (1, false, true) */
     uint64_t gp_output_steps;    /* OutputSteps: points kept per geodesic (0 = all) */
@@ -178,6 +179,7 @@ typedef struct foort_stats {
     uint64_t outward_pointing;
     uint64_t nonfinite;       /* geodesics that ended with a NaN or infinite position or velocity,
                                  counted when the call returns final positions and velocities */
+    uint64_t graph_launches;  /* batches run as one captured HIP graph (use_graph) */
 } foort_stats;
 
 typedef struct foort_gpu_ctx foort_gpu_ctx;

@@ -222,10 +222,11 @@ def test_persistent_pixel_list_verlet():
 
 @pytest.mark.parametrize("schedule", [A.SCHEDULE_ONESHOT, A.SCHEDULE_PERSISTENT])
 def test_graph_batches_replay_bitexact(schedule):
-    """use_graph: a batch shape that comes again runs as a captured HIP graph (index upload,
-    kernels, result copies).  The same batch three times -- run directly, captured, replayed --
-    with eager work of another context between them, then batches of other shapes, all equal
-    the oracle."""
+    """use_graph: every batch that is one launch runs as a launch of the context's executable
+    HIP graph (index upload, kernels, result copies), which each batch's capture updates in
+    place -- sizes, grids and arguments change from batch to batch, as an adaptive mesh's do.
+    Batches of different sizes, a grid batch and a repeat, with eager work of another context
+    between them, all equal the oracle."""
     from tests.cases import sample_pixels
     desc = c1(rlog=True, max_steps=10000, schedule=schedule, use_graph=1)
     eager = c1(rlog=True, max_steps=10000)
@@ -234,14 +235,18 @@ def test_graph_batches_replay_bitexact(schedule):
     print(f"[graph] schedule={schedule}", flush=True)
     F = _gpu()
     with F.Context(desc) as ctx, F.Context(eager) as ectx:
-        for rep in range(3):
-            g = ctx.trace_pixels(96, px)
+        for rep, n in enumerate((900, 100, 517, 900, 3)):
+            g = ctx.trace_pixels(96, px[:n])
+            assert g["stats"].graph_launches == 1, rep
             for k in FIELDS:
-                assert bitwise_equal(g[k], o[k]), (rep, k)
+                assert bitwise_equal(g[k], o[k][:n]), (rep, k)
+            assert g["stats"].steps == int(o["steps"][:n].sum())
             e = ectx.trace_pixels(96, px[::-1])
+            assert e["stats"].graph_launches == 0
             assert bitwise_equal(e["quadrant"], o["quadrant"][::-1])
         g2 = ctx.trace_grid(40)
         o2 = O.trace("fm", desc, 40)
+        assert g2["stats"].graph_launches == 1
         for k in FIELDS:
             assert bitwise_equal(g2[k], o2[k]), k
         g3 = ctx.trace_pixels(96, px[:100])

@@ -266,3 +266,26 @@ def test_loop_message_frequency_prints_progress_and_keeps_results(mesh, pixels, 
     assert all(re.match(r"Approx\. at geodesic \d+ \(\d+\.\d+s elapsed; speed: \d+ geod/s; "
                         r"est\. loop time remaining: \d+\.\d+s\)\.\.\.$", ln) for ln in lines)
     assert _tree(tmp_path / "plain") == _tree(tmp_path / "progress")
+
+
+@pytest.mark.parametrize("schedule", ["OneShot", "Persistent"])
+def test_adaptive_run_with_graphs_launches_every_iteration_as_a_graph(schedule, tmp_path, capfd):
+    """UseGraph on an adaptive mesh, whose batch sizes never repeat: every mesh iteration is a
+    launch of one executable HIP graph updated in place, and the files equal those of the run
+    without graphs."""
+    print(f"[program] graphs on an adaptive V2 mesh, {schedule}", flush=True)
+    cfg = tmp_path / "adaptive.cfg"
+    base = ADAPTIVE.format(mesh="SquareSubdivisionMeshV2")
+    runs = {}
+    for graph in ("false", "true"):
+        cfg.write_text(base + f'GPU = {{ Schedule = "{schedule}"; UseGraph = {graph}; }};\n')
+        capfd.readouterr()
+        runs[graph] = P.run(str(cfg), output_prefix=str(tmp_path / graph / "run"), screen_output_level=1)
+        out = capfd.readouterr().out
+    batches = len(re.findall(r"^Integrating [1-9]\d* geodesics on the GPU", out, re.M))
+    plain, graphs = runs["false"], runs["true"]
+    print(f"[program] {batches} mesh iterations, {graphs.graph_launches} graph launches", flush=True)
+    assert batches >= 4
+    assert plain.graph_launches == 0 and graphs.graph_launches == batches
+    assert _tree(tmp_path / "false") == _tree(tmp_path / "true")
+    assert graphs.steps == plain.steps and graphs.geodesics == plain.geodesics
