@@ -16,7 +16,7 @@ import numpy as np
 from .abi import GeodesicResults, RunDesc, Stats
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libfoort_host.so")
+LIB_PATH = os.environ.get("FOORT_HOST_LIB") or os.path.join(_HERE, "libfoort_host.so")
 CLI_PATH = os.path.join(_HERE, "foort")
 _lib = None
 
