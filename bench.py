@@ -33,6 +33,15 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 if REPO not in sys.path:
     sys.path.insert(0, REPO)
 
+# The render server runs one device queue per metric group (four in the C4 stream), each
+# time-sliced on its own stream; HIP maps streams onto GPU_MAX_HW_QUEUES hardware queues (4
+# by default), and once a process has made other streams (the C1 contexts before the server
+# leg) two groups can land on one hardware queue, where their kernels run one after the other
+# (measured: Manko-Novikov p50 7.8 s -> 12.5 s).  HIP reads the variable when it initialises,
+# before the first HIP call of this process.
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) < 8:
+    os.environ["GPU_MAX_HW_QUEUES"] = "8"
+
 METRIC = "pixels/s & geodesic steps/s, Kerr a=0.99 1024² (FP64 vs mixed); p50/p99 latency"
 PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector peak
 

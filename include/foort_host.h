@@ -150,7 +150,10 @@ int foort_program_run(const char *cfg_path, const foort_program_options *opts);
  * group's ring holds the largest power of two of pixels whose 40 bytes each (indices,
  * request slot and results) fit in hbm_budget_bytes; a batch larger than the ring fails
  * its request with FOORT_ENOMEM.  At most 1024 requests of a group are on the device at
- * once; later ones wait for a slot. */
+ * once; later ones wait for a slot.  The groups' queues run concurrently only on distinct
+ * hardware queues: give the process at least one per group and one more (HIP reads
+ * GPU_MAX_HW_QUEUES, default 4, when it initialises); with too few, two groups can share one
+ * and their launches run one after the other. */
 typedef struct foort_request {
     foort_run_desc desc;
     foort_mesh_desc mesh;         /* pixel lists (InputCertainPixelsMesh) are not accepted */
