@@ -571,6 +571,32 @@ FM_HD_INLINE void kj_contract(const S *d1, const S *d2, const S *u, const S *v, 
     out[3] = u[1] * Ft + u[4] * Fp;
 }
 
+// kj_contract for derivatives given as halves (e = d / 2) and the inverse metric's t and
+// phi rows given as w = -2 g^uu (w0 = tt, w1 = tph, w4 = phph); u2 = g^rr, u3 = g^thth.
+// Same equations, without the multiplications by 2 and 1/2.
+template <class S>
+FM_HD_INLINE void kerr_contract(const S *e1, const S *e2, S w0, S w1, S u2, S u3, S w4, const S *v, S *out)
+{
+    const S r_t = e1[0] * v[0] + e1[1] * v[3];
+    const S r_r = e1[2] * v[1];
+    const S r_h = e1[3] * v[2];
+    const S r_p = e1[1] * v[0] + e1[4] * v[3];
+    const S h_t = e2[0] * v[0] + e2[1] * v[3];
+    const S h_r = e2[2] * v[1];
+    const S h_h = e2[3] * v[2];
+    const S h_p = e2[1] * v[0] + e2[4] * v[3];
+    const S Kr = r_t * v[0] + r_r * v[1] + r_h * v[2] + r_p * v[3];
+    const S Kh = h_t * v[0] + h_r * v[1] + h_h * v[2] + h_p * v[3];
+    const S Ft = r_t * v[1] + h_t * v[2]; // -F_t / 2
+    const S Fp = r_p * v[1] + h_p * v[2]; // -F_ph / 2
+    const S Fr = Kr - S(2) * (r_r * v[1] + h_r * v[2]);
+    const S Fh = Kh - S(2) * (r_h * v[1] + h_h * v[2]);
+    out[0] = w0 * Ft + w1 * Fp;
+    out[1] = u2 * Fr;
+    out[2] = u3 * Fh;
+    out[3] = w1 * Ft + w4 * Fp;
+}
+
 FM_HD_INLINE float srcp(float x)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -586,13 +612,16 @@ FM_HD_INLINE double srcp(double x) { return fm_div(1.0, x); }
 // written with Sigma = r^2 + a^2 cos^2, Delta = r^2 - 2r + a^2,
 // q = Sigma - 2 r^2; then the Euler-Lagrange contraction.  With RLogScale the
 // radial coordinate is u = log r: v^r = r v^u and a^u = a^r / r - (v^u)^2.
-template <class S, class T>
+// RL: 0 or 1 fixes RLogScale at compile time (the step then carries no select for the
+// other case), -1 reads it from R.
+template <class S, class T, int RL = -1>
 FM_HD_INLINE void rhs_kerr_closed_sc(const RunConst &R, const T *p, const T *v, S s, S c, T *out)
 {
+    const bool rlog = RL < 0 ? R.M.rlog != 0 : RL != 0;
     const S a = static_cast<S>(R.M.a);
     const S a2 = a * a;
     S r, vr;
-    if (R.M.rlog) {
+    if (rlog) {
         r = sexp(static_cast<S>(p[1]));
         vr = r * static_cast<S>(v[1]);
     } else {
@@ -611,32 +640,33 @@ FM_HD_INLINE void rhs_kerr_closed_sc(const RunConst &R, const T *p, const T *v, 
     const S iS2 = iS * iS;
     const S q = Sig - S(2) * r2;
     const S qS = q * iS2;
-    const S a2sc = a2 * sc;
-    const S arS = S(4) * r * iS2;  // 4 r / Sigma^2
-    S d1[5], d2[5], uu[5];
-    d1[0] = S(2) * qS;                                       // d_r g_tt
-    d2[0] = arS * a2sc;                                      // d_th g_tt
-    d1[1] = S(-2) * a * s2 * qS;                             // d_r g_tph
-    d2[1] = -(arS * a) * sc * rpa;                           // d_th g_tph
-    d1[2] = (S(2) * r * Del - Sig * (S(2) * r - S(2))) * (iD * iD); // d_r g_rr
-    d2[2] = S(-2) * a2sc * iD;                               // d_th g_rr
-    d1[3] = S(2) * r;                                        // d_r g_thth
-    d2[3] = S(-2) * a2sc;                                    // d_th g_thth
-    d1[4] = s2 * (S(2) * r + S(2) * a2 * s2 * qS);           // d_r g_phph
-    d2[4] = S(2) * sc * rpa + arS * a2sc * s2 * (S(2) * Sig + a2 * s2); // d_th g_phph
-    const S A_ = rpa * rpa - a2 * Del * s2;
-    uu[0] = -A_ * iSD;
-    uu[1] = S(-2) * a * r * iSD;
-    uu[2] = Del * iS;
-    uu[3] = iS;
-    uu[4] = (Del - a2 * s2) * iSD * is2;
+    const S a2sc = a2 * sc, a2s2 = a2 * s2, scr = sc * rpa;
+    // e1, e2: half the derivatives along r and theta of g_tt, g_tph, g_rr, g_thth, g_phph
+    // (every one carries a factor 2, which the contraction below takes back)
+    S e1[5], e2[5];
+    e1[0] = qS;
+    e2[0] = ((r + r) * iS2) * a2sc;                         // 2 r a^2 sin cos / Sigma^2
+    e1[1] = -(a * s2) * qS;
+    e2[1] = -(((r + r) * iS2) * a) * scr;
+    e1[2] = (r * Del - Sig * (r - S(1))) * (iD * iD);
+    e2[2] = -a2sc * iD;
+    e1[3] = r;
+    e2[3] = -a2sc;
+    e1[4] = s2 * (r + a2s2 * qS);
+    e2[4] = scr + (e2[0] * s2) * (S(2) * Sig + a2s2);
+    // g^uu with the factor -2 of the t and phi rows of the contraction folded into
+    // w0 = -2 g^tt, w1 = -2 g^tph, w4 = -2 g^phph
+    const S iSD2 = iSD + iSD;
+    const S w0 = (rpa * rpa - a2s2 * Del) * iSD2;
+    const S w1 = ((a + a) * r) * iSD2;
+    const S w4 = (a2s2 - Del) * (iSD2 * is2);
     const S vv[4] = {static_cast<S>(v[0]), vr, static_cast<S>(v[2]), static_cast<S>(v[3])};
     S o[4];
-    kj_contract<S>(d1, d2, uu, vv, o);
+    kerr_contract<S>(e1, e2, w0, w1, Del * iS, iS, w4, vv, o);
     out[0] = static_cast<T>(o[0]);
     out[2] = static_cast<T>(o[2]);
     out[3] = static_cast<T>(o[3]);
-    if (R.M.rlog) {
+    if (rlog) {
         const T vu = v[1];
         out[1] = static_cast<T>(o[1] * srcp(r)) - vu * vu;
     } else {
@@ -644,12 +674,12 @@ FM_HD_INLINE void rhs_kerr_closed_sc(const RunConst &R, const T *p, const T *v, 
     }
 }
 
-template <class S, class T>
+template <class S, class T, int RL = -1>
 FM_HD_INLINE void rhs_kerr_closed(const RunConst &R, const T *p, const T *v, T *out)
 {
     S s, c;
     ssincos(p[2], &s, &c);
-    rhs_kerr_closed_sc<S, T>(R, p, v, s, c, out);
+    rhs_kerr_closed_sc<S, T, RL>(R, p, v, s, c, out);
 }
 
 // Analytic right-hand side: the metric and its exact derivatives in one
@@ -715,20 +745,20 @@ struct RhsFD {
     }
 };
 
-template <int METRIC, class S, class T = double>
+template <int METRIC, class S, class T = double, int RL = -1>
 FM_HD_INLINE void rhs_analytic(const RunConst &R, const T *p, const T *v, T *out)
 {
     if constexpr (METRIC == FOORT_METRIC_KERR)
-        rhs_kerr_closed<S, T>(R, p, v, out);
+        rhs_kerr_closed<S, T, RL>(R, p, v, out);
     else
         rhs_an<METRIC, S, T>(R, p, v, out);
 }
 
-template <int METRIC, class S, class T = S>
+template <int METRIC, class S, class T = S, int RL = -1>
 FM_HD_INLINE void rhs_analytic_sc(const RunConst &R, const T *p, const T *v, S s, S c, T *out)
 {
     if constexpr (METRIC == FOORT_METRIC_KERR)
-        rhs_kerr_closed_sc<S, T>(R, p, v, s, c, out);
+        rhs_kerr_closed_sc<S, T, RL>(R, p, v, s, c, out);
     else
         rhs_an_sc<METRIC, S, T>(R, p, v, s, c, out);
 }
@@ -757,30 +787,39 @@ FM_HD_INLINE void stage_sincos(double s0, double c0, double th0, double th, doub
         ssincos(th, s, c);
 }
 
-template <int METRIC>
+// The analytic policies take RL as the right-hand sides do (-1: RLogScale read from the
+// run constants); Fixed<RL> is the same policy with RLogScale fixed, which geo_update
+// selects once per step for the closed-form Kerr right-hand side.
+template <int METRIC, int RL = -1>
 struct RhsAn64 {
     static constexpr int metric = METRIC;
     static constexpr bool exact = false;
     static constexpr bool split_rk4 = false;
     static constexpr bool stage_sc = true; // RK4 stages take sin/cos of theta from the step's start
+    static constexpr bool fix_rlog = METRIC == FOORT_METRIC_KERR && RL < 0;
+    template <int X>
+    using Fixed = RhsAn64<METRIC, X>;
     FM_HD_INLINE static void eval(const RunConst &R, const double *p, const double *v, double *out)
     {
-        rhs_analytic<METRIC, double>(R, p, v, out);
+        rhs_analytic<METRIC, double, double, RL>(R, p, v, out);
     }
     FM_HD_INLINE static void eval_sc(const RunConst &R, const double *p, const double *v, double s, double c,
                                      double *out)
     {
-        rhs_analytic_sc<METRIC, double>(R, p, v, s, c, out);
+        rhs_analytic_sc<METRIC, double, double, RL>(R, p, v, s, c, out);
     }
 };
 
-template <int METRIC>
+template <int METRIC, int RL = -1>
 struct RhsMixed {
     static constexpr int metric = METRIC;
     static constexpr bool exact = false;
     static constexpr bool split_rk4 = true; // RK4 steps go through rk4_step_mixed
     static constexpr bool stage_sc = false;
-    using Near = RhsAn64<METRIC>;           // the binary64 step near the hole and the poles
+    static constexpr bool fix_rlog = METRIC == FOORT_METRIC_KERR && RL < 0;
+    template <int X>
+    using Fixed = RhsMixed<METRIC, X>;
+    using Near = RhsAn64<METRIC, RL>;       // the binary64 step near the hole and the poles
     FM_HD_INLINE static bool far(const RunConst &R, const double *p)
     {
         const double th = p[2];
@@ -789,14 +828,14 @@ struct RhsMixed {
     // the binary32 right-hand side of the far field
     FM_HD_INLINE static void eval_far(const RunConst &R, const float *p, const float *v, float *out)
     {
-        rhs_analytic<METRIC, float, float>(R, p, v, out);
+        rhs_analytic<METRIC, float, float, RL>(R, p, v, out);
     }
     FM_HD_INLINE static void eval(const RunConst &R, const double *p, const double *v, double *out)
     {
         if (far(R, p))
-            rhs_analytic<METRIC, float>(R, p, v, out);
+            rhs_analytic<METRIC, float, double, RL>(R, p, v, out);
         else
-            rhs_analytic<METRIC, double>(R, p, v, out);
+            rhs_analytic<METRIC, double, double, RL>(R, p, v, out);
     }
 };
 
@@ -1230,6 +1269,32 @@ FM_HD_INLINE uint32_t skip_nan_steps(const RunConst &R, GeoState &g)
     return skip;
 }
 
+// true for a policy whose fix_rlog is true (policies without the member: false)
+template <class RHS>
+constexpr auto rhs_fixes_rlog_(int) -> decltype(RHS::fix_rlog, bool())
+{
+    return RHS::fix_rlog;
+}
+template <class RHS>
+constexpr bool rhs_fixes_rlog_(long)
+{
+    return false;
+}
+template <class RHS>
+constexpr bool rhs_fixes_rlog()
+{
+    return rhs_fixes_rlog_<RHS>(0);
+}
+
+template <class RHS>
+FM_HD_INLINE void geo_rk4(const RunConst &R, const double *x, const double *v, double *nx, double *nv, double &h)
+{
+    if constexpr (RHS::split_rk4)
+        rk4_step_mixed<RHS>(R, x, v, nx, nv, h);
+    else
+        rk4_step<RHS>(R, x, v, nx, nv, h);
+}
+
 // Geodesic::Update, FOORT/Geodesic.cpp:61-88
 template <class RHS, int INTEG, bool EVERY>
 FM_HD_INLINE void geo_update(const RunConst &R, GeoState &g)
@@ -1237,10 +1302,14 @@ FM_HD_INLINE void geo_update(const RunConst &R, GeoState &g)
     double np[4], nv[4], h;
     if constexpr (INTEG == FOORT_INTEGRATOR_VERLET)
         verlet_step<RHS>(R, g.pos, g.vel, np, nv, h);
-    else if constexpr (RHS::split_rk4)
-        rk4_step_mixed<RHS>(R, g.pos, g.vel, np, nv, h);
-    else
-        rk4_step<RHS>(R, g.pos, g.vel, np, nv, h);
+    else if constexpr (rhs_fixes_rlog<RHS>()) {
+        // one uniform branch per step instead of selects in every right-hand side
+        if (R.M.rlog)
+            geo_rk4<typename RHS::template Fixed<1>>(R, g.pos, g.vel, np, nv, h);
+        else
+            geo_rk4<typename RHS::template Fixed<0>>(R, g.pos, g.vel, np, nv, h);
+    } else
+        geo_rk4<RHS>(R, g.pos, g.vel, np, nv, h);
     g.lambda += h;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {

@@ -43,7 +43,19 @@ void dispatch(int kind, const foort::RunConst &R, const double *p, const double 
     case 0: foort::rhs_fd<M>(R, p, v, out); break;
     case 1: foort::rhs_an<M, double>(R, p, v, out); break;                 // dual numbers, binary64
     case 2: foort::rhs_analytic<M, double>(R, p, v, out); break;           // closed form where built, binary64
-    default: foort::rhs_analytic<M, float>(R, p, v, out); break;           // same in binary32
+    case 3: foort::rhs_analytic<M, float>(R, p, v, out); break;            // same in binary32
+    // the same two with RLogScale fixed at compile time, as the integration kernels run them
+    case 4:
+        if (R.M.rlog)
+            foort::rhs_analytic<M, double, double, 1>(R, p, v, out);
+        else
+            foort::rhs_analytic<M, double, double, 0>(R, p, v, out);
+        break;
+    default:
+        if (R.M.rlog)
+            foort::rhs_analytic<M, float, double, 1>(R, p, v, out);
+        else
+            foort::rhs_analytic<M, float, double, 0>(R, p, v, out);
     }
 }
 } // namespace

@@ -7,7 +7,8 @@ reference's finite differences:
 * both agree with the finite-difference right-hand side (the reference's
   Christoffel symbols, FOORT/Metric.cpp:14-55) within its truncation and
   cancellation error (a few 1e-6 relative at h = 1e-7);
-* the binary32 evaluation stays within 1e-5 relative.
+* the binary32 evaluation stays within 1e-5 relative;
+* the instantiations with RLogScale fixed at compile time give the same values.
 """
 import ctypes as C
 import os
@@ -50,11 +51,13 @@ def test_analytic_formulations_agree(name):
         u = np.log(r) if d.rlog_scale else r
         p = np.array([0.0, u, rng.uniform(0.1, 3.0), 0.0])
         v = rng.normal(size=4)
-        fd, dual, closed, f32 = (_rhs(L, d, k, p, v) for k in range(4))
+        fd, dual, closed, f32, closed_rl, f32_rl = (_rhs(L, d, k, p, v) for k in range(6))
         scale = np.abs(dual).max()
         assert np.abs(closed - dual).max() <= 1e-14 * scale
         assert np.abs(fd - dual).max() <= 5e-6 * scale
         assert np.abs(f32 - dual).max() <= 1e-5 * scale
+        # RLogScale fixed at compile time (the kernels' instantiations): the same values
+        assert (closed_rl == closed).all() and (f32_rl == f32).all()
 
 
 @pytest.mark.parametrize("name", sorted(DESCS))
@@ -65,7 +68,7 @@ def test_nan_state_gives_nan_derivatives(name):
     L = C.CDLL(LIB)
     d = DESCS[name]
     nan = np.full(4, np.nan)
-    for kind in range(4):
+    for kind in range(6):
         assert np.isnan(_rhs(L, d, kind, nan, nan)).all(), kind
 
 
