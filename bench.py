@@ -48,7 +48,7 @@ PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector peak
 
 def hbm_bytes_per_pixel(christoffel, precision, schedule):
     """HBM bytes per pixel of a path's kernels, measured with rocprofv3 FETCH_SIZE and WRITE_SIZE
-    (each calibrated on a plain device copy) on a 256x256 image: foort_amd/hbm_traffic.json,
+    (each calibrated on a plain device copy) on the 1024x1024 image: foort_amd/hbm_traffic.json,
     written by foort_amd/counters.py; None for a path not measured."""
     from foort_amd import counters
     t = counters.load()
@@ -472,7 +472,7 @@ def main():
             "traffic": (lambda b: b * n if b is not None else None)(
                 hbm_bytes_per_pixel(args.christoffel, args.precision, args.schedule)),
             "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE per pixel of this path (calibrated on a device "
-                              "copy, 256x256 image) x the launch's pixels: profiles/hbm_traffic_kerr_rk4_c1_256.md",
+                              "copy, the 1024x1024 image) x the launch's pixels: profiles/hbm_traffic_kerr_rk4_c1.md",
             "kernel": kernel,
             "flops_per_step": closed["rk4_step_flops"],
             "flops_per_step_formulation": closed["formulation"],

@@ -2,12 +2,12 @@
 
 bench.py reports `roofline.traffic` (HBM bytes per launch) from foort_amd/hbm_traffic.json,
 which this script writes from two counter runs per path (FETCH_SIZE, then WRITE_SIZE, in KiB
-per dispatch) on a 256x256 Kerr image, each counter scaled by what it reads for a plain device
-copy of 256 MiB.  The commands that produce the runs (repo root; <c> <p> <s> = Christoffel,
-precision, schedule of each path):
+per dispatch) on the benchmark's own 1024x1024 Kerr image, each counter scaled by what it reads
+for a plain device copy of 256 MiB.  The commands that produce the runs (repo root; <c> <p> <s>
+= Christoffel, precision, schedule of each path):
 
     rocprofv3 --pmc FETCH_SIZE -f csv -d <dir>/traffic_<c>_<p>_<s>_FETCH_SIZE -o r -- \\
-        python3 -u bench.py --side 256 --steps 1 --warmup 0 --no-variants --no-cpu --no-server \\
+        python3 -u bench.py --steps 1 --warmup 0 --no-variants --no-cpu --no-server \\
         --christoffel <c> --precision <p> --schedule <s>
     (the same with WRITE_SIZE)
     rocprofv3 --pmc FETCH_SIZE -f csv -d <dir>/copy_FETCH_SIZE -o r -- python3 -c "import torch; \\
@@ -24,9 +24,9 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "hbm_traffic.json")
 PATHS = [("fd", "fp64", "oneshot"), ("fd", "fp64", "persistent"), ("analytic", "fp64", "oneshot"),
-         ("analytic", "mixed", "oneshot"), ("analytic", "mixed", "persistent")]
+         ("analytic", "fp64", "persistent"), ("analytic", "mixed", "oneshot"), ("analytic", "mixed", "persistent")]
 COPY_BYTES = 256 << 20
-PIXELS = 256 * 256
+PIXELS = 1024 * 1024
 
 
 def _dispatches(path):
@@ -46,7 +46,7 @@ def build(run_dir):
     copy = {c: _sum(_dispatches(csv_of(f"copy_{c}")), c, lambda n: "copyBuffer" in n)
             for c in ("FETCH_SIZE", "WRITE_SIZE")}
     factor = {c: COPY_BYTES / 1024.0 / v for c, v in copy.items()}
-    out = {"workload": "Kerr a=0.99, 85 deg, 256x256 SimpleSquareMesh, RK4, FP64 state (C1 settings at 256^2)",
+    out = {"workload": "Kerr a=0.99, 85 deg, 1024x1024 SimpleSquareMesh, RK4, FP64 state (C1)",
            "pixels": PIXELS, "copy_kib": copy, "factor": factor, "paths": {}}
     for c, p, s in PATHS:
         entry = {"kernels": set()}

@@ -28,6 +28,12 @@
 #ifndef FOORT_WPE_ONESHOT_MIXED
 #define FOORT_WPE_ONESHOT_MIXED 4
 #endif
+// Resident kernels of the mixed formulation: three waves per SIMD (a few VGPRs spilled).
+// Measured on MI355X on a 1024x1024 Kerr image: the persistent kernel 3% faster than at two
+// waves, no faster at four (which spills more than the one-shot kernel does).
+#ifndef FOORT_WPE_PERSIST_MIXED
+#define FOORT_WPE_PERSIST_MIXED 3
+#endif
 // Manko-Novikov, Rasheed-Larsen and (except its finite-difference one-shot kernel) Johannsen:
 // their potentials with derivatives need more registers than two waves per SIMD leave, and
 // spills to scratch memory stall every step, worst for a lone wave (the time-outs that end an
@@ -51,7 +57,7 @@ constexpr int waves_per_eu(bool resident)
     if (RHS::metric == FOORT_METRIC_JOHANNSEN)
         return (RHS::exact && !resident) ? FOORT_WPE_ONESHOT : FOORT_WPE_HEAVY;
     if (resident)
-        return FOORT_WPE_PERSIST;
+        return RHS::split_rk4 ? FOORT_WPE_PERSIST_MIXED : FOORT_WPE_PERSIST;
     if (RHS::exact)
         return FOORT_WPE_ONESHOT;
     return RHS::split_rk4 ? FOORT_WPE_ONESHOT_MIXED : FOORT_WPE_ONESHOT_AN64;
