@@ -15,11 +15,13 @@
  * every function returns bit-identical results on host and device.
  *
  * Methods (all constants come from gen_fm_consts.py, computed with mpmath):
- *  - fm_sincos: Cody-Waite reduction by pi/2 with a three-part constant and an
- *    exact product split by fma, carrying the reduced argument as a
- *    double-double; Taylor polynomials of sin and cos on [-pi/4, pi/4]
- *    (degree 17 and 18, truncation error < 2^-62), the cosine assembled with
- *    the exact correction of 1 - x^2/2 (the technique of the fdlibm kernels).
+ *  - fm_sincos: Cody-Waite reduction by pi/2 with a three-part constant (the
+ *    first part subtracted exactly by one fma), carrying the reduced argument
+ *    as a double-double; minimax polynomials of sin and cos on [-pi/4, pi/4]
+ *    (degree 13 and 14, six coefficients each, relative error 2^-57.9 and
+ *    2^-64.0, Remez exchange in gen_fm_consts.py) evaluated by fused
+ *    multiply-adds, the cosine assembled with the exact correction of
+ *    1 - x^2/2 (the technique of the fdlibm kernels).
  *  - fm_exp: P. T. P. Tang, "Table-driven implementation of the exponential
  *    function in IEEE floating-point arithmetic", ACM TOMS 15 (1989): x =
  *    (64k + j) ln2/64 + r, exp(x) = 2^k 2^(j/64) exp(r), |r| <= ln2/128,
@@ -27,8 +29,10 @@
  *  - fm_pow2/3/4/8: integer powers by double-double products (Dekker's exact
  *    product via fma), so the result is the correctly rounded power except in
  *    rare near-midpoint cases.
- * Accuracy (tests/test_fm_math.py, against mpmath): below 0.52 ulp on the
- * argument ranges the path uses.
+ * Accuracy against mpmath (tests/test_fm_math.py): sin and cos within 0.69 ulp
+ * on the path's arguments (about 1.5% of results not correctly rounded), exp
+ * within 0.55 ulp, the integer powers correctly rounded but for rare
+ * near-midpoint cases.
  */
 #ifndef FOORT_FM_MATH_H
 #define FOORT_FM_MATH_H
@@ -42,6 +46,20 @@
 #endif
 
 #include "fm_consts.inc"
+
+/* A polynomial coefficient as an operand of a fused multiply-add.  On the GPU the value is
+   placed in a scalar register pair, which the multiply-add reads directly; as a literal the
+   compiler copies it into vector registers first (two extra vector instructions per term).
+   The empty statement emits no instruction and leaves the value unchanged. */
+#if defined(__HIP_DEVICE_COMPILE__)
+FM_FN double fm_k(double c)
+{
+    __asm__("" : "+s"(c));
+    return c;
+}
+#else
+#define fm_k(c) (c)
+#endif
 
 /* round-to-nearest-even of |y| < 2^51 by the 1.5*2^52 shift */
 #define FM_SHIFT 0x1.8p52
@@ -112,25 +130,36 @@ FM_FN void fm_sincos(double x, double *s, double *c)
         *c = x - x;
         return;
     }
-    double kd = (x * FM_TWO_OVER_PI + FM_SHIFT) - FM_SHIFT;
-    double t = kd * FM_PIO2_1;
-    double e = __builtin_fma(kd, FM_PIO2_1, -t); /* kd*P1 == t + e exactly */
-    double r = x - t;                             /* exact (Sterbenz) */
-    double cc = e + kd * FM_PIO2_2;
-    cc = cc + kd * FM_PIO2_3;
+    /* x = kd pi/2 + (hi + lo).  kd is the integer nearest x 2/pi.  x - kd P1 is exact in one
+       fused multiply-add: for |x| < 2^30, x and kd P1 are both multiples of 2^-53 (|x| < 1)
+       or of 2^-52 (|x| >= 1) and differ by less than 0.8, so the difference fits 53 bits.
+       The two lower parts of pi/2 follow as a double-double. */
+    double kd = __builtin_fma(x, FM_TWO_OVER_PI, FM_SHIFT) - FM_SHIFT;
+    double r = __builtin_fma(-kd, FM_PIO2_1, x);
+    double cc = __builtin_fma(kd, FM_PIO2_2, kd * FM_PIO2_3);
     double lo;
     double hi = fm_two_sum(r, -cc, &lo);
 
     double z = hi * hi;
-    /* sin(hi + lo) = hi + hi^3 S(z) + lo (1 - z/2) */
-    double ps = FM_S5 + z * (FM_S7 + z * (FM_S9 + z * (FM_S11 + z * (FM_S13 + z * (FM_S15 + z * FM_S17)))));
-    double sn = hi + ((hi * z) * (FM_S3 + z * ps) + lo * (1.0 - 0.5 * z));
-    /* cos(hi + lo) = 1 - z/2 + z^2 C(z) - hi lo, with z = zh + zl exactly */
-    double zl = __builtin_fma(hi, hi, -z);
-    double pc = (z * z) * (FM_C4 + z * (FM_C6 + z * (FM_C8 + z * (FM_C10 + z * (FM_C12 + z * (FM_C14 + z * (FM_C16 + z * FM_C18)))))));
     double hz = 0.5 * z;
+    /* sin(hi + lo) = hi + hi^3 S(z) + lo (1 - z/2) */
+    double ps = __builtin_fma(z, FM_S13, fm_k(FM_S11));
+    ps = __builtin_fma(z, ps, fm_k(FM_S9));
+    ps = __builtin_fma(z, ps, fm_k(FM_S7));
+    ps = __builtin_fma(z, ps, fm_k(FM_S5));
+    ps = __builtin_fma(z, ps, fm_k(FM_S3));
+    double sn = hi + __builtin_fma(hi * z, ps, __builtin_fma(-hz, lo, lo));
+    /* cos(hi + lo) = 1 - z/2 + z^2 C(z) - hi lo, with hi^2 = z + zl exactly; 1 - z/2 as
+       w + ((1 - w) - z/2) with the exact correction of w = 1 - z/2 (the fdlibm kernels' device) */
+    double zl = __builtin_fma(hi, hi, -z);
+    double pc = __builtin_fma(z, FM_C14, fm_k(FM_C12));
+    pc = __builtin_fma(z, pc, fm_k(FM_C10));
+    pc = __builtin_fma(z, pc, fm_k(FM_C8));
+    pc = __builtin_fma(z, pc, fm_k(FM_C6));
+    pc = __builtin_fma(z, pc, fm_k(FM_C4));
+    pc = (z * z) * pc;
     double w = 1.0 - hz;
-    double cs = w + ((((1.0 - w) - hz) - 0.5 * zl) + (pc - hi * lo));
+    double cs = w + (__builtin_fma(-0.5, zl, (1.0 - w) - hz) + __builtin_fma(-hi, lo, pc));
 
     /* quadrant n: (s, c) = (sn, cs), (cs, -sn), (-sn, -cs), (-cs, sn) for n = 0..3;
        selected without branches (only exact moves and negations) */
