@@ -14,7 +14,14 @@ metrics every g_dd and g^uu has the same non-zero pattern {00, 03 = 30, 11,
 structurally zero adds +-0 to the running sum and leaves every non-zero
 value unchanged, so dropping it keeps the result bit-identical (only the sign
 of an exact zero can differ).  Inside a bracket, zero operands are dropped
-the same way: (A + 0) - C == A - C, (0 + 0) - C == 0 - C.
+the same way: (A + 0) - C == A - C, (0 + 0) - C == -C.
+
+A Christoffel symbol's leading +0 (0 + t1 + t2 ...) and the 0 of 0 - C are
+dropped as well: they change only the sign of a zero symbol, and a symbol
+enters the result only as a product subtracted from a running sum that
+starts at +0, which no subtraction of a zero (of either sign) changes and
+which never becomes -0 (x - x is +0 in round-to-nearest).  The running sums
+keep their leading +0, so every output, zeros included, is the reference's.
 
 Inputs of the generated function: d1[5], d2[5] (d/dr, d/dtheta of g00 g03
 g11 g22 g33), u[5] (g^uu in the same order), v[4].  Output: out[4].
@@ -45,7 +52,7 @@ def bracket(nu, rho, sigma):
     else:
         s = A or B
     if Cc:
-        s = "(%s - %s)" % (s, Cc) if s else "(T(0) - %s)" % Cc
+        s = "(%s - %s)" % (s, Cc) if s else "(-%s)" % Cc
     return s
 
 
@@ -69,7 +76,7 @@ def main():
                         terms.append("%s * %s" % (un, br))
                 if terms:
                     name = "G%d%d%d" % (mu, nu, rho)
-                    expr = "T(0) + " + terms[0]
+                    expr = terms[0]
                     for t in terms[1:]:
                         expr = "(%s) + %s" % (expr, t)
                     w("    const T %s = %s;" % (name, expr))
