@@ -293,6 +293,9 @@ struct foort_gpu_ctx {
     size_t d_cap = 0;
     unsigned long long *d_counters = nullptr; // [0] steps, [1] next pixel, [2] pixels done
     unsigned long long *d_steps = nullptr;
+    // device copy of rc, in the same allocation (the resident kernels' refills read the camera
+    // from it); written at open and by every configure that changes rc
+    RunConst *d_rc = nullptr;
     // persistent schedule
     LaneSlots slots{};
     void *d_slots = nullptr;
@@ -416,13 +419,17 @@ int lane_count(foort_gpu_ctx *ctx, uint64_t &L)
     return FOORT_OK;
 }
 
+// offset of the device copy of the run constants in a context's counter allocation
+constexpr size_t kRunConstOffset = 256;
+
 // bytes of lane state per resident lane: kSlotD doubles, the pixel index, kSlotU words
 constexpr size_t kLaneBytes = foort::kSlotD * 8 + 8 + foort::kSlotU * 4;
 
 // Point the lane slots of S at L lanes of state starting at `base` (256-byte aligned), with
 // the claim and done counters of the batch counters c.
-void place_slots(LaneSlots &S, char *base, uint64_t L, unsigned long long *c)
+void place_slots(LaneSlots &S, char *base, uint64_t L, unsigned long long *c, const RunConst *rc)
 {
+    S.rc = rc;
     S.d = reinterpret_cast<double *>(base);
     S.pix = reinterpret_cast<long long *>(base + L * foort::kSlotD * 8);
     S.u = reinterpret_cast<uint32_t *>(base + L * (foort::kSlotD * 8 + 8));
@@ -443,7 +450,7 @@ int setup_persistent(foort_gpu_ctx *ctx)
     hipError_t e = hipMalloc(&ctx->d_slots, L * kLaneBytes);
     if (e != hipSuccess)
         return fail(ctx, FOORT_ENOMEM, std::string("hipMalloc of lane slots: ") + hipGetErrorString(e));
-    place_slots(ctx->slots, static_cast<char *>(ctx->d_slots), L, ctx->d_counters);
+    place_slots(ctx->slots, static_cast<char *>(ctx->d_slots), L, ctx->d_counters, ctx->d_rc);
     return FOORT_OK;
 }
 
@@ -525,7 +532,7 @@ int scratch_work(foort_gpu_ctx *ctx, Work &w)
                         "holds at most " + std::to_string(kScratchLanes));
         L = kScratchLanes;
     }
-    place_slots(w.slots, ctx->scratch + kScratchCounterBytes, L, w.counters);
+    place_slots(w.slots, ctx->scratch + kScratchCounterBytes, L, w.counters, ctx->d_rc);
     return FOORT_OK;
 }
 
@@ -871,6 +878,11 @@ int configure(foort_gpu_ctx *ctx, const foort_run_desc &d)
         ctx->d_slots = nullptr;
     }
     drop_graph(ctx);
+    if (ctx->d_rc && std::memcmp(&ctx->rc, &R, sizeof R) != 0) {
+        // kernels of earlier batches may still read the device copy (on any stream)
+        HIP_TRY(ctx, hipDeviceSynchronize());
+        HIP_TRY(ctx, hipMemcpy(ctx->d_rc, &R, sizeof R, hipMemcpyHostToDevice));
+    }
     ctx->desc = d;
     ctx->rc = R;
     ctx->kernels = kernels;
@@ -1022,9 +1034,12 @@ int foort_gpu_open(const foort_run_desc *desc, foort_gpu_ctx **out)
     if (e == hipSuccess)
         e = hipDeviceGetAttribute(&ctx->clock_khz, hipDeviceAttributeWallClockRate, ctx->device);
     if (e == hipSuccess)
-        e = hipMalloc(reinterpret_cast<void **>(&ctx->d_counters), 8 * sizeof(unsigned long long));
-    if (e == hipSuccess)
+        e = hipMalloc(reinterpret_cast<void **>(&ctx->d_counters), kRunConstOffset + sizeof(RunConst));
+    if (e == hipSuccess) {
         ctx->d_steps = ctx->d_counters;
+        ctx->d_rc = reinterpret_cast<RunConst *>(reinterpret_cast<char *>(ctx->d_counters) + kRunConstOffset);
+        e = hipMemcpy(ctx->d_rc, &ctx->rc, sizeof(RunConst), hipMemcpyHostToDevice);
+    }
     if (e == hipSuccess)
         e = hipEventCreate(&ctx->ev0);
     if (e == hipSuccess)

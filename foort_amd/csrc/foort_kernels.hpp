@@ -90,6 +90,7 @@ struct LaneSlots {
     uint64_t ticks;            // wall-clock ticks after which a launch ends (0 = no time limit)
     uint64_t tail;             // leave once every pixel is claimed and at most this many are unfinished,
                                // for the four-lane tail kernel (0 = never)
+    const RunConst *rc;        // device copy of the launch's run constants (refills read the camera)
 };
 
 // A ring of pixels from many requests that share one kernel and every run
@@ -170,6 +171,17 @@ __device__ __forceinline__ void pixel_unit(uint64_t side, uint64_t first, const 
 __device__ __forceinline__ void term_after_run(int32_t &term)
 {
     asm volatile("" : "+v"(term));
+}
+
+// The run constants for a refill's initial conditions (camera frame, screen), read from the
+// context's device copy where a lane takes a new pixel.  The empty asm hides where the pointer
+// points, so the compiler cannot hoist the loads out of the resident kernel's loop: the camera
+// constants, held in scalar registers across the step loop (as the by-value argument's are),
+// were spilled to vector-register lanes and reloaded by vector instructions inside it.
+__device__ __forceinline__ const RunConst &refill_consts(const RunConst *rc)
+{
+    asm volatile("" : "+s"(rc));
+    return *rc;
 }
 
 __device__ __forceinline__ void add_steps(unsigned long long *total, unsigned long long mine)
@@ -325,7 +337,7 @@ trace_persistent(const RunConst R, uint64_t side, uint64_t first,
                     pix = static_cast<long long>(k);
                     double ux, uy;
                     pixel_unit(side, first, si, k, ux, uy);
-                    geo_init<EVERY>(R, ux, uy, g);
+                    geo_init<EVERY>(refill_consts(S.rc), ux, uy, g);
                 } else {
                     exhausted = true;
                 }
