@@ -71,16 +71,16 @@ FM_HD_INLINE double sexp(double x)
     double r = __builtin_fma(-k, 0x1.62e42fefa39efp-1, x);
     r = __builtin_fma(-k, 0x1.abc9e3b39803fp-56, r);
     double p = 1.0 / 6227020800.0;            // 1/13!
-    p = __builtin_fma(p, r, 1.0 / 479001600.0); // 1/12!
-    p = __builtin_fma(p, r, 1.0 / 39916800.0);
-    p = __builtin_fma(p, r, 1.0 / 3628800.0);
-    p = __builtin_fma(p, r, 1.0 / 362880.0);
-    p = __builtin_fma(p, r, 1.0 / 40320.0);
-    p = __builtin_fma(p, r, 1.0 / 5040.0);
-    p = __builtin_fma(p, r, 1.0 / 720.0);
-    p = __builtin_fma(p, r, 1.0 / 120.0);
-    p = __builtin_fma(p, r, 1.0 / 24.0);
-    p = __builtin_fma(p, r, 1.0 / 6.0);
+    p = __builtin_fma(p, r, fm_k(1.0 / 479001600.0)); // 1/12!
+    p = __builtin_fma(p, r, fm_k(1.0 / 39916800.0));
+    p = __builtin_fma(p, r, fm_k(1.0 / 3628800.0));
+    p = __builtin_fma(p, r, fm_k(1.0 / 362880.0));
+    p = __builtin_fma(p, r, fm_k(1.0 / 40320.0));
+    p = __builtin_fma(p, r, fm_k(1.0 / 5040.0));
+    p = __builtin_fma(p, r, fm_k(1.0 / 720.0));
+    p = __builtin_fma(p, r, fm_k(1.0 / 120.0));
+    p = __builtin_fma(p, r, fm_k(1.0 / 24.0));
+    p = __builtin_fma(p, r, fm_k(1.0 / 6.0));
     p = __builtin_fma(p, r, 0.5);
     p = __builtin_fma(p, r, 1.0);
     p = __builtin_fma(p, r, 1.0);
