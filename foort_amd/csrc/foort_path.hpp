@@ -110,7 +110,7 @@ template <class S>
 FM_HD_INLINE D2<S> fsqrt(const D2<S> &a)
 {
     const S r = ssqrt(a.v);
-    const S k = S(0.5) / r;
+    const S k = S(0.5) * arcp(r);
     return D2<S>(r, a.du * k, a.dt * k);
 }
 template <class S>
@@ -463,6 +463,28 @@ FM_HD_INLINE void mn_uu_from(const MetricParams &M, const MNCommon<T> &c, T sint
     g[0] = g00; g[1] = g03; g[2] = g11; g[3] = g22; g[4] = g33;
 }
 
+// mn_uu_from for the analytic formulations: the same quantities with five reciprocals
+// (arcp, not correctly rounded) in place of eight divisions.
+template <class T>
+FM_HD_INLINE void mn_uu_from_an(const MetricParams &M, const MNCommon<T> &c, T sint, T *g)
+{
+    const T inv_f = arcp(c.f);
+    const T beta_d = c.omega * c.f;
+    const T gamma_dd = -c.f * c.omega * c.omega + c.delta * sint * sint * inv_f;
+    const T inv_g = arcp(gamma_dd);
+    const T beta_u = beta_d * inv_g;
+    const T alpha_sq = beta_u * beta_d + c.f;
+    const T inv_a = arcp(alpha_sq);
+    T g00 = -inv_a;
+    T g22 = c.f * arcp(c.exp_2gamma * c.rho_sq);
+    T g11 = c.delta * g22;
+    T g33 = inv_g - beta_u * beta_u * inv_a;
+    T g03 = beta_u * inv_a;
+    if (M.rlog)
+        g11 *= arcp(c.r * c.r);
+    g[0] = g00; g[1] = g03; g[2] = g11; g[3] = g22; g[4] = g33;
+}
+
 template <class T>
 FM_HD_INLINE void mn_uu(const MetricParams &M, T u, T sint, T cost, T *g)
 {
@@ -699,7 +721,7 @@ FM_HD_INLINE void rhs_an_sc(const RunConst &R, const T *p, const T *v, S s, S c,
         mn_common_an(R.M, U, ST, CT, cd);
         mn_dd_from(R.M, cd, ST, g);
         const MNCommon<S> cv{cd.r.v, cd.sint.v, cd.f.v, cd.exp_2gamma.v, cd.omega.v, cd.rho_sq.v, cd.delta.v};
-        mn_uu_from(R.M, cv, s, uu);
+        mn_uu_from_an(R.M, cv, s, uu);
     } else {
         D2<S> gu[5];
         metric_dd<METRIC, D2<S>>(R.M, U, ST, CT, g);

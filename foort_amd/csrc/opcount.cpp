@@ -38,6 +38,7 @@ inline void ssincos(double x, Cnt *s, Cnt *c) { cnt::trans += 2; s->v = fm_sin(x
 // exp is transcendental; the integer powers cost what their double-double
 // evaluation costs (an fma counted as 2)
 inline Cnt srcp(Cnt x) { ++cnt::div; return Cnt(1.0 / x.v); }
+inline Cnt arcp(Cnt x) { ++cnt::div; return Cnt(1.0 / x.v); }
 inline Cnt fexp(Cnt x) { ++cnt::trans; return Cnt(fm_exp(x.v)); }
 inline Cnt fsqrt(Cnt x) { ++cnt::sqrt_; return Cnt(__builtin_sqrt(x.v)); }
 inline Cnt fpow2(Cnt x) { ++cnt::mul; return Cnt(x.v * x.v); }
