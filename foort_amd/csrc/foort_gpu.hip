@@ -872,17 +872,21 @@ int configure(foort_gpu_ctx *ctx, const foort_run_desc &d)
     std::string err;
     if (build_run(d, R, kernels, err) != FOORT_OK)
         return fail(ctx, FOORT_EINVAL, err);
+    if (ctx->d_rc && std::memcmp(&ctx->rc, &R, sizeof R) != 0) {
+        // kernels of earlier batches may still read the device copy (on any stream)
+        HIP_TRY(ctx, hipDeviceSynchronize());
+        if (hipMemcpy(ctx->d_rc, &R, sizeof R, hipMemcpyHostToDevice) != hipSuccess) {
+            // leave the copy as the previous description has it
+            (void)hipMemcpy(ctx->d_rc, &ctx->rc, sizeof R, hipMemcpyHostToDevice);
+            return fail(ctx, FOORT_EDEVICE, "writing the device copy of the run constants failed");
+        }
+    }
     if (ctx->kernels.persistent != kernels.persistent && ctx->d_slots) {
         // the lane count follows the persistent kernel's occupancy: size the slots again
         (void)hipFree(ctx->d_slots);
         ctx->d_slots = nullptr;
     }
     drop_graph(ctx);
-    if (ctx->d_rc && std::memcmp(&ctx->rc, &R, sizeof R) != 0) {
-        // kernels of earlier batches may still read the device copy (on any stream)
-        HIP_TRY(ctx, hipDeviceSynchronize());
-        HIP_TRY(ctx, hipMemcpy(ctx->d_rc, &R, sizeof R, hipMemcpyHostToDevice));
-    }
     ctx->desc = d;
     ctx->rc = R;
     ctx->kernels = kernels;

@@ -197,8 +197,10 @@ int foort_gpu_open(const foort_run_desc *desc, foort_gpu_ctx **ctx);
 
 /* Give a context another run description (no counterpart in the reference): the
  * constants and kernels change, its device buffers, stream and lane state stay allocated,
- * so a server can move one context from request to request.  On failure (FOORT_EINVAL) the
- * context keeps its previous description. */
+ * so a server can move one context from request to request.  When the run constants change,
+ * it first waits for the device's work (hipDeviceSynchronize), since batches in flight may
+ * still read the context's device copy of them.  On failure (FOORT_EINVAL, or FOORT_EDEVICE
+ * if that copy cannot be written) the context keeps its previous description. */
 int foort_gpu_configure(foort_gpu_ctx *ctx, const foort_run_desc *desc);
 void foort_gpu_close(foort_gpu_ctx *ctx);
 
