@@ -16,6 +16,7 @@
 #include "host.hpp"
 
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <deque>
 #include <map>
@@ -82,6 +83,7 @@ struct Req {
     uint32_t vd = 0;
     bool finished = false;         // the mesh is done (or the request failed)
     uint32_t slot = UINT32_MAX;
+    std::vector<uint64_t> input_px; // InputCertainPixelsMesh: the request's (row, column) list, copied
     std::vector<uint64_t> idx;     // the batch to push: (row, column) pairs
     uint64_t n = 0, first = 0;     // its pixels and, once pushed, its ring position
     uint64_t pushed = 0;           // pixels pushed for the slot since it was added
@@ -404,9 +406,20 @@ int foort_server_submit(foort_server *srv, const foort_request *req)
         set_error("NULL argument");
         return FOORT_EINVAL;
     }
-    if (req->mesh.type == FOORT_MESH_INPUTCERTAINPIXELS) {
-        set_error("the server takes grid and subdivision meshes");
-        return FOORT_EINVAL;
+    const bool listed = req->mesh.type == FOORT_MESH_INPUTCERTAINPIXELS;
+    if (listed) {
+        // the list is checked here as the console dialogue checks each entry (FOORT/Mesh.cpp:81-137)
+        const uint64_t side = static_cast<uint64_t>(std::sqrt(static_cast<double>(req->mesh.total_pixels)));
+        if (req->mesh.pixel_count && !req->mesh.pixels) {
+            set_error("InputCertainPixelsMesh: NULL pixel list");
+            return FOORT_EINVAL;
+        }
+        for (uint64_t i = 0; i < 2 * req->mesh.pixel_count; ++i)
+            if (req->mesh.pixels[i] >= side) {
+                set_error("InputCertainPixelsMesh: pixel " + std::to_string(i / 2) + " lies outside the " +
+                          std::to_string(side) + " x " + std::to_string(side) + " screen");
+                return FOORT_EINVAL;
+            }
     }
     if (req->desc.diag_flags & FOORT_DIAGFLAG_GEODESICPOSITION) {
         set_error("the server does not run GeodesicPosition");
@@ -414,8 +427,11 @@ int foort_server_submit(foort_server *srv, const foort_request *req)
     }
     auto *r = new Req();
     r->req = *req;
-    r->req.mesh.pixels = nullptr; // a request's mesh is adaptive or a grid; pixel lists are not kept
-    r->req.mesh.pixel_count = 0;
+    if (listed)
+        r->input_px.assign(req->mesh.pixels, req->mesh.pixels + 2 * req->mesh.pixel_count);
+    // the request keeps its own copy of a pixel list (the caller's may go away)
+    r->req.mesh.pixels = listed ? r->input_px.data() : nullptr;
+    r->req.mesh.pixel_count = listed ? req->mesh.pixel_count : 0;
     if (req->output_prefix && *req->output_prefix) {
         r->out_prefix = req->output_prefix;
         r->out_ext = req->output_extension ? req->output_extension : "";

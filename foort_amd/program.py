@@ -190,10 +190,18 @@ class Mesh:
 
 
 def mesh_desc(type, value_diag, total_pixels=0, initial_pixels=0, max_pixels=0, iteration_pixels=0,
-              max_subdivide=1, initial_subdivision_to_final=False):
-    return MeshDesc(type=type, value_diag=value_diag, total_pixels=total_pixels, initial_pixels=initial_pixels,
-                    max_pixels=max_pixels, iteration_pixels=iteration_pixels, max_subdivide=max_subdivide,
-                    initial_subdivision_to_final=int(initial_subdivision_to_final))
+              max_subdivide=1, initial_subdivision_to_final=False, pixels=None):
+    """A foort_mesh_desc; `pixels` ((row, column) pairs) is InputCertainPixelsMesh's list, kept
+    alive with the descriptor."""
+    m = MeshDesc(type=type, value_diag=value_diag, total_pixels=total_pixels, initial_pixels=initial_pixels,
+                 max_pixels=max_pixels, iteration_pixels=iteration_pixels, max_subdivide=max_subdivide,
+                 initial_subdivision_to_final=int(initial_subdivision_to_final))
+    if pixels is not None:
+        px = np.ascontiguousarray(pixels, dtype=np.uint64).reshape(-1, 2)
+        m._pixels = px
+        m.pixels = px.ctypes.data
+        m.pixel_count = px.shape[0]
+    return m
 
 
 class Config:

@@ -156,7 +156,8 @@ int foort_program_run(const char *cfg_path, const foort_program_options *opts);
  * and their launches run one after the other. */
 typedef struct foort_request {
     foort_run_desc desc;
-    foort_mesh_desc mesh;         /* pixel lists (InputCertainPixelsMesh) are not accepted */
+    foort_mesh_desc mesh;         /* any mesh; an InputCertainPixelsMesh list is copied at submission
+                                     and every (row, column) must lie on the screen */
     uint64_t tag;                 /* the caller's identifier, returned with the result */
     /* NULL: no files.  Else the request writes the reference's output files, as the program
      * does for a configuration with Output.FilePrefix = output_prefix, FileExtension =

@@ -143,3 +143,42 @@ def test_server_mixed_requests_close_to_oracle(tmp_path):
         mismatched += int(mism.sum())
     print(f"[server-oracle] all requests: {mismatched} of {total} common pixels differ", flush=True)
     assert mismatched <= 0.02 * total
+
+
+def test_server_input_pixel_requests_match_oracle(tmp_path):
+    """InputCertainPixelsMesh requests (FOORT/Mesh.cpp:81-185: a pixel list, one iteration),
+    finite differences in binary64: each file holds the list's pixels in its order, with the
+    oracle's values (ClosestRadius printed as the reference prints it, std::to_string).  A list
+    with a pixel off the screen is refused at submission."""
+    from tests.cases import c0, sample_pixels
+    tmp = str(tmp_path)
+    side = 64
+    reqs = []
+    for k, (a, inc) in enumerate([(0.5, 80.0), (0.9, 30.0), (0.99, 85.0)]):
+        desc = c0(params=(a,), inclination_deg=inc, rlog=True, max_steps=20000)
+        px = sample_pixels(side, 40, seed=k)
+        mesh = P.mesh_desc(P.MESH_INPUTCERTAINPIXELS, A.DIAGFLAG_FOURCOLORSCREEN, total_pixels=side * side,
+                           pixels=px)
+        reqs.append((k, desc, mesh, px))
+    with P.Server(workers=2, hbm_budget_bytes=64 << 20, launch_us=500) as srv:
+        for k, desc, mesh, px in reqs:
+            srv.submit(desc, mesh, k, output_prefix=os.path.join(tmp, f"list_{k}"), output_extension="dat")
+        bad = P.mesh_desc(P.MESH_INPUTCERTAINPIXELS, A.DIAGFLAG_FOURCOLORSCREEN, total_pixels=side * side,
+                          pixels=[[3, 4], [side, 0]])
+        with pytest.raises(P.HostError):
+            srv.submit(reqs[0][1], bad, 99)
+        srv.drain()
+        res = {r.tag: r for r in srv.results(cap=8)}
+    assert sorted(res) == [k for k, _, _, _ in reqs]
+    for k, desc, mesh, px in reqs:
+        assert res[k].status == 0 and res[k].pixels == len(px) and res[k].iterations == 1, k
+        o = O.trace("fm", desc, side, screen_index=px)
+        want = {"FourColorScreen": [str(int(v)) for v in o["quadrant"]],
+                "EquatPasses": [str(int(v)) for v in o["equat_passes"]],
+                "ClosestRadius": ["%f" % v for v in o["closest_radius"]]}
+        for name in NAMES:
+            with open(os.path.join(tmp, f"list_{k}_{name}.dat")) as f:
+                rows = [line.split() for line in f.read().splitlines()[1:]]
+            assert [(int(r), int(c)) for r, c, _ in rows] == [tuple(map(int, p)) for p in px], (k, name)
+            assert [v for _, _, v in rows] == want[name], (k, name)
+    print(f"[server-oracle] {len(reqs)} pixel-list requests: files equal the oracle's values", flush=True)
