@@ -116,8 +116,8 @@ std::string diag_description(uint32_t flag, const RunConfig &rc)
     switch (flag) {
     case FOORT_DIAGFLAG_FOURCOLORSCREEN: return "Four-color screen";
     case FOORT_DIAGFLAG_GEODESICPOSITION:
-        return "Geodesic position (output " + std::to_string(rc.gp_output_steps) + " steps, updates every " +
-               std::to_string(static_cast<unsigned long>(rc.gp_update.update_n_steps)) + " steps)";
+        return "Geodesic position (output " + std::to_string(rc.desc.gp_output_steps) + " steps, updates every " +
+               std::to_string(static_cast<unsigned long>(rc.desc.gp_update.update_n_steps)) + " steps)";
     case FOORT_DIAGFLAG_EQUATORIALPASSES:
         return "Equatorial passes (threshold = " + to_str(rc.desc.ep_threshold) + ")";
     case FOORT_DIAGFLAG_CLOSESTRADIUS: return "Closest radius";
@@ -341,8 +341,6 @@ void init_diagnostics(const Setting &root, RunConfig &rc)
             read_update(*g, n, s, f);
             unsigned long steps = 0;
             g->lookup_counter("OutputSteps", steps);
-            rc.gp_output_steps = steps;
-            rc.gp_update = {n, s, f};
             rc.desc.gp_output_steps = steps;
             rc.desc.gp_update = {n, s, f};
             use_for_mesh(g, FOORT_DIAGFLAG_GEODESICPOSITION);

@@ -67,40 +67,9 @@ struct BatchResults {
         default: return closest[i];
         }
     }
-    // getFullDataStr of FourColorScreen, EquatorialPasses (std::to_string of the int) and
-    // ClosestRadius (std::to_string of the double), FOORT/Diagnostics.cpp:202-206, :382-386, :448-452
     std::string text(uint32_t diag, uint64_t i) const
     {
-        switch (diag) {
-        case FOORT_DIAGFLAG_FOURCOLORSCREEN: return std::to_string(quadrant[i]);
-        case FOORT_DIAGFLAG_EQUATORIALPASSES: return std::to_string(passes[i]);
-        default: return std::to_string(closest[i]);
-        }
-    }
-};
-
-// The kept positions of a batch (GeodesicPosition): pixel i has points offsets[i] .. offsets[i + 1].
-struct Positions {
-    std::vector<uint64_t> offsets;
-    std::vector<double> points;
-    // GeodesicPositionDiagnostic::getFullDataStr, FOORT/Diagnostics.cpp:286-301:
-    // "<points> ;; " and then "x " for every coordinate of every point
-    std::string text(uint64_t i) const
-    {
-        const uint64_t a = offsets[i], b = offsets[i + 1];
-        std::string s = std::to_string(static_cast<unsigned long long>(b - a)) + " ;; ";
-        for (uint64_t k = 4 * a; k < 4 * b; ++k)
-            s += std::to_string(points[k]) + " ";
-        return s;
-    }
-    // getFinalDataVal (:303-308): (theta, phi) of the last kept point.  The reference reads
-    // the last element of an empty list when nothing was saved; here that gives (0, 0).
-    MeshValue last_angles(uint64_t i) const
-    {
-        const uint64_t b = offsets[i + 1];
-        if (b == offsets[i])
-            return MeshValue{};
-        return MeshValue{points[4 * (b - 1) + 2], points[4 * (b - 1) + 3]};
+        return diag_value_text(diag, quadrant[i], passes[i], closest[i]);
     }
 };
 

@@ -41,9 +41,6 @@ struct RunConfig {
     foort_mesh_desc mesh{};
     std::vector<uint64_t> input_pixels; // InputCertainPixelsMesh list (read from stdin)
     std::string integrator_name = "RK4"; // Integrators::IntegratorDescription
-    // GeodesicPosition options (read for the description; the GPU path does not run it)
-    unsigned long gp_output_steps = 0;
-    foort_update_frequency gp_update{1, 0, 1};
     // output handler (Config::GetOutputHandler)
     bool to_files = false;
     std::string prefix, timestamp, extension;
@@ -103,6 +100,42 @@ protected:
 };
 
 std::unique_ptr<Mesh> make_mesh(const foort_mesh_desc &d);
+
+// getFullDataStr of FourColorScreen, EquatorialPasses (std::to_string of the int) and
+// ClosestRadius (std::to_string of the double), FOORT/Diagnostics.cpp:202-206, :382-386, :448-452
+inline std::string diag_value_text(uint32_t diag, int32_t quadrant, int32_t passes, double closest)
+{
+    switch (diag) {
+    case FOORT_DIAGFLAG_FOURCOLORSCREEN: return std::to_string(quadrant);
+    case FOORT_DIAGFLAG_EQUATORIALPASSES: return std::to_string(passes);
+    default: return std::to_string(closest);
+    }
+}
+
+// The kept positions of a batch (GeodesicPosition): pixel i has points offsets[i] .. offsets[i + 1].
+struct Positions {
+    std::vector<uint64_t> offsets;
+    std::vector<double> points;
+    // GeodesicPositionDiagnostic::getFullDataStr, FOORT/Diagnostics.cpp:286-301:
+    // "<points> ;; " and then "x " for every coordinate of every point
+    std::string text(uint64_t i) const
+    {
+        const uint64_t a = offsets[i], b = offsets[i + 1];
+        std::string s = std::to_string(static_cast<unsigned long long>(b - a)) + " ;; ";
+        for (uint64_t k = 4 * a; k < 4 * b; ++k)
+            s += std::to_string(points[k]) + " ";
+        return s;
+    }
+    // getFinalDataVal (:303-308): (theta, phi) of the last kept point.  The reference reads
+    // the last element of an empty list when nothing was saved; here that gives (0, 0).
+    MeshValue last_angles(uint64_t i) const
+    {
+        const uint64_t b = offsets[i + 1];
+        if (b == offsets[i])
+            return MeshValue{};
+        return MeshValue{points[4 * (b - 1) + 2], points[4 * (b - 1) + 3]};
+    }
+};
 
 // ---- GeodesicOutputHandler: FOORT/InputOutput.h:62-133 ----
 class OutputHandler {

@@ -83,6 +83,7 @@ struct Req {
     uint32_t vd = 0;
     bool finished = false;         // the mesh is done (or the request failed)
     uint32_t slot = UINT32_MAX;
+    bool direct = false;           // GeodesicPosition: run whole by a pool thread on a context of its own
     std::vector<uint64_t> input_px; // InputCertainPixelsMesh: the request's (row, column) list, copied
     std::vector<uint64_t> idx;     // the batch to push: (row, column) pairs
     uint64_t n = 0, first = 0;     // its pixels and, once pushed, its ring position
@@ -221,6 +222,86 @@ struct foort_server {
             post_event(r);
     }
 
+    // A request with GeodesicPosition, whose saved points the queue kernel does not keep: the
+    // pool thread runs it whole, as the program does (FOORT/Main.cpp:313-419), on a context of
+    // its own -- each iteration one foort_gpu_trace_positions batch, its values to the mesh and
+    // its texts to the output files.
+    void run_direct(Req *r)
+    {
+        r->res.start_s = now();
+        foort_gpu_ctx *ctx = nullptr;
+        int rc = foort_gpu_open(&r->req.desc, &ctx);
+        if (rc) {
+            set_error(std::string("render server: ") + foort_gpu_last_error(nullptr));
+            r->res.status = rc;
+            complete(r);
+            return;
+        }
+        try {
+            r->mesh = make_mesh(r->req.mesh);
+            r->order = diagnostic_order(r->req.desc.diag_flags, r->req.desc.value_diag);
+            r->vd = r->order.front();
+            if (!r->out_prefix.empty())
+                r->out = make_output(r->req.desc, r->req.mesh, *r->mesh, r->out_prefix, r->out_ext);
+            const bool grid = r->req.mesh.type == FOORT_MESH_SIMPLESQUARE;
+            while (!r->mesh->IsFinished() && !rc) {
+                const uint64_t n = r->mesh->getCurNrGeodesics();
+                if (r->out)
+                    r->out->PrepareForOutput(n);
+                if (n) {
+                    r->idx.resize(2 * n);
+                    r->mesh->screenIndices(r->idx.data());
+                    r->term.assign(n, 0), r->quad.assign(n, 0), r->passes.assign(n, 0), r->closest.assign(n, 0);
+                    foort_geodesic_results view{r->term.data(), r->quad.data(), r->passes.data(), r->closest.data(),
+                                                nullptr,        nullptr,        nullptr,          nullptr};
+                    foort_positions pos{};
+                    foort_stats st{};
+                    rc = foort_gpu_trace_positions(ctx, r->mesh->rowColumnSize(), n, grid ? nullptr : r->idx.data(),
+                                                   &view, &pos, &st);
+                    if (rc) {
+                        set_error(std::string("render server: ") + foort_gpu_last_error(ctx));
+                        break;
+                    }
+                    Positions p;
+                    p.offsets.assign(pos.offsets, pos.offsets + n + 1);
+                    p.points.assign(pos.points, pos.points + 4 * pos.total);
+                    foort_positions_free(&pos);
+                    r->res.steps += st.steps;
+                    r->res.kernel_ms += st.device_ms;
+                    r->res.pixels += n;
+                    r->res.footprint_bytes = std::max(r->res.footprint_bytes, n * kBytesPerPixel + 32 * pos.total);
+                    for (uint64_t i = 0; i < n; ++i) {
+                        r->mesh->GeodesicFinished(i, r->vd == FOORT_DIAGFLAG_GEODESICPOSITION
+                                                         ? p.last_angles(i)
+                                                         : MeshValue{r->vd == FOORT_DIAGFLAG_FOURCOLORSCREEN ? r->quad[i]
+                                                                     : r->vd == FOORT_DIAGFLAG_EQUATORIALPASSES
+                                                                         ? r->passes[i]
+                                                                         : r->closest[i],
+                                                                     0});
+                        if (r->out) {
+                            std::vector<std::string> texts;
+                            for (uint32_t d : r->order)
+                                texts.push_back(d == FOORT_DIAGFLAG_GEODESICPOSITION
+                                                    ? p.text(i)
+                                                    : diag_value_text(d, r->quad[i], r->passes[i], r->closest[i]));
+                            r->out->NewGeodesicOutput(i, r->idx[2 * i], r->idx[2 * i + 1], std::move(texts));
+                        }
+                    }
+                }
+                r->mesh->EndCurrentLoop();
+                ++r->res.iterations;
+            }
+            if (!rc && r->out)
+                r->out->OutputFinished();
+        } catch (const std::exception &e) {
+            set_error(e.what());
+            rc = FOORT_EINVAL;
+        }
+        foort_gpu_close(ctx);
+        r->res.status = rc;
+        complete(r);
+    }
+
     void pool_thread()
     {
         for (;;) {
@@ -233,7 +314,10 @@ struct foort_server {
                 t = tasks.front();
                 tasks.pop_front();
             }
-            mesh_step(t.first, t.second);
+            if (t.first->direct)
+                run_direct(t.first);
+            else
+                mesh_step(t.first, t.second);
         }
     }
 
@@ -421,12 +505,9 @@ int foort_server_submit(foort_server *srv, const foort_request *req)
                 return FOORT_EINVAL;
             }
     }
-    if (req->desc.diag_flags & FOORT_DIAGFLAG_GEODESICPOSITION) {
-        set_error("the server does not run GeodesicPosition");
-        return FOORT_EINVAL;
-    }
     auto *r = new Req();
     r->req = *req;
+    r->direct = (req->desc.diag_flags & FOORT_DIAGFLAG_GEODESICPOSITION) != 0;
     if (listed)
         r->input_px.assign(req->mesh.pixels, req->mesh.pixels + 2 * req->mesh.pixel_count);
     // the request keeps its own copy of a pixel list (the caller's may go away)
@@ -446,6 +527,12 @@ int foort_server_submit(foort_server *srv, const foort_request *req)
             return FOORT_EINVAL;
         }
         r->res.submit_s = srv->now();
+        if (r->direct) {
+            ++srv->submitted;
+            srv->tasks.emplace_back(r, true);
+            srv->cv_task.notify_one();
+            return FOORT_OK;
+        }
         const std::string key = group_key(req->desc);
         std::unique_ptr<Group> &g = srv->groups[key];
         if (!g) {

@@ -141,8 +141,8 @@ void foort_program_options_init(foort_program_options *opts);
 int foort_program_run(const char *cfg_path, const foort_program_options *opts);
 
 /* ---- render server (no counterpart in the reference) ----
- * Requests -- each a run description and its mesh (a grid or an adaptive subdivision
- * mesh) -- run concurrently on one GPU by continuous batching.  Requests whose descriptions
+ * Requests -- each a run description and its mesh (a grid, an adaptive subdivision mesh or
+ * a pixel list) -- run concurrently on one GPU by continuous batching.  Requests whose descriptions
  * differ only in metric parameters, camera and screen share a group: one device request
  * queue (foort_gpu_queue_*, see foort_gpu.h) whose persistent kernel integrates the pixels
  * of all of them together, in time slices of launch_us.  Each request keeps one iteration
@@ -153,7 +153,9 @@ int foort_program_run(const char *cfg_path, const foort_program_options *opts);
  * once; later ones wait for a slot.  The groups' queues run concurrently only on distinct
  * hardware queues: give the process at least one per group and one more (HIP reads
  * GPU_MAX_HW_QUEUES, default 4, when it initialises); with too few, two groups can share one
- * and their launches run one after the other. */
+ * and their launches run one after the other.  A request with GeodesicPosition (whose saved
+ * points the queue kernel does not keep) runs whole on one of the worker threads, with a
+ * context of its own and foort_gpu_trace_positions for each iteration, as the program does. */
 typedef struct foort_request {
     foort_run_desc desc;
     foort_mesh_desc mesh;         /* any mesh; an InputCertainPixelsMesh list is copied at submission

@@ -182,3 +182,53 @@ def test_server_input_pixel_requests_match_oracle(tmp_path):
             assert [(int(r), int(c)) for r, c, _ in rows] == [tuple(map(int, p)) for p in px], (k, name)
             assert [v for _, _, v in rows] == want[name], (k, name)
     print(f"[server-oracle] {len(reqs)} pixel-list requests: files equal the oracle's values", flush=True)
+
+
+GP_TEXT = """
+Metric = {{ Name = "Kerr"; a = 0.8; RLogScale = true; }};
+Diagnostics = {{ GeodesicPosition = {{ On = true; UpdateFrequency = 20; OutputSteps = 12; UseForMesh = {gp_mesh}; }};
+                EquatorialPasses = {{ On = true; UseForMesh = {ep_mesh}; }}; }};
+Terminations = {{ Horizon = {{ On = true; }}; BoundarySphere = {{ On = true; SphereRadius = 1000.0; }};
+                 TimeOut = {{ On = true; MaxSteps = 3000; }}; }};
+ViewScreen = {{ Position = {{ r = 1000.0; theta = 1.2; }}; ScreenSize = {{ x = 20.0; y = 20.0; }};
+               Mesh = {{ Type = "{mesh}"; InitialPixels = 100; MaxPixels = 500; IterationPixels = 30;
+                        MaxSubdivide = 3; TotalPixels = 400; }}; }};
+Integrator = {{ Type = "RK4"; }};
+"""
+
+
+def _positions_tracer(desc, side, si):
+    if desc.diag_flags & A.DIAGFLAG_GEODESICPOSITION:
+        return O.trace_positions("fm", desc, side, screen_index=si)
+    return O.trace("fm", desc, side, screen_index=si)
+
+
+def test_server_geodesic_position_requests_equal_oracle_files(tmp_path):
+    """Requests with GeodesicPosition (FOORT/Diagnostics.cpp:239-335) -- as the mesh value on an
+    adaptive mesh and beside EquatorialPasses on a grid -- run whole on a worker thread next to
+    a queue request; their files equal those of the oracle-driven program byte for byte."""
+    tmp = str(tmp_path)
+    reqs = []
+    for k, (mesh, gp_mesh) in enumerate((("SquareSubdivisionMeshV2", "true"), ("SimpleSquareMesh", "false"))):
+        c = P.Config(text=GP_TEXT.format(mesh=mesh, gp_mesh=gp_mesh, ep_mesh="false" if gp_mesh == "true" else "true"))
+        reqs.append((k, c.run_desc(), c.mesh_desc(), c.diag_names()))
+    queued = _requests()[0]
+    with P.Server(workers=3, hbm_budget_bytes=64 << 20, launch_us=500) as srv:
+        for k, desc, mesh, _ in reqs:
+            srv.submit(desc, mesh, k, output_prefix=os.path.join(tmp, f"gp_{k}"), output_extension="dat")
+        srv.submit(queued[1][1], queued[1][2], 7)
+        srv.drain()
+        res = {r.tag: r for r in srv.results(cap=8)}
+    assert sorted(res) == [0, 1, 7] and all(r.status == 0 for r in res.values())
+    for k, desc, mesh, names in reqs:
+        assert "GeodesicPosition" in names
+        cfg = os.path.join(tmp, f"oracle_gp_{k}.cfg")
+        with open(cfg, "w") as f:
+            f.write(P.config_text(desc, mesh, prefix=os.path.join(tmp, f"oracle_gp_{k}")))
+        P.run(cfg, screen_output_level=0, tracer=_positions_tracer)
+        for name in names:
+            with open(os.path.join(tmp, f"gp_{k}_{name}.dat"), "rb") as a, \
+                    open(os.path.join(tmp, f"oracle_gp_{k}_{name}.dat"), "rb") as b:
+                assert a.read() == b.read(), (k, name)
+        assert res[k].iterations >= 1 and res[k].pixels == sum(1 for _ in open(os.path.join(tmp, f"gp_{k}_{names[0]}.dat"))) - 1
+    print(f"[server-oracle] {len(reqs)} GeodesicPosition requests: files byte for byte", flush=True)

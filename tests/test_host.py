@@ -641,9 +641,9 @@ def test_program_writes_value_texts_like_to_string(tmp_path):
 @pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is present: the requests would run")
 def test_server_without_gpu_returns_failed_requests():
     """Failure handling on a machine without a GPU: every request comes back with an error
-    status (its group's device queue cannot be opened), drain and shutdown do not hang, and a
-    request for GeodesicPosition or with a pixel list reaching off the screen is refused at
-    submission."""
+    status (its group's device queue, or a GeodesicPosition request's own context, cannot be
+    opened), drain and shutdown do not hang, and a request with a pixel list reaching off the
+    screen is refused at submission."""
     reqs = W.c4_requests(seed=5, n=6, rate=1000.0)
     with P.Server(workers=2, hbm_budget_bytes=16 << 20) as srv:
         for k, (_, desc, mesh, _) in enumerate(reqs):
@@ -654,8 +654,10 @@ def test_server_without_gpu_returns_failed_requests():
         assert all(r.status != 0 for r in res)
         d = reqs[0][1]
         d.diag_flags |= A.DIAGFLAG_GEODESICPOSITION
-        with pytest.raises(P.HostError):
-            srv.submit(d, reqs[0][2], 99)
+        srv.submit(d, reqs[0][2], 99)
+        srv.drain()
+        res = srv.results()
+        assert [r.tag for r in res] == [99] and res[0].status != 0
         m = P.mesh_desc(P.MESH_INPUTCERTAINPIXELS, A.DIAGFLAG_FOURCOLORSCREEN, total_pixels=100,
                         pixels=[[0, 0], [9, 10]])
         with pytest.raises(P.HostError):
