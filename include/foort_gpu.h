@@ -234,8 +234,9 @@ int foort_gpu_trace_device(foort_gpu_ctx *ctx, uint64_t row_column_size, uint64_
  * counters and the persistent schedule's lane slots live in a static device array of 18 MiB
  * per GPU (held until the process ends) instead of the context's allocations.  The persistent
  * schedule runs at most 131072 resident lanes there (the occupancy's lane count is capped at
- * that; a lanes_per_cu that asks for more fails with FOORT_EINVAL).  Call it with the
- * context's device current.  Every call on one GPU shares that array and leaves its contents
+ * that; a lanes_per_cu that asks for more fails with FOORT_EINVAL).  The persistent kernels'
+ * refills read the context's device copy of its run constants, allocated by foort_gpu_open.
+ * Call it with the context's device current.  Every call on one GPU shares that array and leaves its contents
  * behind (each call clears what must start at zero): calls that overlap on one GPU, from
  * other threads or on other streams, race and must not be made. */
 int foort_gpu_trace_device_noalloc(foort_gpu_ctx *ctx, uint64_t row_column_size, uint64_t first,
