@@ -22,6 +22,10 @@
  *    2^-64.0, Remez exchange in gen_fm_consts.py) evaluated by fused
  *    multiply-adds, the cosine assembled with the exact correction of
  *    1 - x^2/2 (the technique of the fdlibm kernels).
+ *  - fm_sincos_near / fm_sincos3: sin and cos at points within 2^-12 of x
+ *    (the theta stencil of the finite differences) from x's reduced sine and
+ *    cosine carried as double-doubles, by the addition formulas with short
+ *    Taylor polynomials of the offset.
  *  - fm_exp: P. T. P. Tang, "Table-driven implementation of the exponential
  *    function in IEEE floating-point arithmetic", ACM TOMS 15 (1989): x =
  *    (64k + j) ln2/64 + r, exp(x) = 2^k 2^(j/64) exp(r), |r| <= ln2/128,
@@ -122,13 +126,15 @@ FM_FN double fm_exp(double x)
     return (y * fm_pow2i(k1)) * fm_pow2i(k2);
 }
 
-/* sin(x) and cos(x); arguments |x| < 2^30 are reduced with full accuracy */
-FM_FN void fm_sincos(double x, double *s, double *c)
+/* The reduced sine and cosine of x as unevaluated sums: x = n pi/2 + t with n the nearest
+   integer to x 2/pi (returned mod 4), sin t = sh + sl and cos t = ch + cl, each low part the
+   exact rounding residual of its high part.  Arguments |x| < 2^30 are reduced with full
+   accuracy. */
+FM_FN int fm_sincos_red(double x, double *sh, double *sl, double *ch, double *cl)
 {
     if (!(x - x == 0.0)) { /* NaN or infinity */
-        *s = x - x;
-        *c = x - x;
-        return;
+        *sh = *sl = *ch = *cl = x - x;
+        return 0;
     }
     /* x = kd pi/2 + (hi + lo).  kd is the integer nearest x 2/pi.  x - kd P1 is exact in one
        fused multiply-add: for |x| < 2^30, x and kd P1 are both multiples of 2^-53 (|x| < 1)
@@ -148,7 +154,8 @@ FM_FN void fm_sincos(double x, double *s, double *c)
     ps = __builtin_fma(z, ps, fm_k(FM_S7));
     ps = __builtin_fma(z, ps, fm_k(FM_S5));
     ps = __builtin_fma(z, ps, fm_k(FM_S3));
-    double sn = hi + __builtin_fma(hi * z, ps, __builtin_fma(-hz, lo, lo));
+    double ts = __builtin_fma(hi * z, ps, __builtin_fma(-hz, lo, lo));
+    double sn = hi + ts;
     /* cos(hi + lo) = 1 - z/2 + z^2 C(z) - hi lo, with hi^2 = z + zl exactly; 1 - z/2 as
        w + ((1 - w) - z/2) with the exact correction of w = 1 - z/2 (the fdlibm kernels' device) */
     double zl = __builtin_fma(hi, hi, -z);
@@ -159,16 +166,137 @@ FM_FN void fm_sincos(double x, double *s, double *c)
     pc = __builtin_fma(z, pc, fm_k(FM_C4));
     pc = (z * z) * pc;
     double w = 1.0 - hz;
+    double tc = __builtin_fma(-0.5, zl, (1.0 - w) - hz) + __builtin_fma(-hi, lo, pc);
+    double cs = w + tc;
+    /* the residuals by Fast2Sum, exact since |ts| < |hi| (|ts| <= 0.11 |hi| + |lo|, |lo| <=
+       ulp(hi)/2) and |tc| < 0.02 < |w| */
+    *sh = sn;
+    *sl = ts - (sn - hi);
+    *ch = cs;
+    *cl = tc - (cs - w);
+    return ((int)kd) & 3;
+}
+
+/* sin and cos of n pi/2 + t from sn = sin t, cs = cos t: (sn, cs), (cs, -sn), (-sn, -cs),
+   (-cs, sn) for n = 0..3, selected without branches (only exact moves and negations) */
+FM_FN void fm_quadrant(int n, double sn, double cs, double *s, double *c)
+{
+    int odd = n & 1;
+    double a = odd ? cs : sn;
+    double b = odd ? sn : cs;
+    *s = (n & 2) ? -a : a;
+    *c = ((n + 1) & 2) ? -b : b;
+}
+
+/* sin(x) and cos(x); arguments |x| < 2^30 are reduced with full accuracy.  The operations of
+   fm_sincos_red and fm_quadrant, written out (the same values; a NaN or infinity returns
+   before the quadrant selection, which keeps the kernels' code as it was before the reduced
+   form was split out). */
+FM_FN void fm_sincos(double x, double *s, double *c)
+{
+    if (!(x - x == 0.0)) { /* NaN or infinity */
+        *s = x - x;
+        *c = x - x;
+        return;
+    }
+    double kd = __builtin_fma(x, FM_TWO_OVER_PI, FM_SHIFT) - FM_SHIFT;
+    double r = __builtin_fma(-kd, FM_PIO2_1, x);
+    double cc = __builtin_fma(kd, FM_PIO2_2, kd * FM_PIO2_3);
+    double lo;
+    double hi = fm_two_sum(r, -cc, &lo);
+
+    double z = hi * hi;
+    double hz = 0.5 * z;
+    double ps = __builtin_fma(z, FM_S13, fm_k(FM_S11));
+    ps = __builtin_fma(z, ps, fm_k(FM_S9));
+    ps = __builtin_fma(z, ps, fm_k(FM_S7));
+    ps = __builtin_fma(z, ps, fm_k(FM_S5));
+    ps = __builtin_fma(z, ps, fm_k(FM_S3));
+    double sn = hi + __builtin_fma(hi * z, ps, __builtin_fma(-hz, lo, lo));
+    double zl = __builtin_fma(hi, hi, -z);
+    double pc = __builtin_fma(z, FM_C14, fm_k(FM_C12));
+    pc = __builtin_fma(z, pc, fm_k(FM_C10));
+    pc = __builtin_fma(z, pc, fm_k(FM_C8));
+    pc = __builtin_fma(z, pc, fm_k(FM_C6));
+    pc = __builtin_fma(z, pc, fm_k(FM_C4));
+    pc = (z * z) * pc;
+    double w = 1.0 - hz;
     double cs = w + (__builtin_fma(-0.5, zl, (1.0 - w) - hz) + __builtin_fma(-hi, lo, pc));
 
-    /* quadrant n: (s, c) = (sn, cs), (cs, -sn), (-sn, -cs), (-cs, sn) for n = 0..3;
-       selected without branches (only exact moves and negations) */
     int n = ((int)kd) & 3;
     int odd = n & 1;
     double a = odd ? cs : sn;
     double b = odd ? sn : cs;
     *s = (n & 2) ? -a : a;
     *c = ((n + 1) & 2) ? -b : b;
+}
+
+/* Whether fm_sincos_near shifts y from x: |d| <= 2^-12 for d = y - x, and |x| >= 2^-11, so
+   that d is exact (Sterbenz's lemma) */
+FM_FN int fm_near_ok(double x, double y)
+{
+    return __builtin_fabs(y - x) <= 0x1p-12 && __builtin_fabs(x) >= 0x1p-11;
+}
+
+/* sin(t + d) and cos(t + d) from the reduced sine and cosine of t (sh + sl, ch + cl) by the
+   addition formulas, mapped to quadrant n:
+     sin(t + d) = sh + (sl + sh (cos d - 1) + ch sin d),
+     cos(t + d) = ch + (cl + ch (cos d - 1) - sh sin d),
+   the bracket (of the order of d) evaluated by fused multiply-adds and added once, with
+   sin d = d - d^3/6 + d^5/120 and cos d - 1 = -d^2/2 + d^4/24 (truncation below 2^-80
+   relative for |d| <= 2^-12) and the products sl (cos d - 1) and cl sin d dropped (below
+   2^-64 |d| in absolute terms). */
+FM_FN void fm_sincos_shift(int n, double sh, double sl, double ch, double cl, double d, double *s,
+                           double *c)
+{
+    double z = d * d;
+    double sd = __builtin_fma(d * z, __builtin_fma(z, 1.0 / 120.0, -1.0 / 6.0), d);
+    double cm1 = z * __builtin_fma(z, 1.0 / 24.0, -0.5);
+    double sn = sh + __builtin_fma(ch, sd, __builtin_fma(sh, cm1, sl));
+    double cs = ch + __builtin_fma(-sh, sd, __builtin_fma(ch, cm1, cl));
+    fm_quadrant(n, sn, cs, s, c);
+}
+
+/* sin(y) and cos(y) for y near x, from the reduced sine and cosine of x (fm_sincos_red:
+   quadrant n, sh + sl, ch + cl): shifted by d = y - x (fm_sincos_shift) where fm_near_ok,
+   evaluated in full elsewhere.  The result differs from fm_sincos(y) only by the rounding of
+   the bracket and the accuracy of the polynomials behind the double-double (about 1.6% of
+   results differ in the last bit, as accurate: within 0.75 ulp, with an absolute 2^-52 |d|
+   added where the result is close to zero). */
+FM_FN void fm_sincos_near(int n, double sh, double sl, double ch, double cl, double x, double y,
+                          double *s, double *c)
+{
+    if (fm_near_ok(x, y))
+        fm_sincos_shift(n, sh, sl, ch, cl, y - x, s, c);
+    else
+        fm_sincos(y, s, c);
+}
+
+/* sin and cos at x and at two points xp, xm near it (the finite-difference stencil in theta):
+   each of xp and xm as fm_sincos_near gives it, with one reduction and one pair of polynomials
+   for the three where both shift (the full evaluations in one rarely taken branch) */
+FM_FN void fm_sincos3(double x, double xp, double xm, double *s0, double *c0, double *sp, double *cp,
+                      double *sm, double *cm)
+{
+    double sh, sl, ch, cl;
+    int n = fm_sincos_red(x, &sh, &sl, &ch, &cl);
+    fm_quadrant(n, sh, ch, s0, c0);
+    fm_sincos_shift(n, sh, sl, ch, cl, xp - x, sp, cp);
+    fm_sincos_shift(n, sh, sl, ch, cl, xm - x, sm, cm);
+    int okp = fm_near_ok(x, xp), okm = fm_near_ok(x, xm);
+    if (!(okp && okm)) {
+        double s1, c1, s2, c2;
+        fm_sincos(xp, &s1, &c1);
+        fm_sincos(xm, &s2, &c2);
+        if (!okp) {
+            *sp = s1;
+            *cp = c1;
+        }
+        if (!okm) {
+            *sm = s2;
+            *cm = c2;
+        }
+    }
 }
 
 FM_FN double fm_sin(double x)

@@ -184,6 +184,16 @@ __device__ __forceinline__ const RunConst &refill_consts(const RunConst *rc)
     return *rc;
 }
 
+// The lane index at which a resident kernel parks its lane state after the step loop.  The
+// empty asm makes the compiler form the slot addresses again there instead of keeping the
+// 64-bit addresses it formed when the state was loaded alive across the loop: that register
+// pair was one more than the occupancy target leaves and went to scratch memory.
+__device__ __forceinline__ uint32_t park_index(uint32_t l)
+{
+    asm volatile("" : "+v"(l));
+    return l;
+}
+
 __device__ __forceinline__ void add_steps(unsigned long long *total, unsigned long long mine)
 {
 #pragma unroll
@@ -373,9 +383,10 @@ trace_persistent(const RunConst R, uint64_t side, uint64_t first,
                 atomicAdd(S.done, static_cast<unsigned long long>(__popcll(fin)));
         }
     }
-    S.pix[l] = pix;
+    const uint32_t lp = park_index(l);
+    S.pix[lp] = pix;
     if (pix >= 0)
-        slot_save<EVERY>(S, l, g);
+        slot_save<EVERY>(S, lp, g);
 }
 
 // Persistent schedule over a request queue (QueueArgs): as trace_persistent, with
@@ -452,10 +463,11 @@ trace_queue(const RunConst R, const QueueArgs Q, DevResults out, unsigned long l
                 atomicAdd(S.done, static_cast<unsigned long long>(__popcll(fin)));
         }
     }
-    S.pix[l] = pix;
+    const uint32_t lp = park_index(l);
+    S.pix[lp] = pix;
     if (pix >= 0) {
-        slot_save<EVERY>(S, l, g);
-        S.u[11 * S.L + l] = req;
+        slot_save<EVERY>(S, lp, g);
+        S.u[11 * S.L + lp] = req;
     }
 }
 
@@ -560,7 +572,15 @@ __device__ __forceinline__ void rhs_fd_quad(const RunConst &R, const double *p, 
     const double uq = q == 0 ? u + h : (q == 1 ? u - h : u);
     const double tq = q == 2 ? th + h : (q == 3 ? th - h : th);
     double s, c, g[5];
-    fm_sincos(tq, &s, &c);
+    {
+        // as rhs_fd: theta's sin and cos, those of theta +- h shifted from them
+        double sh, sl, ch, cl, s1, c1;
+        const int n = fm_sincos_red(th, &sh, &sl, &ch, &cl);
+        fm_quadrant(n, sh, ch, &s, &c);
+        fm_sincos_near(n, sh, sl, ch, cl, th, tq, &s1, &c1);
+        s = q >= 2 ? s1 : s;
+        c = q >= 2 ? c1 : c;
+    }
     metric_dd<METRIC, double>(R.M, uq, s, c, g);
     double d1[5], d2[5], uu[5];
 #pragma unroll

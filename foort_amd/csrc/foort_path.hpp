@@ -547,9 +547,9 @@ FM_HD_INLINE void rhs_fd(const RunConst &R, const double *p, const double *v, do
 {
     const double u = p[1], th = p[2], h = R.h_fd;
     double s0, c0, sp, cp, sm, cm;
-    fm_sincos(th, &s0, &c0);
-    fm_sincos(th + h, &sp, &cp);
-    fm_sincos(th - h, &sm, &cm);
+    // sin and cos at theta + h and theta - h shifted from theta's (one reduction, one pair of
+    // polynomials: the same functions as the oracle's)
+    fm_sincos3(th, th + h, th - h, &s0, &c0, &sp, &cp, &sm, &cm);
     double gp[5], gm[5], d1[5], d2[5], uu[5];
     metric_dd<METRIC, double>(R.M, u + h, s0, c0, gp);
     metric_dd<METRIC, double>(R.M, u - h, s0, c0, gm);

@@ -56,6 +56,45 @@ static const double o_pi = 3.1415926535; /* FOORT/Geometry.h:44 */
 typedef double T2[DIM][DIM];
 typedef double T3[DIM][DIM][DIM];
 
+/* sin and cos of the theta at which a metric is evaluated.  The C-library build reads them
+ * in the metric itself, as sin(p[2]) and cos(p[2]) written where the reference writes them
+ * (the compiler merges a sin and a cos of one argument into one sincos call, whose last bit can
+ * differ from sin's alone: the same source shape as the reference gives the same calls); the
+ * portable build takes them from the caller, which shifts the theta stencil's values. */
+typedef struct {
+    double s, c;
+} oangle;
+#ifdef ORACLE_FM
+#define ANG_SIN(p, A) ((A)->s)
+#define ANG_COS(p, A) ((A)->c)
+#else
+#define ANG_SIN(p, A) O_SIN((p)[2])
+#define ANG_COS(p, A) O_COS((p)[2])
+#endif
+
+static oangle o_angle(double theta)
+{
+    oangle A;
+    A.s = O_SIN(theta);
+    A.c = O_COS(theta);
+    return A;
+}
+
+/* sin and cos at theta + h and theta - h, the points of the finite differences in theta
+ * (FOORT/Metric.cpp:24-30).  The C library evaluates each point by itself, as the reference;
+ * the portable set shifts them from theta's reduced sine and cosine (fm_sincos3, the GPU
+ * path's function). */
+static void o_angle_stencil(double theta, double tp, double tm, oangle *Ap, oangle *Am)
+{
+#ifdef ORACLE_FM
+    double s0, c0;
+    fm_sincos3(theta, tp, tm, &s0, &c0, &Ap->s, &Ap->c, &Am->s, &Am->c);
+#else
+    *Ap = o_angle(tp);
+    *Am = o_angle(tm);
+#endif
+}
+
 /* ------------------------------------------------------------------------- */
 /* Metrics (FOORT/Metric.cpp)                                                */
 /* ------------------------------------------------------------------------- */
@@ -125,13 +164,12 @@ static void set5(T2 g, double g00, double g03, double g11, double g22, double g3
 }
 
 /* FOORT/Metric.cpp:131-159 */
-static void kerr_dd(const ometric *M, const double *p, T2 g)
+static void kerr_dd(const ometric *M, const double *p, const oangle *A, T2 g)
 {
     double a = M->a;
     double r = M->rlog ? O_EXP(p[1]) : p[1];
-    double theta = p[2];
-    double sint = O_SIN(theta);
-    double cost = O_COS(theta);
+    double sint = ANG_SIN(p, A);
+    double cost = ANG_COS(p, A);
     double sigma = r * r + a * a * cost * cost;
     double delta = r * r + a * a - 2. * r;
     double A_ = (r * r + a * a) * (r * r + a * a) - delta * a * a * sint * sint;
@@ -146,13 +184,12 @@ static void kerr_dd(const ometric *M, const double *p, T2 g)
 }
 
 /* FOORT/Metric.cpp:162-191 */
-static void kerr_uu(const ometric *M, const double *p, T2 g)
+static void kerr_uu(const ometric *M, const double *p, const oangle *A, T2 g)
 {
     double a = M->a;
     double r = M->rlog ? O_EXP(p[1]) : p[1];
-    double theta = p[2];
-    double sint = O_SIN(theta);
-    double cost = O_COS(theta);
+    double sint = ANG_SIN(p, A);
+    double cost = ANG_COS(p, A);
     double sigma = r * r + a * a * cost * cost;
     double delta = r * r + a * a - 2. * r;
     double A_ = (r * r + a * a) * (r * r + a * a) - delta * a * a * sint * sint;
@@ -167,16 +204,16 @@ static void kerr_uu(const ometric *M, const double *p, T2 g)
 }
 
 /* FOORT/Metric.cpp:218-229 */
-static void flat_dd(const ometric *M, const double *p, T2 g)
+static void flat_dd(const ometric *M, const double *p, const oangle *A, T2 g)
 {
     (void)M;
-    set5(g, -1, 0, 1, p[1] * p[1], p[1] * p[1] * O_SIN(p[2]) * O_SIN(p[2]));
+    set5(g, -1, 0, 1, p[1] * p[1], p[1] * p[1] * ANG_SIN(p, A) * ANG_SIN(p, A));
 }
 
-static void flat_uu(const ometric *M, const double *p, T2 g)
+static void flat_uu(const ometric *M, const double *p, const oangle *A, T2 g)
 {
     (void)M;
-    set5(g, -1, 0, 1, 1 / (p[1] * p[1]), 1 / (p[1] * p[1] * O_SIN(p[2]) * O_SIN(p[2])));
+    set5(g, -1, 0, 1, 1 / (p[1] * p[1]), 1 / (p[1] * p[1] * ANG_SIN(p, A) * ANG_SIN(p, A)));
 }
 
 /* shared part of FOORT/Metric.cpp:276-300 and :321-345 */
@@ -184,9 +221,8 @@ static void flat_uu(const ometric *M, const double *p, T2 g)
     double m = M->m, a = M->a, pp = M->p, qq = M->q;                                              \
     double r = M->rlog ? O_EXP(p[1]) : p[1];                                                      \
     double r2 = r * r;                                                                             \
-    double theta = p[2];                                                                           \
-    double sint = O_SIN(theta);                                                                    \
-    double cost = O_COS(theta);                                                                    \
+    double sint = ANG_SIN(p, A);                                                                   \
+    double cost = ANG_COS(p, A);                                                                   \
     double sint2 = sint * sint;                                                                    \
     double cost2 = cost * cost;                                                                    \
     double delta = r2 + a * a - 2. * r * m;                                                        \
@@ -204,7 +240,7 @@ static void flat_uu(const ometric *M, const double *p, T2 g)
                     sqrt((pp * pp - 4. * m * m) * (qq * qq - 4. * m * m)) * a * cost;               \
     double sqH1H2 = sqrt(H1 * H2);
 
-static void rl_dd(const ometric *M, const double *p, T2 g)
+static void rl_dd(const ometric *M, const double *p, const oangle *A, T2 g)
 {
     RL_COMMON
     double g00 = -H3 / sqH1H2;
@@ -217,7 +253,7 @@ static void rl_dd(const ometric *M, const double *p, T2 g)
     set5(g, g00, g03, g11, g22, g33);
 }
 
-static void rl_uu(const ometric *M, const double *p, T2 g)
+static void rl_uu(const ometric *M, const double *p, const oangle *A, T2 g)
 {
     RL_COMMON
     double g00 = ((H3 * H3 * Bp * Bp) / sint2 - H1 * H2 * delta) / (sqH1H2 * H3 * delta);
@@ -234,9 +270,8 @@ static void rl_uu(const ometric *M, const double *p, T2 g)
 #define JOH_COMMON                                                                                 \
     double a = M->a;                                                                              \
     double r = M->rlog ? O_EXP(p[1]) : p[1];                                                      \
-    double theta = p[2];                                                                           \
-    double sint = O_SIN(theta);                                                                    \
-    double cost = O_COS(theta);                                                                    \
+    double sint = ANG_SIN(p, A);                                                                   \
+    double cost = ANG_COS(p, A);                                                                   \
     double A1 = 1. + M->a13 / (r * r * r);                                                         \
     double A2 = 1. + M->a22 / (r * r);                                                             \
     double A5 = 1. + M->a52 / (r * r);                                                             \
@@ -244,7 +279,7 @@ static void rl_uu(const ometric *M, const double *p, T2 g)
     double rho2 = r * r + a * a * cost * cost + eps_f;                                             \
     double delta = r * r + a * a - 2. * r;
 
-static void joh_dd(const ometric *M, const double *p, T2 g)
+static void joh_dd(const ometric *M, const double *p, const oangle *A, T2 g)
 {
     JOH_COMMON
     double g00 = -rho2 * (delta - a * a * A2 * A2 * sint * sint) /
@@ -260,7 +295,7 @@ static void joh_dd(const ometric *M, const double *p, T2 g)
     set5(g, g00, g03, g11, g22, g33);
 }
 
-static void joh_uu(const ometric *M, const double *p, T2 g)
+static void joh_uu(const ometric *M, const double *p, const oangle *A, T2 g)
 {
     JOH_COMMON
     double g00 = (-1. * O_POW2((r * r + a * a) * A1) + a * a * delta * sint * sint) / (delta * rho2);
@@ -277,9 +312,8 @@ static void joh_uu(const ometric *M, const double *p, T2 g)
 #define MN_COMMON                                                                                  \
     double al = M->mn_alpha, kk = M->mn_k, a3 = M->alpha3;                                        \
     double r = M->rlog ? O_EXP(p[1]) : p[1];                                                      \
-    double theta = p[2];                                                                           \
-    double sint = O_SIN(theta);                                                                    \
-    double cost = O_COS(theta);                                                                    \
+    double sint = ANG_SIN(p, A);                                                                   \
+    double cost = ANG_COS(p, A);                                                                   \
     double xx = (r - 1.) / kk;                                                                     \
     double yy = cost;                                                                              \
     double R = sqrt(xx * xx + yy * yy - 1.);                                                       \
@@ -304,7 +338,7 @@ static void joh_uu(const ometric *M, const double *p, T2 g)
     double rho_sq = (r - 1.) * (r - 1.) - kk * kk * cost * cost;                                   \
     double delta = (r - 1.) * (r - 1.) - kk * kk;
 
-static void mn_dd(const ometric *M, const double *p, T2 g)
+static void mn_dd(const ometric *M, const double *p, const oangle *A, T2 g)
 {
     MN_COMMON
     double g00 = -f;
@@ -317,7 +351,7 @@ static void mn_dd(const ometric *M, const double *p, T2 g)
     set5(g, g00, g03, g11, g22, g33);
 }
 
-static void mn_uu(const ometric *M, const double *p, T2 g)
+static void mn_uu(const ometric *M, const double *p, const oangle *A, T2 g)
 {
     MN_COMMON
     double beta_d = omega * f;
@@ -334,25 +368,25 @@ static void mn_uu(const ometric *M, const double *p, T2 g)
     set5(g, g00, g03, g11, g22, g33);
 }
 
-static void o_gdd(const ometric *M, const double *p, T2 g)
+static void o_gdd(const ometric *M, const double *p, const oangle *A, T2 g)
 {
     switch (M->id) {
-    case FOORT_METRIC_KERR: kerr_dd(M, p, g); break;
-    case FOORT_METRIC_FLATSPACE: flat_dd(M, p, g); break;
-    case FOORT_METRIC_RASHEEDLARSEN: rl_dd(M, p, g); break;
-    case FOORT_METRIC_JOHANNSEN: joh_dd(M, p, g); break;
-    default: mn_dd(M, p, g); break;
+    case FOORT_METRIC_KERR: kerr_dd(M, p, A, g); break;
+    case FOORT_METRIC_FLATSPACE: flat_dd(M, p, A, g); break;
+    case FOORT_METRIC_RASHEEDLARSEN: rl_dd(M, p, A, g); break;
+    case FOORT_METRIC_JOHANNSEN: joh_dd(M, p, A, g); break;
+    default: mn_dd(M, p, A, g); break;
     }
 }
 
-static void o_guu(const ometric *M, const double *p, T2 g)
+static void o_guu(const ometric *M, const double *p, const oangle *A, T2 g)
 {
     switch (M->id) {
-    case FOORT_METRIC_KERR: kerr_uu(M, p, g); break;
-    case FOORT_METRIC_FLATSPACE: flat_uu(M, p, g); break;
-    case FOORT_METRIC_RASHEEDLARSEN: rl_uu(M, p, g); break;
-    case FOORT_METRIC_JOHANNSEN: joh_uu(M, p, g); break;
-    default: mn_uu(M, p, g); break;
+    case FOORT_METRIC_KERR: kerr_uu(M, p, A, g); break;
+    case FOORT_METRIC_FLATSPACE: flat_uu(M, p, A, g); break;
+    case FOORT_METRIC_RASHEEDLARSEN: rl_uu(M, p, A, g); break;
+    case FOORT_METRIC_JOHANNSEN: joh_uu(M, p, A, g); break;
+    default: mn_uu(M, p, A, g); break;
     }
 }
 
@@ -362,6 +396,7 @@ static void o_christoffel(const ometric *M, double h, const double *p, T3 G)
 {
     T3 dd;
     memset(dd, 0, sizeof(T3));
+    const oangle A0 = o_angle(p[2]);
     for (int coord = 0; coord < DIM; ++coord) {
         if (coord == 0 || coord == 3)
             continue;
@@ -372,16 +407,19 @@ static void o_christoffel(const ometric *M, double h, const double *p, T3 G)
             pp[i] = p[i] + shift[i];
             pm[i] = p[i] - shift[i];
         }
+        oangle Ap = A0, Am = A0;
+        if (coord == 2)
+            o_angle_stencil(p[2], pp[2], pm[2], &Ap, &Am);
         T2 gp, gm;
-        o_gdd(M, pp, gp);
-        o_gdd(M, pm, gm);
+        o_gdd(M, pp, &Ap, gp);
+        o_gdd(M, pm, &Am, gm);
         double inv = 1 / (2 * h);
         for (int i = 0; i < DIM; ++i)
             for (int j = 0; j < DIM; ++j)
                 dd[coord][i][j] = (gp[i][j] - gm[i][j]) * inv;
     }
     T2 uu;
-    o_guu(M, p, uu);
+    o_guu(M, p, &A0, uu);
     memset(G, 0, sizeof(T3));
     for (int mu = 0; mu < DIM; ++mu)
         for (int nu = 0; nu < DIM; ++nu)
@@ -539,8 +577,9 @@ static void o_vielbein(orun *R)
     memcpy(pos, R->cam, sizeof(pos));
     pos[1] = R->cam_rlog ? log(pos[1]) : pos[1];
     T2 gdd, guu;
-    o_gdd(&R->M, pos, gdd);
-    o_guu(&R->M, pos, guu);
+    const oangle A = o_angle(pos[2]);
+    o_gdd(&R->M, pos, &A, gdd);
+    o_guu(&R->M, pos, &A, guu);
     memcpy(R->cam_gdd, gdd, sizeof(T2));
     double signs[DIM] = {0, 0, 0, 0};
     T2 V;
@@ -968,8 +1007,9 @@ void oracle_metric(const foort_run_desc *d, const double *p, double *gdd16, doub
 {
     ometric M;
     o_metric_init(&M, d);
-    o_gdd(&M, p, (double(*)[DIM])gdd16);
-    o_guu(&M, p, (double(*)[DIM])guu16);
+    const oangle A = o_angle(p[2]);
+    o_gdd(&M, p, &A, (double(*)[DIM])gdd16);
+    o_guu(&M, p, &A, (double(*)[DIM])guu16);
 }
 
 void oracle_christoffel(const foort_run_desc *d, const double *p, double *G64)
@@ -1023,6 +1063,16 @@ void oracle_fm_eval(const double *x, uint64_t n, int which, double *out1, double
         case 3: out1[i] = fm_pow4(x[i]); break;
         default: out1[i] = fm_pow8(x[i]); break;
         }
+    }
+}
+
+/* sin and cos at y[i] shifted from x[i] (fm_sincos_near), for the accuracy tests */
+void oracle_fm_near(const double *x, const double *y, uint64_t n, double *s, double *c)
+{
+    for (uint64_t i = 0; i < n; ++i) {
+        double sh, sl, ch, cl;
+        int q = fm_sincos_red(x[i], &sh, &sl, &ch, &cl);
+        fm_sincos_near(q, sh, sl, ch, cl, x[i], y[i], &s[i], &c[i]);
     }
 }
 #endif

@@ -73,3 +73,53 @@ def test_integer_powers_correctly_rounded(which, n):
     ref = [float(mp.mpf(float(v)) ** n) for v in x]
     assert np.mean(p == np.array(ref)) > 0.999  # double-double: correctly rounded but for rare midpoint cases
     assert _max_ulp(p, x, lambda X: X ** n) < 0.501
+
+
+def _near(x, y):
+    L = O.lib("fm")
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    s, c = np.zeros_like(x), np.zeros_like(x)
+    L.oracle_fm_near.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p]
+    L.oracle_fm_near(x.ctypes.data_as(C.c_void_p), y.ctypes.data_as(C.c_void_p), len(x),
+                     s.ctypes.data_as(C.c_void_p), c.ctypes.data_as(C.c_void_p))
+    return s, c
+
+
+def _ulp(v):
+    return math.ulp(abs(float(v))) if v != 0 else 2.0 ** -1074
+
+
+@pytest.mark.parametrize("h", [1e-7, 3e-6, 2.0 ** -13])
+def test_sincos_near_stencil_accuracy(h):
+    """sin and cos at theta +- h shifted from theta: within 0.75 ulp, plus an absolute 2^-52 |d|
+    where the result is close to zero (the bracket of the addition formula rounds at the size of
+    d); across the quadrant boundaries pi/4, 3 pi/4, the equator and near the poles."""
+    x = np.concatenate([RNG.uniform(2.0 ** -11, math.pi - 2.0 ** -11, 1500),
+                        math.pi / 4 + RNG.uniform(-2 * h, 2 * h, 150), 3 * math.pi / 4 + RNG.uniform(-2 * h, 2 * h, 150),
+                        math.pi / 2 + RNG.uniform(-2 * h, 2 * h, 150), [2.0 ** -11, math.pi - 2.0 ** -11]])
+    for y in (x + h, x - h):
+        d = y - x  # exact
+        s, c = _near(x, y)
+        for fv, f in ((s, mp.sin), (c, mp.cos)):
+            for v, yy, dd in zip(fv, y, d):
+                ref = f(mp.mpf(float(yy)))
+                err = abs(mp.mpf(float(v)) - ref)
+                assert err <= 0.75 * _ulp(float(ref)) + 2.0 ** -52 * abs(dd), (yy, v, float(ref))
+
+
+def test_sincos_near_agrees_with_direct_and_falls_back():
+    """Mostly the same bits as evaluating the point directly; outside its range (|y - x| > 2^-12
+    or |x| < 2^-11) it is the direct evaluation; NaN stays NaN."""
+    x = RNG.uniform(0.1, 3.0, 4000)
+    y = x + 1e-7
+    s, c = _near(x, y)
+    s2, c2 = _eval(0, y)
+    assert np.mean(s == s2) > 0.97 and np.mean(c == c2) > 0.97
+    xf = np.array([1e-4, -1e-4, 0.3, 2.0, 1e-300])
+    yf = np.array([1e-4 + 1e-7, -1e-4 - 1e-7, 0.3 + 0.01, 2.0 - 2.0 ** -11, 1e-7])
+    s, c = _near(xf, yf)
+    s2, c2 = _eval(0, yf)
+    assert (s == s2).all() and (c == c2).all()
+    s, c = _near(np.array([np.nan, 1.0]), np.array([np.nan, np.nan]))
+    assert np.isnan(s).all() and np.isnan(c).all()
