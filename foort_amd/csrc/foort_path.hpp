@@ -593,32 +593,6 @@ FM_HD_INLINE void kj_contract(const S *d1, const S *d2, const S *u, const S *v, 
     out[3] = u[1] * Ft + u[4] * Fp;
 }
 
-// kj_contract for derivatives given as halves (e = d / 2) and the inverse metric's t and
-// phi rows given as w = -2 g^uu (w0 = tt, w1 = tph, w4 = phph); u2 = g^rr, u3 = g^thth.
-// Same equations, without the multiplications by 2 and 1/2.
-template <class S>
-FM_HD_INLINE void kerr_contract(const S *e1, const S *e2, S w0, S w1, S u2, S u3, S w4, const S *v, S *out)
-{
-    const S r_t = e1[0] * v[0] + e1[1] * v[3];
-    const S r_r = e1[2] * v[1];
-    const S r_h = e1[3] * v[2];
-    const S r_p = e1[1] * v[0] + e1[4] * v[3];
-    const S h_t = e2[0] * v[0] + e2[1] * v[3];
-    const S h_r = e2[2] * v[1];
-    const S h_h = e2[3] * v[2];
-    const S h_p = e2[1] * v[0] + e2[4] * v[3];
-    const S Kr = r_t * v[0] + r_r * v[1] + r_h * v[2] + r_p * v[3];
-    const S Kh = h_t * v[0] + h_r * v[1] + h_h * v[2] + h_p * v[3];
-    const S Ft = r_t * v[1] + h_t * v[2]; // -F_t / 2
-    const S Fp = r_p * v[1] + h_p * v[2]; // -F_ph / 2
-    const S Fr = Kr - S(2) * (r_r * v[1] + h_r * v[2]);
-    const S Fh = Kh - S(2) * (r_h * v[1] + h_h * v[2]);
-    out[0] = w0 * Ft + w1 * Fp;
-    out[1] = u2 * Fr;
-    out[2] = u3 * Fh;
-    out[3] = w1 * Ft + w4 * Fp;
-}
-
 FM_HD_INLINE float srcp(float x)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -629,10 +603,22 @@ FM_HD_INLINE float srcp(float x)
 }
 FM_HD_INLINE double srcp(double x) { return fm_div(1.0, x); }
 
-// Kerr (M = 1) in closed form: the ten non-zero first derivatives of the
-// Boyer-Lindquist g_dd of FOORT/Metric.cpp:131-159 and g^uu of :162-191,
-// written with Sigma = r^2 + a^2 cos^2, Delta = r^2 - 2r + a^2,
-// q = Sigma - 2 r^2; then the Euler-Lagrange contraction.  With RLogScale the
+// Kerr (M = 1) in closed form: the first derivatives of the Boyer-Lindquist g_dd of
+// FOORT/Metric.cpp:131-159 and g^uu of :162-191, written with Sigma = r^2 + a^2 cos^2,
+// Delta = r^2 - 2r + a^2, q = Sigma - 2 r^2, in the Euler-Lagrange form of kj_contract:
+//   g_mb a^b = F_m = 1/2 d_m g(v, v) - d_a g_mb v^a v^b   (a, m over r and theta).
+// With e = half the derivatives (e1 along r, e2 along theta) and A_m = e1_mb v^b,
+// B_m = e2_mb v^b:
+//   -F_t / 2 = A_t v^r + B_t v^th,   -F_ph / 2 = A_ph v^r + B_ph v^th,
+//   F_r  = A_t v^t + A_ph v^ph + (e1_thth v^th - 2 e2_rr v^r) v^th - e1_rr (v^r)^2,
+//   F_th = B_t v^t + B_ph v^ph + (e2_rr v^r - 2 e1_thth v^th) v^r - e2_thth (v^th)^2,
+// where the t and phi rows factor:
+//   A_t = q / Sigma^2 (v^t - a s^2 v^ph),   A_ph = s^2 (r v^ph - a A_t),
+//   B_t = k (a v^t - (r^2 + a^2) v^ph),     k = 2 r a s c / Sigma^2,
+//   B_ph = s c (r^2 + a^2) v^ph + k (a s^2 (2 Sigma + a^2 s^2) v^ph - (r^2 + a^2) v^t),
+// (s = sin theta, c = cos theta; e1_thth = r, e2_thth = -a^2 s c, e2_rr = -a^2 s c / Delta,
+// e1_rr = (r Delta - Sigma (r - 1)) / Delta^2), and a = g^uu F with the factor -2 of the t
+// and phi rows folded into w0 = -2 g^tt, w1 = -2 g^tph, w4 = -2 g^phph.  With RLogScale the
 // radial coordinate is u = log r: v^r = r v^u and a^u = a^r / r - (v^u)^2.
 // RL: 0 or 1 fixes RLogScale at compile time (the step then carries no select for the
 // other case), -1 reads it from R.
@@ -650,49 +636,45 @@ FM_HD_INLINE void rhs_kerr_closed_sc(const RunConst &R, const T *p, const T *v, 
         r = static_cast<S>(p[1]);
         vr = static_cast<S>(v[1]);
     }
+    const S vt = static_cast<S>(v[0]), vh = static_cast<S>(v[2]), vp = static_cast<S>(v[3]);
     const S r2 = r * r, s2 = s * s, sc = s * c;
     const S rpa = r2 + a2;
     const S Sig = r2 + a2 * (c * c);
-    const S Del = r2 - S(2) * r + a2;
+    const S Del = rpa - S(2) * r;
     // one reciprocal for 1/Sigma, 1/Delta and 1/sin^2
     const S SD = Sig * Del;
     const S inv = srcp(SD * s2);
     const S iSD = s2 * inv;
     const S iS = Del * iSD, iD = Sig * iSD, is2 = SD * inv;
     const S iS2 = iS * iS;
-    const S q = Sig - S(2) * r2;
-    const S qS = q * iS2;
-    const S a2sc = a2 * sc, a2s2 = a2 * s2, scr = sc * rpa;
-    // e1, e2: half the derivatives along r and theta of g_tt, g_tph, g_rr, g_thth, g_phph
-    // (every one carries a factor 2, which the contraction below takes back)
-    S e1[5], e2[5];
-    e1[0] = qS;
-    e2[0] = ((r + r) * iS2) * a2sc;                         // 2 r a^2 sin cos / Sigma^2
-    e1[1] = -(a * s2) * qS;
-    e2[1] = -(((r + r) * iS2) * a) * scr;
-    e1[2] = (r * Del - Sig * (r - S(1))) * (iD * iD);
-    e2[2] = -a2sc * iD;
-    e1[3] = r;
-    e2[3] = -a2sc;
-    e1[4] = s2 * (r + a2s2 * qS);
-    e2[4] = scr + (e2[0] * s2) * (S(2) * Sig + a2s2);
-    // g^uu with the factor -2 of the t and phi rows of the contraction folded into
-    // w0 = -2 g^tt, w1 = -2 g^tph, w4 = -2 g^phph
+    const S qS = (Sig - S(2) * r2) * iS2;
+    const S as2 = a * s2, a2s2 = a2 * s2, a2sc = a2 * sc;
+    // the t and phi rows
+    const S At = qS * (vt - as2 * vp);
+    const S Ap = s2 * (r * vp - a * At);
+    const S k = ((r + r) * iS2) * (a * sc);
+    const S Bt = k * (a * vt - rpa * vp);
+    const S Bp = (sc * rpa) * vp + k * ((as2 * (S(2) * Sig + a2s2)) * vp - rpa * vt);
+    // the r and theta rows
+    const S e1rr = (r * Del - Sig * (r - S(1))) * (iD * iD);
+    const S P = r * vh, Q = -(a2sc * iD) * vr;
+    const S Ft = At * vr + Bt * vh;
+    const S Fp = Ap * vr + Bp * vh;
+    const S Fr = At * vt + Ap * vp + (P - S(2) * Q) * vh - e1rr * (vr * vr);
+    const S Fh = Bt * vt + Bp * vp + (Q - S(2) * P) * vr + a2sc * (vh * vh);
     const S iSD2 = iSD + iSD;
     const S w0 = (rpa * rpa - a2s2 * Del) * iSD2;
     const S w1 = ((a + a) * r) * iSD2;
     const S w4 = (a2s2 - Del) * (iSD2 * is2);
-    const S vv[4] = {static_cast<S>(v[0]), vr, static_cast<S>(v[2]), static_cast<S>(v[3])};
-    S o[4];
-    kerr_contract<S>(e1, e2, w0, w1, Del * iS, iS, w4, vv, o);
-    out[0] = static_cast<T>(o[0]);
-    out[2] = static_cast<T>(o[2]);
-    out[3] = static_cast<T>(o[3]);
+    out[0] = static_cast<T>(w0 * Ft + w1 * Fp);
+    out[2] = static_cast<T>(iS * Fh);
+    out[3] = static_cast<T>(w1 * Ft + w4 * Fp);
+    const S o1 = (Del * iS) * Fr;
     if (rlog) {
         const T vu = v[1];
-        out[1] = static_cast<T>(o[1] * srcp(r)) - vu * vu;
+        out[1] = static_cast<T>(o1 * srcp(r)) - vu * vu;
     } else {
-        out[1] = static_cast<T>(o[1]);
+        out[1] = static_cast<T>(o1);
     }
 }
 

@@ -82,6 +82,29 @@ def test_analytic_persistent_equals_oneshot(name, precision):
         assert np.array_equal(a[k].view(np.uint64), b[k].view(np.uint64)), k
 
 
+@pytest.mark.parametrize("opts", [dict(lanes_per_cu=4, steps_per_launch=50),
+                                  dict(lanes_per_cu=8, refill_lanes=64, refill_interval=8),
+                                  dict(lanes_per_cu=4, refill_interval=5, launch_us=300)],
+                         ids=["budget50", "refill64_interval8", "time_slices"])
+@pytest.mark.parametrize("name,side", [("kerr099_rlog", 64), ("test_cfg", 40)])
+def test_mixed_persistent_schedules_equal_oneshot(name, side, opts):
+    """The mixed-precision persistent kernel over several workgroups with refills (1024 or 2048
+    resident lanes for 1600 or 4096 pixels), launch budgets, refill settings and time slices:
+    every result bit for bit the one-shot kernel's."""
+    mk, _ = CASES[name]
+    F = _gpu()
+    kw = dict(christoffel=A.CHRISTOFFEL_ANALYTIC, precision=A.PRECISION_MIXED)
+    with F.Context(mk(**kw)) as ctx:
+        a = ctx.trace_grid(side)
+    print(f"[analytic] mixed persistent {name} {opts}", flush=True)
+    with F.Context(mk(schedule=A.SCHEDULE_PERSISTENT, **opts, **kw)) as ctx:
+        b = ctx.trace_grid(side)
+    for k in ("term", "quadrant", "equat_passes", "steps"):
+        assert np.array_equal(a[k], b[k]), k
+    for k in ("final_pos", "final_vel", "lambda", "closest_radius"):
+        assert np.array_equal(a[k].view(np.uint64), b[k].view(np.uint64)), k
+
+
 @pytest.mark.parametrize("path", ["fd", "analytic", "mixed"])
 def test_kerr_constants_of_motion_on_every_path(path):
     """E = -p_t, L = p_phi, Carter's Q and the null norm at the end of escaping Kerr geodesics
