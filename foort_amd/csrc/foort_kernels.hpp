@@ -18,15 +18,15 @@
 #ifndef FOORT_WPE_PERSIST
 #define FOORT_WPE_PERSIST 2
 #endif
-// One-shot kernels of the closed-form formulations: more waves per SIMD hide the latency of
-// their dependent chains better than the spills they cause cost.  Measured on MI355X on a
-// 1024x1024 Kerr image: binary64 3.3% faster at three waves (slower at four), mixed 8.6%
-// faster at four.
+// One-shot kernels of the closed-form formulations: three waves per SIMD hide the latency of
+// their dependent chains better than two.  Measured on MI355X on a 1024x1024 Kerr image:
+// binary64 3.3% faster at three waves (slower at four); mixed 1.5% faster at three than at
+// four (where it spilled 100 B/lane to scratch memory; none at three).
 #ifndef FOORT_WPE_ONESHOT_AN64
 #define FOORT_WPE_ONESHOT_AN64 3
 #endif
 #ifndef FOORT_WPE_ONESHOT_MIXED
-#define FOORT_WPE_ONESHOT_MIXED 4
+#define FOORT_WPE_ONESHOT_MIXED 3
 #endif
 // Resident kernels of the mixed formulation: three waves per SIMD (a few VGPRs spilled).
 // Measured on MI355X on a 1024x1024 Kerr image: the persistent kernel 3% faster than at two
