@@ -194,6 +194,23 @@ __device__ __forceinline__ uint32_t park_index(uint32_t l)
     return l;
 }
 
+// The number of lanes below this one whose bit is set in a wavefront mask (v_mbcnt: no
+// per-lane mask kept across the resident kernels' loops).
+__device__ __forceinline__ uint32_t lanes_below(unsigned long long mask)
+{
+    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+}
+
+// A uniform value the compiler must take as new where it is used: the resident kernels' refills
+// divide pixel indices by the grid side, and the division's set-up, hoisted out of the step
+// loop, held registers across it that went to scratch memory.
+__device__ __forceinline__ uint64_t opaque_u64(uint64_t x)
+{
+    asm volatile("" : "+s"(x));
+    return x;
+}
+
 __device__ __forceinline__ void add_steps(unsigned long long *total, unsigned long long mine)
 {
 #pragma unroll
@@ -342,11 +359,11 @@ trace_persistent(const RunConst R, uint64_t side, uint64_t first,
                 base = atomicAdd(S.next, static_cast<unsigned long long>(__popcll(need)));
             base = __shfl(base, leader);
             if (pix < 0 && !exhausted) {
-                const unsigned long long k = base + __popcll(need & ((1ull << lane_id) - 1ull));
+                const unsigned long long k = base + lanes_below(need);
                 if (k < count) {
                     pix = static_cast<long long>(k);
                     double ux, uy;
-                    pixel_unit(side, first, si, k, ux, uy);
+                    pixel_unit(opaque_u64(side), first, si, k, ux, uy);
                     geo_init<EVERY>(refill_consts(S.rc), ux, uy, g);
                 } else {
                     exhausted = true;
@@ -423,7 +440,7 @@ trace_queue(const RunConst R, const QueueArgs Q, DevResults out, unsigned long l
                 base = atomicAdd(S.next, static_cast<unsigned long long>(__popcll(need)));
             base = __shfl(base, leader);
             if (pix < 0 && !exhausted) {
-                const unsigned long long k = base + __popcll(need & ((1ull << lane_id) - 1ull));
+                const unsigned long long k = base + lanes_below(need);
                 if (k < Q.count) {
                     pix = static_cast<long long>(k);
                     const uint64_t at = k & Q.mask;
