@@ -1066,6 +1066,14 @@ void oracle_fm_eval(const double *x, uint64_t n, int which, double *out1, double
     }
 }
 
+/* sin and cos at x[i], xp[i], xm[i] by fm_sincos3 (out: 6 per i), for its tests */
+void oracle_fm_stencil(const double *x, const double *xp, const double *xm, uint64_t n, double *out)
+{
+    for (uint64_t i = 0; i < n; ++i)
+        fm_sincos3(x[i], xp[i], xm[i], &out[6 * i], &out[6 * i + 1], &out[6 * i + 2], &out[6 * i + 3],
+                   &out[6 * i + 4], &out[6 * i + 5]);
+}
+
 /* sin and cos at y[i] shifted from x[i] (fm_sincos_near), for the accuracy tests */
 void oracle_fm_near(const double *x, const double *y, uint64_t n, double *s, double *c)
 {

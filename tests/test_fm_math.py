@@ -123,3 +123,29 @@ def test_sincos_near_agrees_with_direct_and_falls_back():
     assert (s == s2).all() and (c == c2).all()
     s, c = _near(np.array([np.nan, 1.0]), np.array([np.nan, np.nan]))
     assert np.isnan(s).all() and np.isnan(c).all()
+
+
+def test_sincos3_is_direct_at_x_and_near_at_each_stencil_point():
+    """fm_sincos3 gives fm_sincos at x and, at each of xp and xm, what fm_sincos_near gives for
+    that point alone -- also where one point shifts and the other is evaluated in full (h just
+    above 2^-12 with x just below 1, where x + h rounds in the next binade), and at the pole
+    threshold |x| = 2^-11."""
+    h = 2.0 ** -12 + 2.0 ** -53
+    x = np.concatenate([RNG.uniform(0.3, 3.0, 2000), 1.0 - RNG.uniform(0, 2 * h, 2000),
+                        RNG.uniform(2.0 ** -12, 2.0 ** -10, 500), [2.0 ** -11, np.nextafter(2.0 ** -11, 0)]])
+    for hh in (1e-7, h, 1e-3):
+        xp, xm = x + hh, x - hh
+        L = O.lib("fm")
+        out = np.zeros(6 * len(x))
+        L.oracle_fm_stencil.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]
+        L.oracle_fm_stencil(x.ctypes.data_as(C.c_void_p), xp.ctypes.data_as(C.c_void_p), xm.ctypes.data_as(C.c_void_p),
+                            len(x), out.ctypes.data_as(C.c_void_p))
+        out = out.reshape(-1, 6)
+        s0, c0 = _eval(0, x)
+        sp, cp = _near(x, xp)
+        sm, cm = _near(x, xm)
+        for got, want in ((out[:, 0], s0), (out[:, 1], c0), (out[:, 2], sp), (out[:, 3], cp), (out[:, 4], sm), (out[:, 5], cm)):
+            assert np.array_equal(got.view(np.uint64), want.view(np.uint64))
+    # the mixed case occurs: one point within 2^-12 of x, the other not
+    dp, dm = np.abs((x + h) - x), np.abs((x - h) - x)
+    assert np.any((dp <= 2.0 ** -12) != (dm <= 2.0 ** -12))

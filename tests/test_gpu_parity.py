@@ -318,3 +318,34 @@ def test_initial_velocity_warnings_and_nonfinite_results_counted(schedule):
     with F.Context(NAN_CASE(max_steps=3000, schedule=schedule)) as ctx:
         n = ctx.trace_grid(40)
     assert (n["stats"].nonfinite, n["stats"].future_pointing, n["stats"].outward_pointing) == (1, 0, 0)
+
+
+THETA_STENCIL_CASES = {
+    # every theta +- h evaluated in full (|d| > 2^-12)
+    "derivative_h_1e-3": (lambda: c0(params=(0.7,), rlog=True, inclination_deg=60.0, max_steps=20000,
+                                     derivative_h=1e-3), 10),
+    # h = 2^-12 + 2^-53: for theta in [1, 2) the rounding of theta +- h is a tie, and |d| lands on
+    # 2^-12 or one ulp above it with the last bit of theta -- both choices within fm_sincos3 and
+    # from one lane to the next; below 1 every |d| is h, above the threshold
+    "derivative_h_near_2^-12": (lambda: c0(params=(0.7,), rlog=True, inclination_deg=60.0, max_steps=20000,
+                                           derivative_h=2.0 ** -12 + 2.0 ** -53), 10),
+    # camera 3e-4 rad from the pole: theta below 2^-11, the stencil evaluated in full there
+    "camera_near_pole": (lambda: c0(params=(0.5,), rlog=True, theta=3e-4, max_steps=20000), 10),
+}
+
+
+@pytest.mark.parametrize("name", sorted(THETA_STENCIL_CASES))
+def test_theta_stencil_fallbacks_bitexact(name):
+    """sin and cos at theta +- h come from theta's reduced sine and cosine where |d| <= 2^-12 and
+    |theta| >= 2^-11, and from a full evaluation elsewhere (fm_sincos3): both branches, and
+    lanes taking different ones, bit for bit the oracle's."""
+    mk, side = THETA_STENCIL_CASES[name]
+    desc = mk()
+    print(f"[parity] theta stencil case={name} side={side}", flush=True)
+    F = _gpu()
+    with F.Context(desc) as ctx:
+        g = ctx.trace_grid(side)
+    o = O.trace("fm", desc, side)
+    bad = [k for k in FIELDS if not bitwise_equal(g[k], o[k])]
+    assert not bad, (name, bad)
+    assert g["stats"].steps == int(o["steps"].sum())
