@@ -376,8 +376,8 @@ def main():
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-server", action="store_true", help="c1: skip the render-server (C4) leg")
     ap.add_argument("--tail-geodesics", type=int, default=None,
-                    help="c2: the GPU group's TailGeodesics (unfinished geodesics handed to the "
-                         "four-lanes-per-geodesic kernel; 4294967295 = never)")
+                    help="c1, c2: unfinished geodesics handed to the four-lanes-per-geodesic "
+                         "kernel (finite differences, persistent schedule; 4294967295 = never)")
     ap.add_argument("--c2-variant", default=None,
                     help="c2: run only the named variant (fd_tuned_graph, fd_oneshot, fd_persistent, ...)")
     ap.add_argument("--christoffel", choices=["fd", "analytic"], default="fd",
@@ -417,6 +417,8 @@ def main():
                 schedule=A.SCHEDULE_ONESHOT if args.schedule == "oneshot" else A.SCHEDULE_PERSISTENT,
                 steps_per_launch=args.steps_per_launch, block_size=args.block, refill_lanes=args.refill,
                 refill_interval=args.interval, launch_us=args.launch_us)
+    if args.tail_geodesics is not None:
+        path["tail_geodesics"] = args.tail_geodesics
     desc = W.c1(**path)
     ref_desc = W.c1()
     res = F.device_results(n)
