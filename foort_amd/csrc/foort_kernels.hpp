@@ -242,14 +242,14 @@ trace_oneshot(const RunConst R, uint64_t side, uint64_t first, uint64_t count,
         if constexpr (RHS::exact) {
             do {
                 for (uint32_t k = 0; k < kNanTestSteps && g.term == FOORT_TERM_CONTINUE; ++k)
-                    geo_update<RHS, INTEG, EVERY>(R, g);
+                    geo_update<RHS, INTEG, EVERY, true>(R, g);
                 term_after_run(g.term);
                 geo_skip_nan<EVERY>(R, g);
             } while (__any(g.term == FOORT_TERM_CONTINUE));
         } else {
             for (;;) {
                 for (uint32_t k = 0; k < kNanTestSteps && g.term == FOORT_TERM_CONTINUE; ++k)
-                    geo_update<RHS, INTEG, EVERY>(R, g);
+                    geo_update<RHS, INTEG, EVERY, true>(R, g);
                 term_after_run(g.term);
                 geo_skip_nan<EVERY>(R, g);
                 if (g.term != FOORT_TERM_CONTINUE)
