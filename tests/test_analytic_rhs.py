@@ -126,3 +126,23 @@ def test_stage_sincos_shift_within_a_few_ulp():
         scale = np.spacing(np.maximum(np.abs(ref), np.abs(start)))
         err = np.abs(got - ref) / scale
         assert err.max() <= 4.0, (err.max(), th0[err.argmax()], d[err.argmax()])
+
+
+@pytest.mark.parametrize("a", [0.0, 0.5, 0.9, 0.998])
+def test_closed_form_kerr_near_horizon_and_poles(a):
+    """The closed-form Kerr right-hand side (factored t and phi rows, r and theta rows contracted
+    directly) against the dual-number derivative of the reference formula where its terms are
+    largest: r within 1e-4..1 of the outer horizon and theta within 1e-5..0.1 of a pole."""
+    import math
+    L = C.CDLL(LIB)
+    d = A.make_desc(A.METRIC_KERR, (a,))
+    rh = 1.0 + math.sqrt(1.0 - a * a)
+    rng = np.random.default_rng(17)
+    for _ in range(300):
+        r = rh * (1.0 + 10.0 ** rng.uniform(-4, 0)) if rng.random() < 0.5 else rng.uniform(2.0, 1000.0)
+        pole = 10.0 ** rng.uniform(-5, -1)
+        th = [pole, math.pi - pole, rng.uniform(0.1, 3.0)][int(rng.integers(3))]
+        p = np.array([0.0, r, th, 0.0])
+        v = rng.normal(size=4)
+        dual, closed = _rhs(L, d, 1, p, v), _rhs(L, d, 2, p, v)
+        assert np.abs(closed - dual).max() <= 1e-12 * np.abs(dual).max(), (r, th)
