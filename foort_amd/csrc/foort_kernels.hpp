@@ -34,6 +34,11 @@
 #ifndef FOORT_WPE_PERSIST_MIXED
 #define FOORT_WPE_PERSIST_MIXED 3
 #endif
+// Resident kernels of the closed-form binary64 formulation: three waves per SIMD (no spills).
+// Measured on MI355X on a 1024x1024 Kerr image: the persistent kernel 5.4% faster than at two.
+#ifndef FOORT_WPE_PERSIST_AN64
+#define FOORT_WPE_PERSIST_AN64 3
+#endif
 // Manko-Novikov, Rasheed-Larsen and (except its finite-difference one-shot kernel) Johannsen:
 // their potentials with derivatives need more registers than two waves per SIMD leave, and
 // spills to scratch memory stall every step, worst for a lone wave (the time-outs that end an
@@ -57,7 +62,7 @@ constexpr int waves_per_eu(bool resident)
     if (RHS::metric == FOORT_METRIC_JOHANNSEN)
         return (RHS::exact && !resident) ? FOORT_WPE_ONESHOT : FOORT_WPE_HEAVY;
     if (resident)
-        return RHS::split_rk4 ? FOORT_WPE_PERSIST_MIXED : FOORT_WPE_PERSIST;
+        return RHS::split_rk4 ? FOORT_WPE_PERSIST_MIXED : (RHS::exact ? FOORT_WPE_PERSIST : FOORT_WPE_PERSIST_AN64);
     if (RHS::exact)
         return FOORT_WPE_ONESHOT;
     return RHS::split_rk4 ? FOORT_WPE_ONESHOT_MIXED : FOORT_WPE_ONESHOT_AN64;
