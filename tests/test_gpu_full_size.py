@@ -60,11 +60,12 @@ def _boundary(img):
     return b
 
 
-def test_c1_mixed_precision_mismatch_is_small_and_on_boundaries(fd_oneshot):
+@pytest.mark.parametrize("schedule", [A.SCHEDULE_PERSISTENT, A.SCHEDULE_ONESHOT], ids=["persistent", "oneshot"])
+def test_c1_mixed_precision_mismatch_is_small_and_on_boundaries(fd_oneshot, schedule):
     import foort_amd
-    print("[full] C1 1024x1024 mixed persistent", flush=True)
+    print(f"[full] C1 1024x1024 mixed schedule={schedule}", flush=True)
     with foort_amd.Context(c1(christoffel=A.CHRISTOFFEL_ANALYTIC, precision=A.PRECISION_MIXED,
-                              schedule=A.SCHEDULE_PERSISTENT)) as ctx:
+                              schedule=schedule)) as ctx:
         m = ctx.trace_grid(SIDE)
     n = SIDE * SIDE
     off = 0
@@ -74,3 +75,19 @@ def test_c1_mixed_precision_mismatch_is_small_and_on_boundaries(fd_oneshot):
         assert diff.sum() <= 1e-3 * n, (k, int(diff.sum()))
         off += int((diff & ~_boundary(ref)).sum())
     assert off == 0
+
+
+def test_c1_mixed_oneshot_and_persistent_bit_identical():
+    """The whole 1024x1024 image in mixed precision through the one-shot kernel (flags tested
+    afresh on every step) and the persistent kernel (flags hoisted, lanes refilled): every field
+    bit for bit the same."""
+    import foort_amd
+    print("[full] C1 1024x1024 mixed one-shot vs persistent", flush=True)
+    out = []
+    for schedule in (A.SCHEDULE_ONESHOT, A.SCHEDULE_PERSISTENT):
+        with foort_amd.Context(c1(christoffel=A.CHRISTOFFEL_ANALYTIC, precision=A.PRECISION_MIXED,
+                                  schedule=schedule)) as ctx:
+            out.append(ctx.trace_grid(SIDE))
+    a, b = out
+    for k in FIELDS:
+        assert bitwise_equal(a[k], b[k]), k
