@@ -496,6 +496,12 @@ def main():
                          ("mixed_persistent", dict(christoffel=A.CHRISTOFFEL_ANALYTIC, precision=A.PRECISION_MIXED,
                                                    schedule=A.SCHEDULE_PERSISTENT))]:
             with F.Context(W.c1(**kw)) as vctx:
+                tuned = None
+                if name == "mixed_persistent":
+                    # the north-star path with the options the project's autotuner picks for this
+                    # image (steps between refills, refill threshold; its trials are not timed)
+                    chosen, _ = vctx.autotune(side, count=n)
+                    tuned = {"refill_interval": int(chosen.refill_interval), "refill_lanes": int(chosen.refill_lanes)}
                 vctx.trace_device(side, 0, n, res)  # warm-up
                 F.synchronize()
                 t = time.perf_counter()
@@ -510,6 +516,8 @@ def main():
                               "kernel_ms": st.device_ms, "launches": st.launches,
                               "hbm_bytes_per_pixel": hbm_bytes_per_pixel(c, p, s),
                               "vs_fd_oneshot": agreement(base, out)}
+            if tuned:
+                variants[name]["autotuned"] = tuned
             log(f"{name}: {n / dt:.3e} px/s")
         line["variants"] = variants
 
